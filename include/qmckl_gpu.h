@@ -1,0 +1,344 @@
+/*
+ * qmckl_gpu.h -- C ABI of the MI355X-native per-electron basis path.
+ *
+ * Drop-in for the en_distance -> ao_vgl -> mo_vgl path of the QMCkl GPU
+ * context/provider API as declared by the reference in
+ * include/qmckl_gpu.hpp (paths relative to the reference checkout).  Every
+ * entry point keeps the reference's name, argument order and meaning and
+ * its exit codes; the comment above each cites the declaration it
+ * replaces.  Differences:
+ *   - the header is plain C with C linkage (the reference's symbols are
+ *     C++-mangled), so cgo / JNI / ctypes / Fortran bind it directly;
+ *   - qmckl_context_create_device takes a HIP device ordinal instead of a
+ *     sycl::queue (the context owns one HIP stream on that device);
+ *     device_id == QMCKL_NO_DEVICE creates a context with no device bound,
+ *     on which argument validation, masks and scalar setters work and every
+ *     device operation fails with QMCKL_FAILURE_DEVICE;
+ *   - getters that the reference lacks are added where the path needs them
+ *     (qmckl_get_electron_en_distance_device, qmckl_get_error_device);
+ *   - the precision, batched and adjoint forms are new entry points at the
+ *     end of this file.
+ *
+ * All array arguments are device pointers (HIP device memory) unless stated
+ * otherwise; scalar outputs are host pointers.  Setters deep-copy their
+ * inputs; getters copy into a caller buffer after checking size_max and
+ * return with the data complete (the call synchronises the context's
+ * stream).  One context is not safe for concurrent compute from several
+ * threads (as in the reference, the mutex guards the memory table and the
+ * error record only).
+ */
+#ifndef QMCKL_GPU_H
+#define QMCKL_GPU_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The library is built with hidden visibility; these are its exports. */
+#if defined(__GNUC__)
+#pragma GCC visibility push(default)
+#endif
+
+/* ---- types and codes (include/qmckl_types.hpp:12-116) ------------------ */
+
+typedef int32_t qmckl_exit_code_device;
+typedef int64_t qmckl_context_device;
+
+#define QMCKL_NULL_CONTEXT_DEVICE ((qmckl_context_device)0)
+#define QMCKL_NO_DEVICE (-1)
+
+#define QMCKL_SUCCESS_DEVICE ((qmckl_exit_code_device)0)
+#define QMCKL_INVALID_ARG_1_DEVICE ((qmckl_exit_code_device)1)
+#define QMCKL_INVALID_ARG_2_DEVICE ((qmckl_exit_code_device)2)
+#define QMCKL_INVALID_ARG_3_DEVICE ((qmckl_exit_code_device)3)
+#define QMCKL_INVALID_ARG_4_DEVICE ((qmckl_exit_code_device)4)
+#define QMCKL_INVALID_ARG_5_DEVICE ((qmckl_exit_code_device)5)
+#define QMCKL_INVALID_ARG_6_DEVICE ((qmckl_exit_code_device)6)
+#define QMCKL_INVALID_ARG_7_DEVICE ((qmckl_exit_code_device)7)
+#define QMCKL_INVALID_ARG_8_DEVICE ((qmckl_exit_code_device)8)
+#define QMCKL_INVALID_ARG_9_DEVICE ((qmckl_exit_code_device)9)
+#define QMCKL_INVALID_ARG_10_DEVICE ((qmckl_exit_code_device)10)
+#define QMCKL_INVALID_ARG_11_DEVICE ((qmckl_exit_code_device)11)
+#define QMCKL_INVALID_ARG_12_DEVICE ((qmckl_exit_code_device)12)
+#define QMCKL_INVALID_ARG_13_DEVICE ((qmckl_exit_code_device)13)
+#define QMCKL_INVALID_ARG_14_DEVICE ((qmckl_exit_code_device)14)
+#define QMCKL_INVALID_ARG_15_DEVICE ((qmckl_exit_code_device)15)
+#define QMCKL_INVALID_ARG_16_DEVICE ((qmckl_exit_code_device)16)
+#define QMCKL_INVALID_ARG_17_DEVICE ((qmckl_exit_code_device)17)
+#define QMCKL_INVALID_ARG_18_DEVICE ((qmckl_exit_code_device)18)
+#define QMCKL_INVALID_ARG_19_DEVICE ((qmckl_exit_code_device)19)
+#define QMCKL_INVALID_ARG_20_DEVICE ((qmckl_exit_code_device)20)
+#define QMCKL_FAILURE_DEVICE ((qmckl_exit_code_device)101)
+#define QMCKL_ERRNO_DEVICE ((qmckl_exit_code_device)102)
+#define QMCKL_INVALID_CONTEXT_DEVICE ((qmckl_exit_code_device)103)
+#define QMCKL_ALLOCATION_FAILED_DEVICE ((qmckl_exit_code_device)104)
+#define QMCKL_DEALLOCATION_FAILED_DEVICE ((qmckl_exit_code_device)105)
+#define QMCKL_NOT_PROVIDED_DEVICE ((qmckl_exit_code_device)106)
+#define QMCKL_OUT_OF_BOUNDS_DEVICE ((qmckl_exit_code_device)107)
+#define QMCKL_ALREADY_SET_DEVICE ((qmckl_exit_code_device)108)
+#define QMCKL_INVALID_EXIT_CODE_DEVICE ((qmckl_exit_code_device)109)
+
+#define QMCKL_MAX_FUN_LEN_DEVICE 256
+#define QMCKL_MAX_MSG_LEN_DEVICE 1024
+
+/* include/qmckl_types.hpp:100-103 */
+typedef struct qmckl_memory_info_struct_device {
+  size_t size;
+  void *pointer;
+} qmckl_memory_info_struct_device;
+
+/* ---- error (include/qmckl_gpu.hpp:384-387; src/qmckl_basic_functions.cpp:42-79) */
+
+qmckl_exit_code_device qmckl_failwith_device(qmckl_context_device context,
+                                             qmckl_exit_code_device exit_code,
+                                             const char *function, const char *message);
+/* src/qmckl_basic_functions.cpp:42-48 (returns a real description here). */
+const char *qmckl_string_of_error_device(qmckl_exit_code_device error);
+/* New: read back the last error recorded by qmckl_failwith_device.  Any
+ * output pointer may be NULL; function_name must hold
+ * QMCKL_MAX_FUN_LEN_DEVICE chars and message QMCKL_MAX_MSG_LEN_DEVICE. */
+qmckl_exit_code_device qmckl_get_error_device(qmckl_context_device context,
+                                              qmckl_exit_code_device *exit_code,
+                                              char *function_name, char *message);
+
+/* ---- context (include/qmckl_gpu.hpp:473-497) ----------------------------- */
+
+/* qmckl_gpu.hpp:483 qmckl_context_create_device(sycl::queue) */
+qmckl_context_device qmckl_context_create_device(int device_id);
+/* qmckl_gpu.hpp:484-485 */
+qmckl_exit_code_device qmckl_context_destroy_device(qmckl_context_device context);
+/* qmckl_gpu.hpp:473-474 */
+qmckl_exit_code_device qmckl_context_touch_device(qmckl_context_device context);
+/* qmckl_gpu.hpp:493-494 */
+qmckl_context_device qmckl_context_check_device(qmckl_context_device context);
+/* qmckl_gpu.hpp:496-497 */
+void qmckl_lock_device(qmckl_context_device context);
+void qmckl_unlock_device(qmckl_context_device context);
+/* qmckl_gpu.hpp:487-491 (static inline in the reference) */
+int64_t qmckl_get_device_id(qmckl_context_device context);
+/* qmckl_gpu.hpp:476-481 */
+qmckl_exit_code_device qmckl_init_point_device(qmckl_context_device context);
+qmckl_exit_code_device qmckl_init_ao_basis_device(qmckl_context_device context);
+qmckl_exit_code_device qmckl_init_mo_basis_device(qmckl_context_device context);
+
+/* ---- memory (include/qmckl_gpu.hpp:397-423) ---------------------------- */
+
+/* qmckl_gpu.hpp:401-402 */
+void *qmckl_malloc_host(qmckl_context_device context, qmckl_memory_info_struct_device info);
+/* qmckl_gpu.hpp:404 */
+void *qmckl_malloc_device(qmckl_context_device context, size_t size);
+/* qmckl_gpu.hpp:406-410 */
+qmckl_exit_code_device qmckl_free_host(qmckl_context_device context, void *ptr);
+qmckl_exit_code_device qmckl_free_device(qmckl_context_device context, void *ptr);
+/* qmckl_gpu.hpp:414-423 */
+qmckl_exit_code_device qmckl_memcpy_H2D(qmckl_context_device context, void *dest,
+                                        void *src, size_t size);
+qmckl_exit_code_device qmckl_memcpy_D2H(qmckl_context_device context, void *dest,
+                                        void *src, size_t size);
+qmckl_exit_code_device qmckl_memcpy_D2D(qmckl_context_device context, void *dest,
+                                        void *src, size_t size);
+
+/* ---- point (include/qmckl_gpu.hpp:513-519) ------------------------------- */
+
+qmckl_exit_code_device qmckl_set_point_device(qmckl_context_device context, char transp,
+                                              int64_t num, double *coord, int64_t size_max);
+qmckl_exit_code_device qmckl_get_point_device(qmckl_context_device context, char transp,
+                                              double *coord, int64_t size_max);
+
+/* ---- nucleus (include/qmckl_gpu.hpp:525-556) ----------------------------- */
+
+bool qmckl_nucleus_provided_device(qmckl_context_device context);
+qmckl_exit_code_device qmckl_get_nucleus_num_device(qmckl_context_device context, int64_t *num);
+qmckl_exit_code_device qmckl_set_nucleus_num_device(qmckl_context_device context, int64_t num);
+qmckl_exit_code_device qmckl_set_nucleus_charge_device(qmckl_context_device context,
+                                                       double *charge, int64_t size_max);
+qmckl_exit_code_device qmckl_set_nucleus_coord_device(qmckl_context_device context, char transp,
+                                                      double *coord, int64_t size_max);
+qmckl_exit_code_device qmckl_get_nucleus_coord_device(qmckl_context_device context, char transp,
+                                                      double *coord, int64_t size_max);
+qmckl_exit_code_device qmckl_get_nucleus_charge_device(qmckl_context_device context,
+                                                       double *charge, int64_t size_max);
+
+/* ---- electron (include/qmckl_gpu.hpp:562-580; include/qmckl_electron.hpp) */
+
+bool qmckl_electron_provided_device(qmckl_context_device context);
+qmckl_exit_code_device qmckl_set_electron_num_device(qmckl_context_device context,
+                                                     int64_t up_num, int64_t down_num);
+qmckl_exit_code_device qmckl_get_electron_num_device(qmckl_context_device context, int64_t *num);
+qmckl_exit_code_device qmckl_set_electron_coord_device(qmckl_context_device context, char transp,
+                                                       int64_t walk_num, double *coord,
+                                                       int64_t size_max);
+qmckl_exit_code_device qmckl_get_electron_coord_device(qmckl_context_device context, char transp,
+                                                       double *coord, int64_t size_max);
+
+/* ---- electron-nucleus distances ------------------------------------------ */
+
+/* qmckl_gpu.hpp:1045-1046 */
+qmckl_exit_code_device qmckl_provide_en_distance_device(qmckl_context_device context);
+/* include/qmckl_electron.hpp:30-32 (src/qmckl_electron.cpp:302-331).
+ * elec_coord [3][point_num], nucl_coord [3][nucl_num], en_distance
+ * [point_num][nucl_num]. */
+qmckl_exit_code_device qmckl_compute_en_distance_device(qmckl_context_device context,
+                                                        int64_t point_num, int64_t nucl_num,
+                                                        const double *elec_coord,
+                                                        const double *nucl_coord,
+                                                        double *en_distance);
+/* New getter (the reference has none): en_distance [P][N], size_max >= P*N. */
+qmckl_exit_code_device qmckl_get_electron_en_distance_device(qmckl_context_device context,
+                                                             double *en_distance,
+                                                             int64_t size_max);
+
+/* ---- AO basis (include/qmckl_gpu.hpp:586-752) ---------------------------- */
+
+bool qmckl_ao_basis_provided_device(qmckl_context_device context);
+
+/* setters, qmckl_gpu.hpp:664-697 */
+qmckl_exit_code_device qmckl_set_ao_basis_type_device(qmckl_context_device context, char basis_type);
+qmckl_exit_code_device qmckl_set_ao_basis_shell_num_device(qmckl_context_device context, int64_t shell_num);
+qmckl_exit_code_device qmckl_set_ao_basis_prim_num_device(qmckl_context_device context, int64_t prim_num);
+qmckl_exit_code_device qmckl_set_ao_basis_ao_num_device(qmckl_context_device context, int64_t ao_num);
+qmckl_exit_code_device qmckl_set_ao_basis_nucleus_index_device(qmckl_context_device context,
+                                                               int64_t *nucleus_index, int64_t size_max);
+qmckl_exit_code_device qmckl_set_ao_basis_nucleus_shell_num_device(qmckl_context_device context,
+                                                                   int64_t *nucleus_shell_num,
+                                                                   int64_t size_max);
+qmckl_exit_code_device qmckl_set_ao_basis_shell_ang_mom_device(qmckl_context_device context,
+                                                               int32_t *shell_ang_mom, int64_t size_max);
+qmckl_exit_code_device qmckl_set_ao_basis_shell_prim_num_device(qmckl_context_device context,
+                                                                int64_t *shell_prim_num, int64_t size_max);
+qmckl_exit_code_device qmckl_set_ao_basis_shell_prim_index_device(qmckl_context_device context,
+                                                                  int64_t *shell_prim_index,
+                                                                  int64_t size_max);
+qmckl_exit_code_device qmckl_set_ao_basis_shell_factor_device(qmckl_context_device context,
+                                                              double *shell_factor, int64_t size_max);
+qmckl_exit_code_device qmckl_set_ao_basis_exponent_device(qmckl_context_device context,
+                                                          double *exponent, int64_t size_max);
+qmckl_exit_code_device qmckl_set_ao_basis_coefficient_device(qmckl_context_device context,
+                                                             double *coefficient, int64_t size_max);
+qmckl_exit_code_device qmckl_set_ao_basis_prim_factor_device(qmckl_context_device context,
+                                                             double *prim_factor, int64_t size_max);
+qmckl_exit_code_device qmckl_set_ao_basis_ao_factor_device(qmckl_context_device context,
+                                                           double *ao_factor, int64_t size_max);
+
+/* getters, qmckl_gpu.hpp:641-652 and 702-746 */
+qmckl_exit_code_device qmckl_get_ao_basis_type_device(qmckl_context_device context, char *basis_type);
+qmckl_exit_code_device qmckl_get_ao_basis_shell_num_device(qmckl_context_device context, int64_t *shell_num);
+qmckl_exit_code_device qmckl_get_ao_basis_prim_num_device(qmckl_context_device context, int64_t *prim_num);
+qmckl_exit_code_device qmckl_get_ao_basis_ao_num_device(qmckl_context_device context, int64_t *ao_num);
+qmckl_exit_code_device qmckl_get_ao_basis_nucleus_index_device(qmckl_context_device context,
+                                                               int64_t *nucleus_index, int64_t size_max);
+qmckl_exit_code_device qmckl_get_ao_basis_nucleus_shell_num_device(qmckl_context_device context,
+                                                                   int64_t *nucleus_shell_num,
+                                                                   int64_t size_max);
+qmckl_exit_code_device qmckl_get_ao_basis_shell_ang_mom_device(qmckl_context_device context,
+                                                               int32_t *shell_ang_mom, int64_t size_max);
+qmckl_exit_code_device qmckl_get_ao_basis_shell_prim_num_device(qmckl_context_device context,
+                                                                int64_t *shell_prim_num, int64_t size_max);
+qmckl_exit_code_device qmckl_get_ao_basis_shell_prim_index_device(qmckl_context_device context,
+                                                                  int64_t *shell_prim_index,
+                                                                  int64_t size_max);
+qmckl_exit_code_device qmckl_get_ao_basis_shell_factor_device(qmckl_context_device context,
+                                                              double *shell_factor, int64_t size_max);
+qmckl_exit_code_device qmckl_get_ao_basis_exponent_device(qmckl_context_device context,
+                                                          double *exponent, int64_t size_max);
+qmckl_exit_code_device qmckl_get_ao_basis_coefficient_device(qmckl_context_device context,
+                                                             double *coefficient, int64_t size_max);
+qmckl_exit_code_device qmckl_get_ao_basis_prim_factor_device(qmckl_context_device context,
+                                                             double *prim_factor, int64_t size_max);
+qmckl_exit_code_device qmckl_get_ao_basis_ao_factor_device(qmckl_context_device context,
+                                                           double *ao_factor, int64_t size_max);
+
+/* finalize, qmckl_gpu.hpp:658-659 (src/qmckl_ao_sycl.cpp:1067-1236) */
+qmckl_exit_code_device qmckl_finalize_ao_basis_device(qmckl_context_device context);
+
+/* providers and getters of the computed arrays, qmckl_gpu.hpp:620-639, 748-750 */
+qmckl_exit_code_device qmckl_provide_ao_basis_shell_vgl_device(qmckl_context_device context);
+qmckl_exit_code_device qmckl_provide_ao_basis_ao_vgl_device(qmckl_context_device context);
+qmckl_exit_code_device qmckl_provide_ao_basis_ao_value_device(qmckl_context_device context);
+/* src/qmckl_ao.cpp:207-239: shell_vgl [P][5][S] */
+qmckl_exit_code_device qmckl_get_ao_basis_shell_vgl_device(qmckl_context_device context,
+                                                           double *shell_vgl, int64_t size_max);
+/* ao_vgl [P][5][A], size_max >= 5*A*P */
+qmckl_exit_code_device qmckl_get_ao_basis_ao_vgl_device(qmckl_context_device context,
+                                                        double *ao_vgl, int64_t size_max);
+qmckl_exit_code_device qmckl_get_ao_basis_ao_vgl_inplace_device(qmckl_context_device context,
+                                                                double *ao_vgl, int64_t size_max);
+/* ao_value [P][A], size_max >= A*P */
+qmckl_exit_code_device qmckl_get_ao_basis_ao_value_device(qmckl_context_device context,
+                                                          double *ao_value, int64_t size_max);
+qmckl_exit_code_device qmckl_get_ao_basis_ao_value_inplace_device(qmckl_context_device context,
+                                                                  double *ao_value, int64_t size_max);
+
+/* low-level launchers, qmckl_gpu.hpp:586-613 (src/qmckl_ao_sycl.cpp:12-102,
+ * 106-434).  coord [3][point_num], nucl_coord [3][nucl_num];
+ * coef_normalized and nucleus_range as produced by the finalize step.
+ * compute_ao_vgl_gaussian recomputes the radial parts itself and ignores
+ * the shell_vgl argument's contents (it may be NULL). */
+qmckl_exit_code_device qmckl_compute_ao_basis_shell_gaussian_vgl_device(
+    qmckl_context_device context, int prim_num, int shell_num, int point_num, int nucl_num,
+    int64_t *nucleus_shell_num, int64_t *nucleus_index, double *nucleus_range,
+    int64_t *shell_prim_index, int64_t *shell_prim_num, double *coord, double *nucl_coord,
+    double *expo, double *coef_normalized, double *shell_vgl);
+qmckl_exit_code_device qmckl_compute_ao_vgl_gaussian_device(
+    qmckl_context_device context, int64_t ao_num, int64_t shell_num, int64_t point_num,
+    int64_t nucl_num, const double *coord, const double *nucl_coord,
+    const int64_t *nucleus_index, const int64_t *nucleus_shell_num,
+    const double *nucleus_range, const int32_t *nucleus_max_ang_mom,
+    const int32_t *shell_ang_mom, const double *ao_factor, double *shell_vgl, double *ao_vgl);
+
+/* ---- MO basis (include/qmckl_gpu.hpp:758-801) ---------------------------- */
+
+bool qmckl_mo_basis_provided_device(qmckl_context_device context);
+qmckl_exit_code_device qmckl_get_mo_basis_mo_num_device(qmckl_context_device context, int64_t *mo_num);
+qmckl_exit_code_device qmckl_set_mo_basis_mo_num_device(qmckl_context_device context, int64_t mo_num);
+/* coefficient [mo_num][ao_num]; the reference takes no size (qmckl_mo.cpp:526-583) */
+qmckl_exit_code_device qmckl_set_mo_basis_coefficient_device(qmckl_context_device context,
+                                                             double *coefficient);
+qmckl_exit_code_device qmckl_finalize_mo_basis_device(qmckl_context_device context);
+qmckl_exit_code_device qmckl_provide_mo_basis_mo_vgl_device(qmckl_context_device context);
+qmckl_exit_code_device qmckl_provide_mo_basis_mo_value_device(qmckl_context_device context);
+/* mo_vgl [P][5][M], size_max >= 5*M*P */
+qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_device(qmckl_context_device context,
+                                                        double *mo_vgl, int64_t size_max);
+qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_inplace_device(qmckl_context_device context,
+                                                                double *mo_vgl, int64_t size_max);
+/* mo_value [P][M], size_max >= M*P */
+qmckl_exit_code_device qmckl_get_mo_basis_mo_value_device(qmckl_context_device context,
+                                                          double *mo_value, int64_t size_max);
+qmckl_exit_code_device qmckl_get_mo_basis_mo_value_inplace_device(qmckl_context_device context,
+                                                                  double *mo_value, int64_t size_max);
+/* qmckl_gpu.hpp:790-793 (src/qmckl_mo_sycl.cpp:11-56): coefficient_t
+ * [ao_num][mo_num], ao_vgl [P][5][A] -> mo_vgl [P][5][M], FP64. */
+qmckl_exit_code_device qmckl_compute_mo_basis_mo_vgl_device(qmckl_context_device context,
+                                                            int64_t ao_num, int64_t mo_num,
+                                                            int64_t point_num, double *coefficient_t,
+                                                            double *ao_vgl, double *mo_vgl);
+/* qmckl_gpu.hpp:785-788 (src/qmckl_mo_sycl.cpp:60-134) */
+qmckl_exit_code_device qmckl_compute_mo_basis_mo_value_device(qmckl_context_device context,
+                                                              int64_t ao_num, int64_t mo_num,
+                                                              int64_t point_num, double *coefficient_t,
+                                                              double *ao_value, double *mo_value);
+
+/* ---- kernel timing (new; no counterpart in the reference) ----------------
+ * When enabled, every kernel the context launches is bracketed by HIP events
+ * on the context's stream and its elapsed time is accumulated under the
+ * kernel's name ("en_distance", "ao_vgl", "ao_value", "shell_vgl",
+ * "mo_gemm_f64", ...).  For benchmarks and profiling; off by default. */
+qmckl_exit_code_device qmckl_set_kernel_timing_device(qmckl_context_device context, bool enable);
+qmckl_exit_code_device qmckl_reset_kernel_timing_device(qmckl_context_device context);
+qmckl_exit_code_device qmckl_get_kernel_timing_device(qmckl_context_device context,
+                                                      const char *kernel, int64_t *launches,
+                                                      double *total_ms);
+
+#if defined(__GNUC__)
+#pragma GCC visibility pop
+#endif
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QMCKL_GPU_H */

@@ -1,0 +1,209 @@
+// Internal state of a qmckl context and the launcher interface between the
+// host-side provider layer (C++) and the HIP kernels (.hip files).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/qmckl_gpu.h"
+
+namespace qmckl_amd {
+
+constexpr uint64_t kValidTag = 0xBEEFFACEull;    // include/qmckl_types.hpp:12
+constexpr uint64_t kInvalidTag = 0xDEADBEEFull;  // include/qmckl_types.hpp:13
+constexpr double kCutoff = 27.631021115928547;   // -ln(1e-12), src/qmckl_ao_sycl.cpp:22
+constexpr int kMaxL = 6;                          // highest shell angular momentum on the device
+constexpr int kMaxCart = (kMaxL + 1) * (kMaxL + 2) / 2;
+
+// Device-side description of one shell, in AO order (nucleus-major, shells
+// of a nucleus in order), built at finalize.
+struct ShellDesc {
+  int32_t nucl;        // nucleus index
+  int32_t prim_start;  // first primitive
+  int32_t prim_num;    // number of primitives
+  int32_t ao_start;    // first AO
+  int32_t l;           // angular momentum
+  int32_t shell;       // index in the user's shell arrays
+};
+
+// Basis tables on the device (owned by the context, rebuilt by finalize).
+struct DeviceBasis {
+  int64_t nucl_num = 0, shell_num = 0, prim_num = 0, ao_num = 0;
+  ShellDesc *shells = nullptr;     // [S] in AO order
+  int32_t *ao_shell = nullptr;     // [A] position of the AO's shell in `shells`
+  int32_t *ao_abc = nullptr;       // [A] a | b<<8 | c<<16
+  double *expo = nullptr;          // [Np]
+  double *coef_norm = nullptr;     // [Np]
+  double *nucl_range = nullptr;    // [N]
+  int32_t *nucl_max_l = nullptr;   // [N]
+  int64_t *nucl_prim_index = nullptr;  // [N+1]
+  int max_shell_prims = 0;         // host copy: largest prim count of a shell
+  int max_l = 0;                   // host copy
+};
+
+struct PointState {
+  int64_t num = 0;
+  uint64_t date = 0;
+  double *coord = nullptr;  // [3][num]
+  int64_t capacity = 0;     // points the allocation holds
+};
+
+struct Walker {
+  int64_t num = 0;
+  PointState point;
+};
+
+struct Nucleus {
+  int64_t num = 0;
+  double *coord = nullptr;   // [3][num]
+  double *charge = nullptr;  // [num]
+  int32_t uninitialized = (1 << 3) - 1;
+  bool provided = false;
+};
+
+struct Electron {
+  int64_t num = 0, up_num = 0, down_num = 0;
+  Walker walker, walker_old;
+  double *en_distance = nullptr;  // [P][N]
+  int64_t en_distance_cap = 0;
+  uint64_t en_distance_date = 0;
+  int32_t uninitialized = (1 << 1) - 1;
+  bool provided = false;
+};
+
+struct AoBasis {
+  char type = 0;
+  int64_t shell_num = 0, prim_num = 0, ao_num = 0;
+  int64_t *nucleus_index = nullptr, *nucleus_shell_num = nullptr;
+  int32_t *shell_ang_mom = nullptr;
+  int64_t *shell_prim_num = nullptr, *shell_prim_index = nullptr;
+  double *shell_factor = nullptr, *exponent = nullptr, *coefficient = nullptr,
+         *prim_factor = nullptr, *ao_factor = nullptr;
+  DeviceBasis dev;
+  double *shell_vgl = nullptr;  // [P][5][S]
+  int64_t shell_vgl_cap = 0;
+  uint64_t shell_vgl_date = 0;
+  double *ao_vgl = nullptr;     // [P][5][A]
+  int64_t ao_vgl_cap = 0;
+  uint64_t ao_vgl_date = 0;
+  double *ao_value = nullptr;   // [P][A]
+  int64_t ao_value_cap = 0;
+  uint64_t ao_value_date = 0;
+  int32_t uninitialized = (1 << 14) - 1;
+  bool provided = false;
+};
+
+struct MoBasis {
+  int64_t mo_num = 0;
+  double *coefficient = nullptr;    // [M][A]
+  double *coefficient_t = nullptr;  // [A][M]
+  double *mo_vgl = nullptr;         // [P][5][M]
+  int64_t mo_vgl_cap = 0;
+  uint64_t mo_vgl_date = 0;
+  double *mo_value = nullptr;       // [P][M]
+  int64_t mo_value_cap = 0;
+  uint64_t mo_value_date = 0;
+  int32_t uninitialized = (1 << 2) - 1;
+  bool provided = false;
+};
+
+struct Context {
+  uint64_t tag = kValidTag;
+  int device_id = QMCKL_NO_DEVICE;
+  hipStream_t stream = nullptr;
+  std::recursive_mutex mutex;
+  int lock_count = 0;
+  qmckl_exit_code_device err_code = QMCKL_SUCCESS_DEVICE;
+  char err_function[QMCKL_MAX_FUN_LEN_DEVICE] = {0};
+  char err_message[QMCKL_MAX_MSG_LEN_DEVICE] = {0};
+  std::unordered_map<void *, size_t> device_allocs;
+  std::unordered_map<void *, size_t> host_allocs;
+  uint64_t date = 0;
+  bool timing = false;
+  struct TimerAcc {
+    int64_t launches = 0;
+    double total_ms = 0.0;
+  };
+  std::unordered_map<std::string, TimerAcc> timers;
+  PointState point;
+  Nucleus nucleus;
+  Electron electron;
+  AoBasis ao;
+  MoBasis mo;
+};
+
+// Brackets one kernel launch with HIP events on the context's stream when
+// timing is enabled; the elapsed time is added to the context's timers when
+// the object goes out of scope (after the caller's synchronisation).
+class KernelTimer {
+ public:
+  KernelTimer(Context *c, const char *name);
+  void stop();
+  ~KernelTimer();
+  KernelTimer(const KernelTimer &) = delete;
+  KernelTimer &operator=(const KernelTimer &) = delete;
+
+ private:
+  Context *c_;
+  const char *name_;
+  hipEvent_t start_ = nullptr, stop_ = nullptr;
+  bool stopped_ = false;
+};
+
+// ---- host helpers (context.cpp) ----
+Context *ctx_of(qmckl_context_device context);  // nullptr when invalid
+qmckl_exit_code_device fail(qmckl_context_device context, qmckl_exit_code_device code,
+                            const char *function, const char *message);
+// Tracked device allocation on the context's device (nullptr on failure or
+// when no device is bound).
+void *dmalloc(Context *c, size_t bytes);
+qmckl_exit_code_device dfree(Context *c, void *ptr);
+// Replace *slot by a fresh allocation of `bytes` (freeing the old one).
+bool drealloc(Context *c, void **slot, size_t bytes);
+bool has_device(const Context *c);
+hipError_t sync(Context *c);
+qmckl_exit_code_device hip_check(qmckl_context_device context, hipError_t e, const char *function);
+
+// Device tables from host copies of the basis arrays (ao.cpp).  Returns an
+// error message, empty on success.  expo_dev is stored as b.expo.
+std::string build_basis_tables(Context *c, DeviceBasis &b, int64_t N, int64_t S, int64_t Np,
+                               int64_t A, const std::vector<int64_t> &nucleus_index,
+                               const std::vector<int64_t> &nucleus_shell_num,
+                               const std::vector<int32_t> &shell_ang_mom,
+                               const std::vector<int64_t> &shell_prim_num,
+                               const std::vector<int64_t> &shell_prim_index,
+                               const std::vector<double> &shell_factor,
+                               const std::vector<double> &exponent,
+                               const std::vector<double> &coefficient,
+                               const std::vector<double> &prim_factor, double *expo_dev);
+void release_basis_tables(Context *c, DeviceBasis &b);
+
+// ---- kernel launchers (.hip files); all enqueue on `stream` ----
+hipError_t launch_soa_from_aos(hipStream_t s, int64_t n, const double *aos, double *soa);
+hipError_t launch_aos_from_soa(hipStream_t s, int64_t n, const double *soa, double *aos);
+hipError_t launch_transpose(hipStream_t s, int64_t rows, int64_t cols, const double *in,
+                            double *out);
+hipError_t launch_en_distance(hipStream_t s, int64_t P, int64_t N, const double *coord,
+                              const double *nucl, double *out);
+hipError_t launch_shell_vgl(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
+                            const double *nucl, double *shell_vgl);
+hipError_t launch_ao_vgl(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
+                         const double *nucl, const double *ao_factor, double *ao_vgl,
+                         int ncomp);
+hipError_t launch_ao_from_shell_table(hipStream_t s, const DeviceBasis &b, int64_t P,
+                                      const double *coord, const double *nucl,
+                                      const double *ao_factor, const double *shell_vgl,
+                                      int64_t S_table, double *out, int ncomp);
+hipError_t launch_mo_gemm_f64(hipStream_t s, int64_t rows, int64_t K, int64_t Ncol,
+                              const double *A, int64_t lda, const double *B, int64_t ldb,
+                              double *C, int64_t ldc);
+hipError_t launch_strided_copy(hipStream_t s, int64_t rows, int64_t cols, const double *in,
+                               int64_t ld_in, double *out, int64_t ld_out);
+
+}  // namespace qmckl_amd

@@ -1,0 +1,289 @@
+// Shell/AO VGL kernels (FP64).
+//
+// Arithmetic of src/qmckl_ao_sycl.cpp: radial shell sums with the nucleus
+// screen r^2 > cutoff*range and the primitive screen alpha r^2 > cutoff
+// (40-95), Cartesian monomials x^a y^b z^c in order of descending a then b
+// (291-331), and the product-rule assembly times ao_factor (367-422).
+// Screened entries are exact zeros.
+//
+// Work decomposition (one workgroup = 64 points x all AOs, 4 wavefronts):
+//   for each chunk of kShellChunk shells (in AO order):
+//     stage the chunk's shell descriptors and its (exponent, normalised
+//     coefficient) pairs in LDS;
+//     phase 1: each wavefront takes whole shells, lane = point, so the
+//       primitive loop is wavefront-uniform (LDS broadcast reads) and a
+//       primitive or shell screened for all 64 points costs no exp: the
+//       branch is skipped when every lane's predicate is false.  Radial
+//       R, dR/dx, dR/dy, dR/dz, lap R land in LDS [shell][5][point];
+//     phase 2: thread = (point, AO of the chunk), AO fastest, forms the
+//       monomial and its derivatives from the AO's (a,b,c) and writes the
+//       five components: consecutive lanes store consecutive AOs of one
+//       [p][c][*] row, so every store is a contiguous run.
+// r^2 is fma(x,x,fma(y,y,z*z)), as in the CPU oracle, so the screens decide
+// identically.  The kernel is bound by the HBM write of ao_vgl:
+// 40 B per (point, AO) plus 24 B per point.
+#include <hip/hip_runtime.h>
+
+#include "internal.hpp"
+
+namespace qmckl_amd {
+
+namespace {
+
+constexpr int kTP = 64;          // points per workgroup
+constexpr int kThreads = 256;    // 4 wavefronts
+constexpr int kShellChunk = 16;  // shells per LDS chunk
+constexpr int kPrimLds = 384;    // staged primitives per chunk
+
+struct Mono {
+  double v, gx, gy, gz, lap;
+};
+
+// (x^a, a x^(a-1), a(a-1) x^(a-2)) by the recurrence
+// (f, f', f'') -> (x f, f + x f', 2 f' + x f'').
+__device__ inline void pow3(double x, int a, double &f0, double &f1, double &f2) {
+  double p0 = 1.0, p1 = 0.0, p2 = 0.0;
+  for (int i = 0; i < a; ++i) {
+    const double n2 = 2.0 * p1 + x * p2;
+    const double n1 = p0 + x * p1;
+    p0 = x * p0;
+    p1 = n1;
+    p2 = n2;
+  }
+  f0 = p0;
+  f1 = p1;
+  f2 = p2;
+}
+
+__device__ inline Mono monomial(int abc, double x, double y, double z) {
+  const int a = abc & 0xff, b = (abc >> 8) & 0xff, c = (abc >> 16) & 0xff;
+  double X0, X1, X2, Y0, Y1, Y2, Z0, Z1, Z2;
+  pow3(x, a, X0, X1, X2);
+  pow3(y, b, Y0, Y1, Y2);
+  pow3(z, c, Z0, Z1, Z2);
+  Mono m;
+  const double yz = Y0 * Z0;
+  m.v = X0 * yz;
+  m.gx = X1 * yz;
+  m.gy = X0 * Y1 * Z0;
+  m.gz = X0 * Y0 * Z1;
+  m.lap = X2 * yz + X0 * Y2 * Z0 + X0 * Y0 * Z2;
+  return m;
+}
+
+// NCOMP = 5 (VGL) or 1 (value).  FROM_TABLE: radial parts are read from a
+// precomputed shell_vgl [P][5][S] instead of being summed from primitives.
+template <int NCOMP, bool FROM_TABLE>
+__global__ void __launch_bounds__(kThreads)
+ao_kernel(int64_t P, int S, int A, int N, const double *__restrict__ coord,
+          const double *__restrict__ nucl, const ShellDesc *__restrict__ shells,
+          const int32_t *__restrict__ ao_shell, const int32_t *__restrict__ ao_abc,
+          const double *__restrict__ ao_factor, const double *__restrict__ expo,
+          const double *__restrict__ coef, const double *__restrict__ nucl_range,
+          const double *__restrict__ shell_table, int S_table, double *__restrict__ out) {
+  __shared__ double rad[kShellChunk][NCOMP][kTP];
+  __shared__ double pt[3][kTP];
+  __shared__ double2 prim[kPrimLds];
+  __shared__ ShellDesc sd[kShellChunk];
+  __shared__ int prim_off[kShellChunk + 1];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int64_t p0 = (int64_t)blockIdx.x * kTP;
+  const int np = (int)((P - p0) < kTP ? (P - p0) : kTP);
+
+  if (tid < kTP) {
+    const int64_t p = p0 + tid;
+    const bool ok = tid < np;
+    pt[0][tid] = ok ? coord[p] : 0.0;
+    pt[1][tid] = ok ? coord[P + p] : 0.0;
+    pt[2][tid] = ok ? coord[2 * (int64_t)P + p] : 0.0;
+  }
+
+  for (int s0 = 0; s0 < S; s0 += kShellChunk) {
+    const int ns = (S - s0) < kShellChunk ? (S - s0) : kShellChunk;
+    __syncthreads();  // previous chunk's phase 2 is done with rad/sd
+    if (tid < ns) sd[tid] = shells[s0 + tid];
+    __syncthreads();
+    if (tid == 0) {
+      int off = 0;
+      for (int i = 0; i < ns; ++i) {
+        prim_off[i] = off;
+        off += sd[i].prim_num;
+      }
+      prim_off[ns] = off;
+    }
+    __syncthreads();
+    const bool staged = !FROM_TABLE && prim_off[ns] <= kPrimLds;
+    if (staged) {
+      for (int i = 0; i < ns; ++i) {
+        const int base = sd[i].prim_start, cnt = sd[i].prim_num, o = prim_off[i];
+        for (int j = tid; j < cnt; j += kThreads) prim[o + j] = make_double2(expo[base + j], coef[base + j]);
+      }
+    }
+    __syncthreads();
+
+    // phase 1: radial parts, wavefront per shell, lane per point
+    for (int sl = wave; sl < ns; sl += kThreads / 64) {
+      const ShellDesc d = sd[sl];
+      double t[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      if (FROM_TABLE) {
+        if (lane < np) {
+          const double *row = shell_table + (p0 + lane) * 5 * (int64_t)S_table + d.shell;
+#pragma unroll
+          for (int c = 0; c < NCOMP; ++c) t[c] = row[(int64_t)c * S_table];
+        }
+      } else {
+        const double x = pt[0][lane] - nucl[d.nucl];
+        const double y = pt[1][lane] - nucl[N + d.nucl];
+        const double z = pt[2][lane] - nucl[2 * N + d.nucl];
+        const double r2 = fma(x, x, fma(y, y, z * z));
+        const bool live = lane < np && !(r2 > kCutoff * nucl_range[d.nucl]);
+        for (int j = 0; j < d.prim_num; ++j) {
+          double a, cn;
+          if (staged) {
+            const double2 pr = prim[prim_off[sl] + j];
+            a = pr.x;
+            cn = pr.y;
+          } else {
+            a = expo[d.prim_start + j];
+            cn = coef[d.prim_start + j];
+          }
+          const double ar2 = a * r2;
+          if (live && !(ar2 > kCutoff)) {
+            const double v = cn * exp(-ar2);
+            t[0] += v;
+            if (NCOMP == 5) {
+              const double two_a = -2.0 * a * v;
+              t[1] += two_a * x;
+              t[2] += two_a * y;
+              t[3] += two_a * z;
+              t[4] += two_a * (3.0 - 2.0 * ar2);
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < NCOMP; ++c) rad[sl][c][lane] = t[c];
+    }
+    __syncthreads();
+
+    // phase 2: AOs of the chunk, thread = (point, AO), AO fastest
+    const int ka = sd[0].ao_start;
+    const int kb = sd[ns - 1].ao_start + (sd[ns - 1].l + 1) * (sd[ns - 1].l + 2) / 2;
+    const int nk = kb - ka;
+    const int work = np * nk;
+    for (int i = tid; i < work; i += kThreads) {
+      const int pl = i / nk;
+      const int k = ka + (i - pl * nk);
+      const int sl = ao_shell[k] - s0;
+      const int n = sd[sl].nucl;
+      const double x = pt[0][pl] - nucl[n];
+      const double y = pt[1][pl] - nucl[N + n];
+      const double z = pt[2][pl] - nucl[2 * N + n];
+      const Mono m = monomial(ao_abc[k], x, y, z);
+      const double f = ao_factor[k];
+      double *o = out + (p0 + pl) * (int64_t)NCOMP * A + k;
+      const double R = rad[sl][0][pl];
+      if (NCOMP == 1) {
+        o[0] = m.v * R * f;
+      } else {
+        const double Rx = rad[sl][1 % NCOMP][pl], Ry = rad[sl][2 % NCOMP][pl];
+        const double Rz = rad[sl][3 % NCOMP][pl], RL = rad[sl][4 % NCOMP][pl];
+        o[0] = m.v * R * f;
+        o[A] = (m.gx * R + m.v * Rx) * f;
+        o[2 * (int64_t)A] = (m.gy * R + m.v * Ry) * f;
+        o[3 * (int64_t)A] = (m.gz * R + m.v * Rz) * f;
+        o[4 * (int64_t)A] = (m.lap * R + m.v * RL + 2.0 * (m.gx * Rx + m.gy * Ry + m.gz * Rz)) * f;
+      }
+    }
+  }
+}
+
+// shell_vgl [P][5][S] in the user's shell numbering; thread = (point, shell).
+__global__ void __launch_bounds__(kThreads)
+shell_vgl_kernel(int64_t P, int S, int N, const double *__restrict__ coord,
+                 const double *__restrict__ nucl, const ShellDesc *__restrict__ shells,
+                 const double *__restrict__ expo, const double *__restrict__ coef,
+                 const double *__restrict__ nucl_range, double *__restrict__ out) {
+  const int64_t total = P * (int64_t)S;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = i / S;
+    const ShellDesc d = shells[i - p * S];
+    const double x = coord[p] - nucl[d.nucl];
+    const double y = coord[P + p] - nucl[N + d.nucl];
+    const double z = coord[2 * P + p] - nucl[2 * N + d.nucl];
+    const double r2 = fma(x, x, fma(y, y, z * z));
+    double t[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (!(r2 > kCutoff * nucl_range[d.nucl])) {
+      for (int j = d.prim_start; j < d.prim_start + d.prim_num; ++j) {
+        const double ar2 = expo[j] * r2;
+        if (ar2 > kCutoff) continue;
+        const double v = coef[j] * exp(-ar2);
+        const double two_a = -2.0 * expo[j] * v;
+        t[0] += v;
+        t[1] += two_a * x;
+        t[2] += two_a * y;
+        t[3] += two_a * z;
+        t[4] += two_a * (3.0 - 2.0 * ar2);
+      }
+    }
+    double *o = out + p * 5 * S + d.shell;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) o[(int64_t)c * S] = t[c];
+  }
+}
+
+}  // namespace
+
+hipError_t launch_ao_vgl(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
+                         const double *nucl, const double *ao_factor, double *out, int ncomp) {
+  if (P <= 0) return hipSuccess;
+  const int64_t blocks = (P + kTP - 1) / kTP;
+  if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+  if (ncomp == 5)
+    hipLaunchKernelGGL((ao_kernel<5, false>), dim3((unsigned)blocks), dim3(kThreads), 0, s, P,
+                       (int)b.shell_num, (int)b.ao_num, (int)b.nucl_num, coord, nucl, b.shells,
+                       b.ao_shell, b.ao_abc, ao_factor, b.expo, b.coef_norm, b.nucl_range,
+                       (const double *)nullptr, 0, out);
+  else
+    hipLaunchKernelGGL((ao_kernel<1, false>), dim3((unsigned)blocks), dim3(kThreads), 0, s, P,
+                       (int)b.shell_num, (int)b.ao_num, (int)b.nucl_num, coord, nucl, b.shells,
+                       b.ao_shell, b.ao_abc, ao_factor, b.expo, b.coef_norm, b.nucl_range,
+                       (const double *)nullptr, 0, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_ao_from_shell_table(hipStream_t s, const DeviceBasis &b, int64_t P,
+                                      const double *coord, const double *nucl,
+                                      const double *ao_factor, const double *shell_vgl,
+                                      int64_t S_table, double *out, int ncomp) {
+  if (P <= 0) return hipSuccess;
+  const int64_t blocks = (P + kTP - 1) / kTP;
+  if (ncomp == 5)
+    hipLaunchKernelGGL((ao_kernel<5, true>), dim3((unsigned)blocks), dim3(kThreads), 0, s, P,
+                       (int)b.shell_num, (int)b.ao_num, (int)b.nucl_num, coord, nucl, b.shells,
+                       b.ao_shell, b.ao_abc, ao_factor, (const double *)nullptr,
+                       (const double *)nullptr, (const double *)nullptr, shell_vgl, (int)S_table, out);
+  else
+    hipLaunchKernelGGL((ao_kernel<1, true>), dim3((unsigned)blocks), dim3(kThreads), 0, s, P,
+                       (int)b.shell_num, (int)b.ao_num, (int)b.nucl_num, coord, nucl, b.shells,
+                       b.ao_shell, b.ao_abc, ao_factor, (const double *)nullptr,
+                       (const double *)nullptr, (const double *)nullptr, shell_vgl, (int)S_table, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_shell_vgl(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
+                            const double *nucl, double *shell_vgl) {
+  if (P <= 0) return hipSuccess;
+  const int64_t work = P * b.shell_num;
+  int64_t blocks = (work + kThreads - 1) / kThreads;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(shell_vgl_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, P,
+                     (int)b.shell_num, (int)b.nucl_num, coord, nucl, b.shells, b.expo, b.coef_norm,
+                     b.nucl_range, shell_vgl);
+  return hipGetLastError();
+}
+
+}  // namespace qmckl_amd

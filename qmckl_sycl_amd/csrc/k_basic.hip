@@ -1,0 +1,139 @@
+// Layout kernels and the electron-nucleus distance kernel.
+//
+// en_distance[p][n] = |r_p - R_n| (src/qmckl_electron.cpp:302-331 calls
+// qmckl_distance_device('T','T') whose kernel is at 536-549; here the plain
+// Euclidean distance).  Coordinates are structure-of-arrays: point x, y, z
+// rows of length P are read with unit stride, the nucleus table (3N doubles)
+// is staged in LDS once per workgroup, and each thread writes consecutive n
+// of one point row so stores are contiguous.  The kernel is HBM-bound:
+// 24 B read + 8N B written per point.
+#include <hip/hip_runtime.h>
+
+#include "internal.hpp"
+
+namespace qmckl_amd {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+__global__ void __launch_bounds__(kThreads) soa_from_aos_kernel(int64_t n, const double *__restrict__ aos,
+                                                                double *__restrict__ soa) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 3 * n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    // soa[d][p] = aos[p][d]; read coalesced along the AoS array.
+    const int64_t p = i / 3, d = i - 3 * p;
+    soa[d * n + p] = aos[i];
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) aos_from_soa_kernel(int64_t n, const double *__restrict__ soa,
+                                                                double *__restrict__ aos) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 3 * n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = i / 3, d = i - 3 * p;
+    aos[i] = soa[d * n + p];
+  }
+}
+
+// out[c][r] = in[r][c], 32x32 tiles through LDS.
+__global__ void __launch_bounds__(kThreads) transpose_kernel(int64_t rows, int64_t cols,
+                                                             const double *__restrict__ in,
+                                                             double *__restrict__ out) {
+  __shared__ double tile[32][33];
+  const int64_t r0 = (int64_t)blockIdx.y * 32, c0 = (int64_t)blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int j = ty; j < 32; j += 8) {
+    const int64_t r = r0 + j, c = c0 + tx;
+    tile[j][tx] = (r < rows && c < cols) ? in[r * cols + c] : 0.0;
+  }
+  __syncthreads();
+  for (int j = ty; j < 32; j += 8) {
+    const int64_t c = c0 + j, r = r0 + tx;
+    if (r < rows && c < cols) out[c * rows + r] = tile[tx][j];
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) strided_copy_kernel(int64_t rows, int64_t cols,
+                                                                const double *__restrict__ in,
+                                                                int64_t ld_in, double *__restrict__ out,
+                                                                int64_t ld_out) {
+  const int64_t total = rows * cols;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / cols, c = i - r * cols;
+    out[r * ld_out + c] = in[r * ld_in + c];
+  }
+}
+
+constexpr int kMaxNuclLds = 2048;  // nuclei staged in LDS (at most 48 KiB)
+
+__global__ void __launch_bounds__(kThreads) en_distance_kernel(int64_t P, int64_t N,
+                                                               const double *__restrict__ coord,
+                                                               const double *__restrict__ nucl,
+                                                               double *__restrict__ out) {
+  extern __shared__ double snucl[];  // [3][N] when N <= kMaxNuclLds
+  const bool staged = N <= kMaxNuclLds;
+  if (staged) {
+    for (int i = threadIdx.x; i < 3 * N; i += blockDim.x) snucl[i] = nucl[i];
+    __syncthreads();
+  }
+  const double *R = staged ? snucl : nucl;
+  const int64_t total = P * N;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = i / N, n = i - p * N;
+    const double x = coord[p] - R[n];
+    const double y = coord[P + p] - R[N + n];
+    const double z = coord[2 * P + p] - R[2 * N + n];
+    out[i] = sqrt(fma(x, x, fma(y, y, z * z)));
+  }
+}
+
+unsigned grid_for(int64_t work) {
+  int64_t g = (work + kThreads - 1) / kThreads;
+  if (g < 1) g = 1;
+  if (g > 65536) g = 65536;
+  return (unsigned)g;
+}
+
+}  // namespace
+
+hipError_t launch_soa_from_aos(hipStream_t s, int64_t n, const double *aos, double *soa) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(soa_from_aos_kernel, dim3(grid_for(3 * n)), dim3(kThreads), 0, s, n, aos, soa);
+  return hipGetLastError();
+}
+
+hipError_t launch_aos_from_soa(hipStream_t s, int64_t n, const double *soa, double *aos) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(aos_from_soa_kernel, dim3(grid_for(3 * n)), dim3(kThreads), 0, s, n, soa, aos);
+  return hipGetLastError();
+}
+
+hipError_t launch_transpose(hipStream_t s, int64_t rows, int64_t cols, const double *in, double *out) {
+  if (rows <= 0 || cols <= 0) return hipSuccess;
+  if ((rows + 31) / 32 > 65535) return hipErrorInvalidValue;
+  dim3 grid((unsigned)((cols + 31) / 32), (unsigned)((rows + 31) / 32));
+  hipLaunchKernelGGL(transpose_kernel, grid, dim3(kThreads), 0, s, rows, cols, in, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_strided_copy(hipStream_t s, int64_t rows, int64_t cols, const double *in,
+                               int64_t ld_in, double *out, int64_t ld_out) {
+  if (rows <= 0 || cols <= 0) return hipSuccess;
+  hipLaunchKernelGGL(strided_copy_kernel, dim3(grid_for(rows * cols)), dim3(kThreads), 0, s, rows,
+                     cols, in, ld_in, out, ld_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_en_distance(hipStream_t s, int64_t P, int64_t N, const double *coord,
+                              const double *nucl, double *out) {
+  if (P <= 0 || N <= 0) return hipSuccess;
+  const size_t lds = N <= kMaxNuclLds ? sizeof(double) * 3 * (size_t)N : 0;
+  hipLaunchKernelGGL(en_distance_kernel, dim3(grid_for(P * N)), dim3(kThreads), lds, s, P, N, coord,
+                     nucl, out);
+  return hipGetLastError();
+}
+
+}  // namespace qmckl_amd

@@ -1,0 +1,275 @@
+// MO basis: setters/getters, finalize and the mo_vgl / mo_value providers.
+//
+// Follows src/qmckl_mo.cpp (provided 10-23, provide_mo_vgl 114-181,
+// provide_mo_value 185-286, getters 294-453, set_mo_num 486-524,
+// set_coefficient 526-583) and src/qmckl_mo_sycl.cpp (contraction 11-56,
+// finalize 136-190: coefficient_t[A][M] = coefficient[M][A]).  The
+// contraction mo_vgl[(p,c)][i] = sum_k ao_vgl[(p,c)][k] coefficient_t[k][i]
+// is one GEMM with 5P rows, run by the FP64 MFMA kernel.
+#include <algorithm>
+
+#include "internal.hpp"
+
+using namespace qmckl_amd;
+
+namespace {
+
+bool grow(Context *c, double **slot, int64_t *cap, int64_t need) {
+  if (*slot != nullptr && *cap >= need) return true;
+  if (!drealloc(c, reinterpret_cast<void **>(slot), sizeof(double) * (size_t)std::max<int64_t>(need, 1)))
+    return false;
+  *cap = need;
+  return true;
+}
+
+qmckl_exit_code_device ready(qmckl_context_device context, Context *c, const char *fn) {
+  if (!c->mo.provided) return fail(context, QMCKL_NOT_PROVIDED_DEVICE, fn, nullptr);
+  if (!c->ao.provided) return fail(context, QMCKL_NOT_PROVIDED_DEVICE, fn, "ao_basis not provided");
+  if (!has_device(c)) return fail(context, QMCKL_FAILURE_DEVICE, fn, "no device bound");
+  return QMCKL_SUCCESS_DEVICE;
+}
+
+// mo_vgl for the current points into dst, from the context's ao_vgl.
+qmckl_exit_code_device compute_mo_vgl(qmckl_context_device context, Context *c, double *dst,
+                                      const char *fn) {
+  qmckl_exit_code_device rc = qmckl_provide_ao_basis_ao_vgl_device(context);
+  if (rc != QMCKL_SUCCESS_DEVICE) return fail(context, QMCKL_NOT_PROVIDED_DEVICE, "qmckl_ao_basis", nullptr);
+  const int64_t P = c->point.num, A = c->ao.ao_num, M = c->mo.mo_num;
+  hipError_t e;
+  {
+    KernelTimer t(c, "mo_gemm_f64");
+    e = launch_mo_gemm_f64(c->stream, 5 * P, A, M, c->ao.ao_vgl, A, c->mo.coefficient_t, M, dst, M);
+    t.stop();
+  }
+  if (e == hipSuccess) e = sync(c);
+  return hip_check(context, e, fn);
+}
+
+qmckl_exit_code_device compute_mo_value(qmckl_context_device context, Context *c, double *dst,
+                                        const char *fn) {
+  const int64_t P = c->point.num, A = c->ao.ao_num, M = c->mo.mo_num;
+  hipError_t e;
+  if (c->mo.mo_vgl_date != 0 && c->mo.mo_vgl_date == c->point.date && c->mo.mo_vgl) {
+    // mo_vgl is current: copy its value rows (src/qmckl_mo.cpp:230-256).
+    e = launch_strided_copy(c->stream, P, M, c->mo.mo_vgl, 5 * M, dst, M);
+  } else {
+    qmckl_exit_code_device rc = qmckl_provide_ao_basis_ao_value_device(context);
+    if (rc != QMCKL_SUCCESS_DEVICE)
+      return fail(context, QMCKL_NOT_PROVIDED_DEVICE, "qmckl_ao_basis_ao_value_device", nullptr);
+    KernelTimer t(c, "mo_value_gemm_f64");
+    e = launch_mo_gemm_f64(c->stream, P, A, M, c->ao.ao_value, A, c->mo.coefficient_t, M, dst, M);
+    t.stop();
+  }
+  if (e == hipSuccess) e = sync(c);
+  return hip_check(context, e, fn);
+}
+
+}  // namespace
+
+extern "C" {
+
+bool qmckl_mo_basis_provided_device(qmckl_context_device context) {
+  Context *c = ctx_of(context);
+  return c ? c->mo.provided : false;
+}
+
+qmckl_exit_code_device qmckl_get_mo_basis_mo_num_device(qmckl_context_device context, int64_t *mo_num) {
+  Context *c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+  if (c->mo.uninitialized & 1)
+    return fail(context, QMCKL_NOT_PROVIDED_DEVICE, "qmckl_get_mo_basis_mo_num", nullptr);
+  if (!mo_num) return fail(context, QMCKL_INVALID_ARG_2_DEVICE, "qmckl_get_mo_basis_mo_num", "NULL pointer");
+  *mo_num = c->mo.mo_num;
+  return QMCKL_SUCCESS_DEVICE;
+}
+
+qmckl_exit_code_device qmckl_finalize_mo_basis_device(qmckl_context_device context) {
+  Context *c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+  const char *fn = "qmckl_finalize_mo_basis_device";
+  if (c->mo.coefficient == nullptr) return fail(context, QMCKL_NOT_PROVIDED_DEVICE, fn, nullptr);
+  const int64_t A = c->ao.ao_num, M = c->mo.mo_num;
+  if (!drealloc(c, reinterpret_cast<void **>(&c->mo.coefficient_t), sizeof(double) * A * M))
+    return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
+  hipError_t e = launch_transpose(c->stream, M, A, c->mo.coefficient, c->mo.coefficient_t);
+  if (e == hipSuccess) e = sync(c);
+  if (e != hipSuccess) return hip_check(context, e, fn);
+  c->mo.mo_vgl_date = c->mo.mo_value_date = 0;
+  return QMCKL_SUCCESS_DEVICE;
+}
+
+qmckl_exit_code_device qmckl_set_mo_basis_mo_num_device(qmckl_context_device context, int64_t mo_num) {
+  Context *c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+  const int32_t mask = 1;
+  if (!(c->mo.uninitialized & mask))
+    return fail(context, QMCKL_ALREADY_SET_DEVICE, "qmckl_set_mo_basis_mo_num_device", nullptr);
+  if (mo_num <= 0)
+    return fail(context, QMCKL_INVALID_ARG_2_DEVICE, "qmckl_set_mo_basis_mo_num_device", "mo_num <= 0");
+  c->mo.mo_num = mo_num;
+  c->mo.uninitialized &= ~mask;
+  c->mo.provided = (c->mo.uninitialized == 0);
+  if (c->mo.provided) return qmckl_finalize_mo_basis_device(context);
+  return QMCKL_SUCCESS_DEVICE;
+}
+
+qmckl_exit_code_device qmckl_set_mo_basis_coefficient_device(qmckl_context_device context,
+                                                             double *coefficient) {
+  Context *c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+  const char *fn = "qmckl_set_mo_basis_coefficient_device";
+  const int32_t mask = 1 << 1;
+  if (!(c->mo.uninitialized & mask)) return fail(context, QMCKL_ALREADY_SET_DEVICE, fn, nullptr);
+  if (coefficient == nullptr) return fail(context, QMCKL_INVALID_ARG_2_DEVICE, fn, "NULL pointer");
+  // The reference copies ao_num*mo_num values unchecked (src/qmckl_mo.cpp:558-569).
+  if (c->ao.ao_num <= 0) return fail(context, QMCKL_FAILURE_DEVICE, fn, "ao_num is not set");
+  if (c->mo.mo_num <= 0) return fail(context, QMCKL_FAILURE_DEVICE, fn, "mo_num is not set");
+  if (!has_device(c)) return fail(context, QMCKL_FAILURE_DEVICE, fn, "no device bound");
+  const int64_t n = c->ao.ao_num * c->mo.mo_num;
+  if (!drealloc(c, reinterpret_cast<void **>(&c->mo.coefficient), sizeof(double) * n))
+    return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
+  hipError_t e = hipMemcpyAsync(c->mo.coefficient, coefficient, sizeof(double) * n,
+                                hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess) e = sync(c);
+  if (e != hipSuccess) return hip_check(context, e, fn);
+  c->mo.uninitialized &= ~mask;
+  c->mo.provided = (c->mo.uninitialized == 0);
+  if (c->mo.provided) return qmckl_finalize_mo_basis_device(context);
+  return QMCKL_SUCCESS_DEVICE;
+}
+
+qmckl_exit_code_device qmckl_provide_mo_basis_mo_vgl_device(qmckl_context_device context) {
+  Context *c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+  const char *fn = "qmckl_provide_mo_basis_mo_vgl_device";
+  qmckl_exit_code_device rc = ready(context, c, fn);
+  if (rc != QMCKL_SUCCESS_DEVICE) return rc;
+  if (c->point.date > c->mo.mo_vgl_date || c->mo.mo_vgl_date == 0) {
+    if (!grow(c, &c->mo.mo_vgl, &c->mo.mo_vgl_cap, 5 * c->mo.mo_num * c->point.num))
+      return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
+    rc = compute_mo_vgl(context, c, c->mo.mo_vgl, fn);
+    if (rc != QMCKL_SUCCESS_DEVICE) return rc;
+    c->mo.mo_vgl_date = c->date;
+  }
+  return QMCKL_SUCCESS_DEVICE;
+}
+
+qmckl_exit_code_device qmckl_provide_mo_basis_mo_value_device(qmckl_context_device context) {
+  Context *c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+  const char *fn = "qmckl_provide_mo_basis_mo_value_device";
+  qmckl_exit_code_device rc = ready(context, c, fn);
+  if (rc != QMCKL_SUCCESS_DEVICE) return rc;
+  if (c->point.date > c->mo.mo_value_date || c->mo.mo_value_date == 0) {
+    if (!grow(c, &c->mo.mo_value, &c->mo.mo_value_cap, c->mo.mo_num * c->point.num))
+      return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
+    rc = compute_mo_value(context, c, c->mo.mo_value, fn);
+    if (rc != QMCKL_SUCCESS_DEVICE) return rc;
+    c->mo.mo_value_date = c->date;
+  }
+  return QMCKL_SUCCESS_DEVICE;
+}
+
+qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_device(qmckl_context_device context, double *mo_vgl,
+                                                        int64_t size_max) {
+  Context *c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+  const char *fn = "qmckl_get_mo_basis_mo_vgl_device";
+  if (!mo_vgl) return fail(context, QMCKL_INVALID_ARG_2_DEVICE, fn, "NULL pointer");
+  qmckl_exit_code_device rc = qmckl_provide_mo_basis_mo_vgl_device(context);
+  if (rc != QMCKL_SUCCESS_DEVICE) return rc;
+  const int64_t sze = 5 * c->point.num * c->mo.mo_num;
+  if (size_max < sze) return fail(context, QMCKL_INVALID_ARG_3_DEVICE, fn, "input array too small");
+  hipError_t e = hipMemcpyAsync(mo_vgl, c->mo.mo_vgl, sizeof(double) * sze, hipMemcpyDeviceToDevice,
+                                c->stream);
+  if (e == hipSuccess) e = sync(c);
+  return hip_check(context, e, fn);
+}
+
+qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_inplace_device(qmckl_context_device context,
+                                                                double *mo_vgl, int64_t size_max) {
+  Context *c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+  const char *fn = "qmckl_get_mo_basis_mo_vgl_inplace_device";
+  if (!mo_vgl) return fail(context, QMCKL_INVALID_ARG_2_DEVICE, fn, "NULL pointer");
+  const int64_t sze = 5 * c->mo.mo_num * c->point.num;
+  if (size_max < sze) return fail(context, QMCKL_INVALID_ARG_3_DEVICE, fn, "input array too small");
+  qmckl_exit_code_device rc = qmckl_context_touch_device(context);
+  if (rc != QMCKL_SUCCESS_DEVICE) return rc;
+  rc = ready(context, c, fn);
+  if (rc != QMCKL_SUCCESS_DEVICE) return rc;
+  return compute_mo_vgl(context, c, mo_vgl, fn);
+}
+
+qmckl_exit_code_device qmckl_get_mo_basis_mo_value_device(qmckl_context_device context,
+                                                          double *mo_value, int64_t size_max) {
+  Context *c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+  const char *fn = "qmckl_get_mo_basis_mo_value_device";
+  if (!mo_value) return fail(context, QMCKL_INVALID_ARG_2_DEVICE, fn, "NULL pointer");
+  qmckl_exit_code_device rc = qmckl_provide_mo_basis_mo_value_device(context);
+  if (rc != QMCKL_SUCCESS_DEVICE) return rc;
+  const int64_t sze = c->point.num * c->mo.mo_num;
+  if (size_max < sze) return fail(context, QMCKL_INVALID_ARG_3_DEVICE, fn, "input array too small");
+  hipError_t e = hipMemcpyAsync(mo_value, c->mo.mo_value, sizeof(double) * sze,
+                                hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess) e = sync(c);
+  return hip_check(context, e, fn);
+}
+
+qmckl_exit_code_device qmckl_get_mo_basis_mo_value_inplace_device(qmckl_context_device context,
+                                                                  double *mo_value, int64_t size_max) {
+  Context *c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+  const char *fn = "qmckl_get_mo_basis_mo_value_inplace_device";
+  if (!mo_value) return fail(context, QMCKL_INVALID_ARG_2_DEVICE, fn, "NULL pointer");
+  const int64_t sze = c->mo.mo_num * c->point.num;
+  if (size_max < sze) return fail(context, QMCKL_INVALID_ARG_3_DEVICE, fn, "input array too small");
+  qmckl_exit_code_device rc = qmckl_context_touch_device(context);
+  if (rc != QMCKL_SUCCESS_DEVICE) return rc;
+  rc = ready(context, c, fn);
+  if (rc != QMCKL_SUCCESS_DEVICE) return rc;
+  return compute_mo_value(context, c, mo_value, fn);
+}
+
+qmckl_exit_code_device qmckl_compute_mo_basis_mo_vgl_device(qmckl_context_device context,
+                                                            int64_t ao_num, int64_t mo_num,
+                                                            int64_t point_num, double *coefficient_t,
+                                                            double *ao_vgl, double *mo_vgl) {
+  Context *c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+  const char *fn = "qmckl_compute_mo_basis_mo_vgl_device";
+  if (ao_num <= 0) return fail(context, QMCKL_INVALID_ARG_2_DEVICE, fn, "ao_num <= 0");
+  if (mo_num <= 0) return fail(context, QMCKL_INVALID_ARG_3_DEVICE, fn, "mo_num <= 0");
+  if (point_num < 0) return fail(context, QMCKL_INVALID_ARG_4_DEVICE, fn, "point_num < 0");
+  if (!coefficient_t) return fail(context, QMCKL_INVALID_ARG_5_DEVICE, fn, "NULL pointer");
+  if (!ao_vgl) return fail(context, QMCKL_INVALID_ARG_6_DEVICE, fn, "NULL pointer");
+  if (!mo_vgl) return fail(context, QMCKL_INVALID_ARG_7_DEVICE, fn, "NULL pointer");
+  if (!has_device(c)) return fail(context, QMCKL_FAILURE_DEVICE, fn, "no device bound");
+  hipError_t e = launch_mo_gemm_f64(c->stream, 5 * point_num, ao_num, mo_num, ao_vgl, ao_num,
+                                    coefficient_t, mo_num, mo_vgl, mo_num);
+  if (e == hipSuccess) e = sync(c);
+  return hip_check(context, e, fn);
+}
+
+qmckl_exit_code_device qmckl_compute_mo_basis_mo_value_device(qmckl_context_device context,
+                                                              int64_t ao_num, int64_t mo_num,
+                                                              int64_t point_num, double *coefficient_t,
+                                                              double *ao_value, double *mo_value) {
+  Context *c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+  const char *fn = "qmckl_compute_mo_basis_mo_value_device";
+  if (ao_num <= 0) return fail(context, QMCKL_INVALID_ARG_2_DEVICE, fn, "ao_num <= 0");
+  if (mo_num <= 0) return fail(context, QMCKL_INVALID_ARG_3_DEVICE, fn, "mo_num <= 0");
+  if (point_num < 0) return fail(context, QMCKL_INVALID_ARG_4_DEVICE, fn, "point_num < 0");
+  if (!coefficient_t) return fail(context, QMCKL_INVALID_ARG_5_DEVICE, fn, "NULL pointer");
+  if (!ao_value) return fail(context, QMCKL_INVALID_ARG_6_DEVICE, fn, "NULL pointer");
+  if (!mo_value) return fail(context, QMCKL_INVALID_ARG_7_DEVICE, fn, "NULL pointer");
+  if (!has_device(c)) return fail(context, QMCKL_FAILURE_DEVICE, fn, "no device bound");
+  hipError_t e = launch_mo_gemm_f64(c->stream, point_num, ao_num, mo_num, ao_value, ao_num,
+                                    coefficient_t, mo_num, mo_value, mo_num);
+  if (e == hipSuccess) e = sync(c);
+  return hip_check(context, e, fn);
+}
+
+}  // extern "C"

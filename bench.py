@@ -6,13 +6,18 @@ mo_vgl are provided again through the C ABI, with the inputs already
 resident in HBM.  Default workload: the 64-water cluster at 128 walkers
 (81,920 electron-points, 192 nuclei, 4,160 AOs, 320 MOs).
 
-Prints one JSON line (rank 0, one GPU).  The value is electron-points/s;
-`roofline` is computed for the kernel that takes the most time, from its
-algorithmic work and its average duration measured with HIP events on the
-context's stream over the timed steps; `cpu_baseline` times the FP64 CPU
-oracle on a bounded sample of the same workload on the host cores.
+The headline form (--form, default fp16x2: FP64 distances and AO assembly,
+split-fp16 MO contraction) is timed next to the FP64 plain form on the same
+context; both are checked against the FP64 oracle on sampled points.
+Prints one JSON line (one GPU).  The value is electron-points/s of the
+headline form; `roofline` is computed for the kernel that takes the most
+time, from its algorithmic work and its average duration measured with HIP
+events on the context's stream over the timed steps; `cpu_baseline` times
+the FP64 CPU oracle on a bounded sample of the same workload on the host
+cores.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -24,8 +29,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0, "f16": 2500.0}
 METRIC = "AO+MO VGL electron-points/s: mixed/batched vs FP64 plain, MO rel err ≤1e-6"
+FORM_IDS = {"plain": 0, "fp32": 1, "fp16x2": 2, "bf16x2": 3, "bf16x3": 4}
+GEMM_KERNEL = {"plain": "mo_gemm_f64", "fp32": "mo_gemm_fp32", "fp16x2": "mo_gemm_fp16x2",
+               "bf16x2": "mo_gemm_bf16x2", "bf16x3": "mo_gemm_bf16x3"}
+# published dense peaks (TFLOP/s) of the matrix-core type each GEMM runs on
+GEMM_PEAK = {"mo_gemm_f64": ("f64", 78.6), "mo_gemm_fp32": ("f32", 157.3),
+             "mo_gemm_fp16x2": ("f16", 2500.0), "mo_gemm_bf16x2": ("bf16", 2500.0),
+             "mo_gemm_bf16x3": ("bf16", 2500.0)}
+PRODUCTS = {"mo_gemm_f64": 1, "mo_gemm_fp32": 1, "mo_gemm_fp16x2": 3, "mo_gemm_bf16x2": 3,
+            "mo_gemm_bf16x3": 6}
 
 
 def log(*a):
@@ -44,23 +57,26 @@ def make_system(name, walkers):
 
 
 def kernel_work(name, s):
-    """(kind, amount per launch, unit) of algorithmic work."""
+    """(bound, algorithmic work per launch) of a kernel: bytes for HBM-bound
+    kernels, flops (10 P A M, logical FMAs counted once) for the MO GEMMs."""
     P, N, A, M = s.point_num, s.nucl_num, s.ao_num, s.mo_num
-    if name == "mo_gemm_f64":
-        return "mfma", 10.0 * P * A * M, "f64"
+    if name in GEMM_PEAK:
+        return "mfma", 10.0 * P * A * M
     if name == "ao_vgl":
-        return "hbm", P * (24.0 + 40.0 * A), None
+        return "hbm", P * (24.0 + 40.0 * A)
+    if name == "ao_vgl_f32":
+        return "hbm", P * (24.0 + 20.0 * A)
     if name == "en_distance":
-        return "hbm", P * (24.0 + 8.0 * N), None
-    return None, 0.0, None
+        return "hbm", P * (24.0 + 8.0 * N)
+    return None, 0.0
 
 
-def run_plain(s, steps, warmup):
-    from qmckl_sycl_amd.context import Context, lib
+def run_form(ctx, s, form, steps, warmup):
+    import torch
+    from qmckl_sycl_amd.context import lib
     L = lib()
-    ctx = Context(0)
-    ctx.set_system(s)
     h = ctx.handle
+    ctx.check(L.qmckl_set_mo_basis_precision_device(h, FORM_IDS[form]))
 
     def step():
         ctx.check(L.qmckl_context_touch_device(h))
@@ -69,7 +85,6 @@ def run_plain(s, steps, warmup):
 
     for _ in range(warmup):
         step()
-    import torch
     torch.cuda.synchronize()
     ctx.check(L.qmckl_reset_kernel_timing_device(h))
     ctx.check(L.qmckl_set_kernel_timing_device(h, True))
@@ -79,35 +94,56 @@ def run_plain(s, steps, warmup):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ctx.check(L.qmckl_set_kernel_timing_device(h, False))
-    import ctypes
     kernels = {}
-    for name in ("en_distance", "ao_vgl", "mo_gemm_f64"):
+    for name in ("en_distance", "ao_vgl", "ao_vgl_f32", GEMM_KERNEL[form]):
         n = ctypes.c_int64(0)
         ms = ctypes.c_double(0.0)
         L.qmckl_get_kernel_timing_device(h, name.encode(), ctypes.byref(n), ctypes.byref(ms))
         if n.value:
-            kernels[name] = {"launches": int(n.value), "avg_ms": ms.value / n.value}
-    # parity spot check of the timed output against the oracle
+            k = {"launches": int(n.value), "avg_ms": ms.value / n.value}
+            kind, w = kernel_work(name, s)
+            if kind == "mfma":
+                k["achieved_tflops"] = w / (k["avg_ms"] / 1e3) / 1e12
+            elif kind == "hbm":
+                k["achieved_gbs"] = w / (k["avg_ms"] / 1e3) / 1e9
+            kernels[name] = k
     mo = ctx.mo_vgl(s.point_num)
-    ctx.destroy()
     return (t1 - t0) / steps, kernels, mo
 
 
-def parity_sample(s, mo, n=64, seed=0):
+def roofline(kernels, s):
+    dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
+    kind, work = kernel_work(dom, s)
+    avg_s = kernels[dom]["avg_ms"] / 1e3
+    if kind == "mfma":
+        prec, peak = GEMM_PEAK[dom]
+        ach = work / avg_s / 1e12
+        return {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
+                "frac": ach / peak, "traffic": None, "kernel": dom, "peak_dtype": prec,
+                "mfma_products_per_fma": PRODUCTS[dom]}
+    ach = work / avg_s / 1e9
+    return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": dom}
+
+
+def errors_vs_oracle(s, mos, n=256, seed=0):
+    """Per-form error on sampled points: for each (point, component) row
+    max_i |d| / max_i |ref|, maximised over rows, for value / gradient /
+    Laplacian."""
     from oracle import oracle
     rng = np.random.default_rng(seed)
     idx = np.sort(rng.choice(s.point_num, size=min(n, s.point_num), replace=False))
     coord = np.ascontiguousarray(s.coord_T()[:, idx])
-    ao, am = oracle.ao_vgl(s, coord, with_mag=True)
-    ref, mm = oracle.mo_vgl_from_ao(s.mo_coef, ao, am)
-    got = mo[idx]
-    d = np.abs(got - ref)
-    rel_mag = (d / np.where(mm > 0, mm, 1.0)).max(axis=(0, 2))
-    rowmax = np.abs(ref).max(axis=2, keepdims=True)
-    rel_row = (d / np.where(rowmax > 0, rowmax, 1.0)).max(axis=(0, 2))
-    return {"points_checked": int(idx.size),
-            "max_err_over_term_sum": [float(x) for x in rel_mag],
-            "max_err_over_row_max": [float(x) for x in rel_row]}
+    ref = oracle.mo_vgl_from_ao(s.mo_coef, oracle.ao_vgl(s, coord))
+    rm = np.abs(ref).max(axis=2, keepdims=True)
+    rm = np.where(rm > 0, rm, 1.0)
+    out = {"points_checked": int(idx.size),
+           "measure": "max over (point, component) rows of max_i|d|/max_i|ref|"}
+    for form, mo in mos.items():
+        e = (np.abs(mo[idx] - ref) / rm).max(axis=(0, 2))
+        out[form] = {"value": float(e[0]), "gradient": float(e[1:4].max()),
+                     "laplacian": float(e[4])}
+    return out
 
 
 def cpu_baseline(s, target_s=10.0):
@@ -141,7 +177,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="water64")
     ap.add_argument("--walkers", type=int, default=0)
-    ap.add_argument("--form", default="plain", choices=["plain"])
+    ap.add_argument("--form", default="fp16x2", choices=list(FORM_IDS))
+    ap.add_argument("--no-plain", action="store_true", help="skip the FP64 plain comparison run")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     args = ap.parse_args()
@@ -150,38 +187,39 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    from qmckl_sycl_amd.context import Context
     s = make_system(args.config, args.walkers)
     log("workload", s.summary())
-    sec, kernels, mo = run_plain(s, args.steps, args.warmup)
-    value = s.point_num / sec
-    dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
-    kind, work, prec = kernel_work(dom, s)
-    avg_s = kernels[dom]["avg_ms"] / 1e3
-    if kind == "mfma":
-        ach = work / avg_s / 1e12
-        roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_TFLOPS[prec], "unit": "TFLOP/s",
-                "frac": ach / PEAK_TFLOPS[prec], "traffic": None, "kernel": dom}
-    else:
-        ach = work / avg_s / 1e9
-        roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": dom}
-    for k, v in kernels.items():
-        kk, w, p = kernel_work(k, s)
-        if kk == "mfma":
-            v["achieved_tflops"] = w / (v["avg_ms"] / 1e3) / 1e12
-        elif kk == "hbm":
-            v["achieved_gbs"] = w / (v["avg_ms"] / 1e3) / 1e9
-    parity = parity_sample(s, mo)
+    ctx = Context(0)
+    ctx.set_system(s)
+    results, mos = {}, {}
+    forms = [args.form] if (args.no_plain or args.form == "plain") else ["plain", args.form]
+    for form in forms:
+        sec, kernels, mo = run_form(ctx, s, form, args.steps, args.warmup)
+        results[form] = {"ms_per_step": sec * 1e3, "value": s.point_num / sec, "kernels": kernels}
+        mos[form] = mo
+        log(form, json.dumps(results[form]))
+    ctx.destroy()
+    head = results[args.form]
     out = {
-        "metric": METRIC, "value": value, "unit": "electron-points/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "metric": METRIC, "value": head["value"], "unit": "electron-points/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.form == "plain" else args.form,
         "data": "synthetic",
         "config": {"workload": f"{s.name}: {s.nucl_num} nuclei, {s.ao_num} AOs, {s.mo_num} MOs, "
                                f"{s.walk_num} walkers x {s.elec_num} electrons",
                    "points": s.point_num, "form": args.form},
-        "roofline": roof, "kernels": kernels, "parity": parity,
+        "roofline": roofline(head["kernels"], s),
+        "kernels": head["kernels"],
+        "mo_error_vs_fp64_oracle": errors_vs_oracle(s, mos),
     }
+    if "plain" in results and args.form != "plain":
+        out["plain_fp64"] = {"value": results["plain"]["value"],
+                             "ms_per_step": results["plain"]["ms_per_step"],
+                             "kernels": results["plain"]["kernels"],
+                             "roofline": roofline(results["plain"]["kernels"], s)}
+        out["speedup_vs_plain_fp64"] = head["value"] / results["plain"]["value"]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(s, args.cpu_seconds)
     print(json.dumps(out), flush=True)

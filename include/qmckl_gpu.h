@@ -323,6 +323,44 @@ qmckl_exit_code_device qmckl_compute_mo_basis_mo_value_device(qmckl_context_devi
                                                               int64_t point_num, double *coefficient_t,
                                                               double *ao_value, double *mo_value);
 
+/* ---- precision form of the MO contraction (new; no counterpart) -----------
+ * FP64 (default) is the plain form.  The others keep FP64 for distances,
+ * exponent arguments and the AO assembly, round the AO operand to FP32, and
+ * run the contraction on the matrix cores:
+ *   FP32    FP32 MFMA;
+ *   FP16X2  power-of-two scaled fp16 hi+lo split of both operands, products
+ *           hi*hi + hi*lo + lo*hi;
+ *   BF16X2  bf16 hi+lo split, the same three products;
+ *   BF16X3  bf16 hi+mid+lo split, six products;
+ * all accumulating in FP32 over short K blocks and in FP64 across them.
+ * Results stay FP64 arrays [P][5][M]; changing the form invalidates mo_vgl. */
+typedef int32_t qmckl_precision_device;
+#define QMCKL_PRECISION_FP64_DEVICE ((qmckl_precision_device)0)
+#define QMCKL_PRECISION_FP32_DEVICE ((qmckl_precision_device)1)
+#define QMCKL_PRECISION_FP16X2_DEVICE ((qmckl_precision_device)2)
+#define QMCKL_PRECISION_BF16X2_DEVICE ((qmckl_precision_device)3)
+#define QMCKL_PRECISION_BF16X3_DEVICE ((qmckl_precision_device)4)
+qmckl_exit_code_device qmckl_set_mo_basis_precision_device(qmckl_context_device context,
+                                                           qmckl_precision_device form);
+qmckl_exit_code_device qmckl_get_mo_basis_precision_device(qmckl_context_device context,
+                                                           qmckl_precision_device *form);
+
+/* ---- differentiation form (new; no counterpart in the reference) ----------
+ * Adjoint of L = sum_{p,c,i} weight[p][c][i] * mo_vgl[p][c][i] (FP64) for the
+ * current points: d_nucl_coord [N][3] (dL/dR), d_exponent [prim_num]
+ * (dL/dalpha at fixed coefficient, prim_factor and shell_factor),
+ * d_mo_coefficient [mo_num][ao_num] (the layout of
+ * qmckl_set_mo_basis_coefficient_device).  weight is a device array
+ * [P][5][mo_num] (size_max >= 5*P*mo_num); any output may be NULL.  AO
+ * primitives are recomputed, not stored; reductions over points run in a
+ * fixed order (bitwise reproducible). */
+qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_adjoint_device(qmckl_context_device context,
+                                                                const double *weight,
+                                                                int64_t size_max,
+                                                                double *d_nucl_coord,
+                                                                double *d_exponent,
+                                                                double *d_mo_coefficient);
+
 /* ---- kernel timing (new; no counterpart in the reference) ----------------
  * When enabled, every kernel the context launches is bracketed by HIP events
  * on the context's stream and its elapsed time is accumulated under the

@@ -26,6 +26,7 @@ class MemoryInfo(ctypes.Structure):
 _SCALARS = {
     "qmckl_exit_code_device": ctypes.c_int32,
     "qmckl_context_device": ctypes.c_int64,
+    "qmckl_precision_device": ctypes.c_int32,
     "bool": ctypes.c_bool,
     "int64_t": ctypes.c_int64,
     "int32_t": ctypes.c_int32,

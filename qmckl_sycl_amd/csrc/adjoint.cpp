@@ -1,0 +1,90 @@
+// Differentiation form: adjoint of a weighted sum of mo_vgl with respect to
+// nuclear coordinates, primitive exponents and MO coefficients.
+//   G      = w C                  (FP64 MFMA GEMM, [5P][A])
+//   dL/dC  = w^T AO               (FP64 MFMA GEMM on the transposed weights,
+//                                  split over points, fixed-order reduction)
+//   dL/dR, dL/dalpha              (AO adjoint kernel recomputing primitives,
+//                                  per-block partials, fixed-order reduction)
+// There is no counterpart in the reference; the forward arithmetic it
+// differentiates is src/qmckl_ao_sycl.cpp:40-95 and 291-422 and
+// src/qmckl_mo_sycl.cpp:23-50.
+#include "internal.hpp"
+
+using namespace qmckl_amd;
+
+namespace {
+
+bool grow_d(Context *c, double **slot, int64_t *cap, int64_t n) {
+  return grow_bytes(c, reinterpret_cast<void **>(slot), cap, n * (int64_t)sizeof(double));
+}
+
+}  // namespace
+
+extern "C" {
+
+qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_adjoint_device(qmckl_context_device context,
+                                                                const double *weight,
+                                                                int64_t size_max,
+                                                                double *d_nucl_coord,
+                                                                double *d_exponent,
+                                                                double *d_mo_coefficient) {
+  Context *c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+  const char *fn = "qmckl_get_mo_basis_mo_vgl_adjoint_device";
+  if (weight == nullptr) return fail(context, QMCKL_INVALID_ARG_2_DEVICE, fn, "NULL weight");
+  if (!c->mo.provided || !c->ao.provided || !c->nucleus.provided)
+    return fail(context, QMCKL_NOT_PROVIDED_DEVICE, fn, nullptr);
+  if (c->ao.type != 'G') return fail(context, QMCKL_FAILURE_DEVICE, fn, "basis type != 'G'");
+  if (c->point.num <= 0 || c->point.coord == nullptr)
+    return fail(context, QMCKL_NOT_PROVIDED_DEVICE, fn, "no points set");
+  const int64_t P = c->point.num, A = c->ao.ao_num, M = c->mo.mo_num, N = c->nucleus.num;
+  const int64_t S = c->ao.shell_num, Np = c->ao.prim_num;
+  if (size_max < 5 * P * M) return fail(context, QMCKL_INVALID_ARG_3_DEVICE, fn, "weight array too small");
+  if (!has_device(c)) return fail(context, QMCKL_FAILURE_DEVICE, fn, "no device bound");
+  MoBasis &mo = c->mo;
+  hipError_t e = hipSuccess;
+  if (d_nucl_coord != nullptr || d_exponent != nullptr) {
+    const int64_t nblk = ao_adjoint_blocks(P);
+    if (!grow_d(c, &mo.adj_G, &mo.adj_G_cap, 5 * P * A) ||
+        !grow_d(c, &mo.adj_partR, &mo.adj_partR_cap, nblk * S * 3) ||
+        !grow_d(c, &mo.adj_partA, &mo.adj_partA_cap, nblk * Np))
+      return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
+    {
+      KernelTimer t(c, "adjoint_g_gemm_f64");
+      e = launch_mo_gemm_f64(c->stream, 5 * P, M, A, weight, M, mo.coefficient, A, mo.adj_G, A);
+      t.stop();
+    }
+    // primitives that belong to no shell keep a zero partial
+    if (e == hipSuccess) e = hipMemsetAsync(mo.adj_partA, 0, sizeof(double) * nblk * Np, c->stream);
+    if (e == hipSuccess) {
+      KernelTimer t(c, "ao_adjoint");
+      e = launch_ao_adjoint(c->stream, c->ao.dev, P, c->point.coord, c->nucleus.coord, c->ao.ao_factor,
+                            mo.adj_G, mo.adj_partR, mo.adj_partA);
+      t.stop();
+    }
+    if (e == hipSuccess)
+      e = launch_adjoint_reduce(c->stream, c->ao.dev, nblk, c->ao.nucleus_index, c->ao.nucleus_shell_num,
+                                mo.adj_partR, mo.adj_partA, d_nucl_coord, d_exponent);
+    if (e == hipSuccess) e = sync(c);
+    if (e != hipSuccess) return hip_check(context, e, fn);
+    (void)N;
+  }
+  if (d_mo_coefficient != nullptr) {
+    qmckl_exit_code_device rc = qmckl_provide_ao_basis_ao_vgl_device(context);
+    if (rc != QMCKL_SUCCESS_DEVICE) return rc;
+    const int splits = gemm_tn_splits(5 * P);
+    if (splits > 1 && !grow_d(c, &mo.adj_partC, &mo.adj_partC_cap, (int64_t)splits * M * A))
+      return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
+    {
+      KernelTimer t(c, "adjoint_dc_gemm_f64");
+      e = launch_gemm_tn_f64(c->stream, M, 5 * P, A, weight, M, c->ao.ao_vgl, A, d_mo_coefficient, A,
+                             mo.adj_partC, splits);
+      t.stop();
+    }
+    if (e == hipSuccess) e = sync(c);
+    if (e != hipSuccess) return hip_check(context, e, fn);
+  }
+  return QMCKL_SUCCESS_DEVICE;
+}
+
+}  // extern "C"

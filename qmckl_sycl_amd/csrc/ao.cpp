@@ -213,9 +213,16 @@ qmckl_exit_code_device qmckl_finalize_ao_basis_device(qmckl_context_device conte
     free_tables(c, ao.dev);
     return fail(context, QMCKL_FAILURE_DEVICE, fn, msg.c_str());
   }
-  // A new basis invalidates every cached AO/MO result.
+  // A new basis invalidates every cached AO/MO result and the mixed-form
+  // operands (their padded row length depends on ao_num).
   ao.shell_vgl_date = ao.ao_vgl_date = ao.ao_value_date = 0;
   c->mo.mo_vgl_date = c->mo.mo_value_date = 0;
+  dfree(c, ao.ao_f32);
+  ao.ao_f32 = nullptr;
+  ao.ao_f32_cap = 0;
+  dfree(c, c->mo.coef_split);
+  c->mo.coef_split = nullptr;
+  c->mo.coef_split_form = QMCKL_PRECISION_FP64_DEVICE;
   return QMCKL_SUCCESS_DEVICE;
 }
 
@@ -529,12 +536,13 @@ qmckl_exit_code_device qmckl_provide_ao_basis_shell_vgl_device(qmckl_context_dev
 // Compute ao_vgl (ncomp = 5) or ao_value (ncomp = 1) for the current points
 // into `dst`.
 static qmckl_exit_code_device compute_ao(qmckl_context_device context, Context *c, double *dst,
-                                         int ncomp, const char *fn) {
+                                         int ncomp, const char *fn, uint8_t *flags = nullptr,
+                                         int nkb = 0) {
   hipError_t e;
   {
     KernelTimer t(c, ncomp == 5 ? "ao_vgl" : "ao_value");
     e = launch_ao_vgl(c->stream, c->ao.dev, c->point.num, c->point.coord, c->nucleus.coord,
-                      c->ao.ao_factor, dst, ncomp);
+                      c->ao.ao_factor, dst, ncomp, flags, nkb);
     t.stop();
   }
   if (e == hipSuccess) e = sync(c);
@@ -551,7 +559,17 @@ qmckl_exit_code_device qmckl_provide_ao_basis_ao_vgl_device(qmckl_context_device
     const int64_t need = c->point.num * 5 * c->ao.ao_num;
     if (!grow(c, &c->ao.ao_vgl, &c->ao.ao_vgl_cap, need))
       return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
-    rc = compute_ao(context, c, c->ao.ao_vgl, 5, fn);
+    // nonzero flags per (64 rows, 32 AOs) let the MO contraction skip
+    // screened blocks
+    const int nkb = (int)((c->ao.ao_num + 31) / 32);
+    uint8_t *flags = nullptr;
+    if (nkb <= kMaxFlagBlocks) {
+      const int64_t nflags = (5 * c->point.num + 63) / 64 * nkb;
+      if (!grow_bytes(c, reinterpret_cast<void **>(&c->ao.ao_vgl_flags), &c->ao.ao_vgl_flags_cap, nflags))
+        return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
+      flags = c->ao.ao_vgl_flags;
+    }
+    rc = compute_ao(context, c, c->ao.ao_vgl, 5, fn, flags, flags ? nkb : 0);
     if (rc != QMCKL_SUCCESS_DEVICE) return rc;
     c->ao.ao_vgl_date = c->date;
   }

@@ -33,6 +33,16 @@ qmckl_exit_code_device fail(qmckl_context_device context, qmckl_exit_code_device
 
 bool has_device(const Context *c) { return c->device_id >= 0 && c->stream != nullptr; }
 
+bool grow_bytes(Context *c, void **slot, int64_t *cap, int64_t need) {
+  if (*slot != nullptr && *cap >= need) return true;
+  if (!drealloc(c, slot, (size_t)(need > 0 ? need : 1))) {
+    *cap = 0;
+    return false;
+  }
+  *cap = need;
+  return true;
+}
+
 void *dmalloc(Context *c, size_t bytes) {
   if (!has_device(c)) return nullptr;
   if (bytes == 0) bytes = 1;

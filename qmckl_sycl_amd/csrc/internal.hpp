@@ -13,6 +13,11 @@
 
 #include "../../include/qmckl_gpu.h"
 
+#define QMCKL_AMD_SPLIT_FP32 QMCKL_PRECISION_FP32_DEVICE
+#define QMCKL_AMD_SPLIT_FP16X2 QMCKL_PRECISION_FP16X2_DEVICE
+#define QMCKL_AMD_SPLIT_BF16X2 QMCKL_PRECISION_BF16X2_DEVICE
+#define QMCKL_AMD_SPLIT_BF16X3 QMCKL_PRECISION_BF16X3_DEVICE
+
 namespace qmckl_amd {
 
 constexpr uint64_t kValidTag = 0xBEEFFACEull;    // include/qmckl_types.hpp:12
@@ -30,6 +35,18 @@ struct ShellDesc {
   int32_t ao_start;    // first AO
   int32_t l;           // angular momentum
   int32_t shell;       // index in the user's shell arrays
+};
+
+// Output of the AO kernel: values at out[(p*ncomp + c)*ld + k]; optional
+// nonzero flags per (64-row tile of the (p,c) rows, 32-AO block) and optional
+// max |value| per (p,c) row.
+template <typename OutT>
+struct AoOut {
+  OutT *out;
+  int64_t ld;
+  uint8_t *flags;  // [ceil(ncomp*P/64)][nkb] or nullptr
+  int nkb;
+  float *rowmax;   // [ncomp*P] or nullptr
 };
 
 // Basis tables on the device (owned by the context, rebuilt by finalize).
@@ -95,6 +112,17 @@ struct AoBasis {
   double *ao_value = nullptr;   // [P][A]
   int64_t ao_value_cap = 0;
   uint64_t ao_value_date = 0;
+  // nonzero flags of ao_vgl per (64 rows of [5P], 32 AOs), written with it
+  uint8_t *ao_vgl_flags = nullptr;
+  int64_t ao_vgl_flags_cap = 0;
+  // FP32 AO operand of the mixed contraction: [5P][ld32], ld32 = A rounded
+  // up to 32 (padding zero), its row maxima and nonzero flags
+  float *ao_f32 = nullptr;
+  int64_t ao_f32_cap = 0;
+  float *ao_f32_rowmax = nullptr;
+  int64_t ao_f32_rowmax_cap = 0;
+  uint8_t *ao_f32_flags = nullptr;
+  int64_t ao_f32_flags_cap = 0;
   int32_t uninitialized = (1 << 14) - 1;
   bool provided = false;
 };
@@ -109,6 +137,22 @@ struct MoBasis {
   double *mo_value = nullptr;       // [P][M]
   int64_t mo_value_cap = 0;
   uint64_t mo_value_date = 0;
+  qmckl_precision_device precision = QMCKL_PRECISION_FP64_DEVICE;
+  int flush_blocks = 2;             // 32-wide K blocks per FP32 partial sum
+  // MO coefficients prepared for the mixed forms: [terms][M][ld32] 16-bit
+  // split terms (or [M][ld32] FP32), inverse column scales [M]
+  uint16_t *coef_split = nullptr;
+  float *coef_colinv = nullptr;
+  qmckl_precision_device coef_split_form = QMCKL_PRECISION_FP64_DEVICE;
+  // adjoint scratch: G = w C [5P][A], per-block partials, split-K partials
+  double *adj_G = nullptr;
+  int64_t adj_G_cap = 0;
+  double *adj_partR = nullptr;
+  int64_t adj_partR_cap = 0;
+  double *adj_partA = nullptr;
+  int64_t adj_partA_cap = 0;
+  double *adj_partC = nullptr;
+  int64_t adj_partC_cap = 0;
   int32_t uninitialized = (1 << 2) - 1;
   bool provided = false;
 };
@@ -166,6 +210,9 @@ void *dmalloc(Context *c, size_t bytes);
 qmckl_exit_code_device dfree(Context *c, void *ptr);
 // Replace *slot by a fresh allocation of `bytes` (freeing the old one).
 bool drealloc(Context *c, void **slot, size_t bytes);
+// Grow-only buffer: reallocates *slot when cap < need (bytes).
+bool grow_bytes(Context *c, void **slot, int64_t *cap, int64_t need);
+constexpr int kMaxFlagBlocks = 2048;  // 32-AO blocks the AO kernel can flag (65,536 AOs)
 bool has_device(const Context *c);
 hipError_t sync(Context *c);
 qmckl_exit_code_device hip_check(qmckl_context_device context, hipError_t e, const char *function);
@@ -195,14 +242,39 @@ hipError_t launch_shell_vgl(hipStream_t s, const DeviceBasis &b, int64_t P, cons
                             const double *nucl, double *shell_vgl);
 hipError_t launch_ao_vgl(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
                          const double *nucl, const double *ao_factor, double *ao_vgl,
-                         int ncomp);
+                         int ncomp, uint8_t *flags = nullptr, int nkb = 0);
+hipError_t launch_ao_vgl_f32(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
+                             const double *nucl, const double *ao_factor, float *out, int64_t ld,
+                             float *rowmax, uint8_t *flags, int nkb);
 hipError_t launch_ao_from_shell_table(hipStream_t s, const DeviceBasis &b, int64_t P,
                                       const double *coord, const double *nucl,
                                       const double *ao_factor, const double *shell_vgl,
                                       int64_t S_table, double *out, int ncomp);
 hipError_t launch_mo_gemm_f64(hipStream_t s, int64_t rows, int64_t K, int64_t Ncol,
                               const double *A, int64_t lda, const double *B, int64_t ldb,
-                              double *C, int64_t ldc);
+                              double *C, int64_t ldc, const uint8_t *flags = nullptr, int nkb = 0);
+// C[R][Ncol] = A^T B with A stored [K][lda] (rows of A^T contiguous along
+// K-major storage), split over K into `splits` partials (partial holds
+// splits*R*Ncol doubles) reduced in a fixed order.
+int gemm_tn_splits(int64_t K);
+hipError_t launch_gemm_tn_f64(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const double *A,
+                              int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc,
+                              double *partial, int splits);
+hipError_t launch_ao_adjoint(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
+                             const double *nucl, const double *ao_factor, const double *G,
+                             double *partR, double *partA);
+int64_t ao_adjoint_blocks(int64_t P);
+hipError_t launch_adjoint_reduce(hipStream_t s, const DeviceBasis &b, int64_t nblk,
+                                 const int64_t *nucleus_index, const int64_t *nucleus_shell_num,
+                                 const double *partR, const double *partA, double *dR, double *dA);
+int split_terms(int form);
+hipError_t launch_split_coef(hipStream_t s, int form, int64_t M, int64_t K, const double *coef,
+                             int64_t ldb, uint16_t *B, float *colinv);
+hipError_t launch_mo_gemm_mixed(hipStream_t s, int form, int64_t R, int64_t K, int64_t Ncol,
+                                const float *A, int64_t lda, const float *rowmax, const void *B,
+                                int64_t ldb, const float *colinv, const uint8_t *flags, int nkb,
+                                int flush, double *C, int64_t ldc);
+inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 hipError_t launch_strided_copy(hipStream_t s, int64_t rows, int64_t cols, const double *in,
                                int64_t ld_in, double *out, int64_t ld_out);
 

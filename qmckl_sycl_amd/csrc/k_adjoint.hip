@@ -1,0 +1,219 @@
+// Adjoint of L = sum_{p,c,i} w[p][c][i] mo_vgl[p][c][i] with respect to the
+// nuclear coordinates and the primitive exponents (FP64).
+//
+// With G[p][c][k] = sum_i w[p][c][i] C[i][k] (one GEMM, done by the caller),
+// L depends on a nucleus and an exponent only through the AOs:
+//   dL/dR_n     = - sum_p sum_{k on n} sum_c G[p][c][k] d/dr AO_c[p][k]
+//   dL/dalpha_j =   sum_p sum_{k in shell(j)} sum_c G[p][c][k] dAO_c/dalpha_j
+// (AO_c depends on r - R_n; the exponent derivative holds the normalised
+// contraction coefficient fixed, as the basis arrays are independent inputs).
+// AO_k = f P(d) g(s), d = r - R_n, s = |d|^2, g = sum_j c_j e^{-a_j s}:
+//   d_i g = 2 d_i g', d_ij g = 4 d_i d_j g'' + 2 delta_ij g',
+//   lap g = 4 s g'' + 6 g', d_j lap g = 2 d_j (10 g'' + 4 s g'''),
+// and the exponent derivative replaces g by h_j = -s c_j e^{-a_j s}
+// (h' = -c e^{-as}(1 - a s), h'' = c a e^{-as}(2 - a s)), which enters
+// linearly through three per-shell sums U0, U1, U2.
+//
+// The primitives are recomputed here, not stored by the forward pass.
+// Work: workgroup = 64 points, wavefront = one shell at a time, lane =
+// point; per (point, shell) the lane evaluates g..g''', loops over the
+// shell's AOs (monomials to third order), and reduces over the wavefront in
+// a fixed shuffle order.  Each (workgroup, shell) writes one partial dR and
+// each (workgroup, primitive) one partial dalpha; a second kernel sums the
+// partials in a fixed order, so the result is bitwise reproducible.
+#include <hip/hip_runtime.h>
+
+#include "internal.hpp"
+
+namespace qmckl_amd {
+
+namespace {
+
+constexpr int kTP = 64;
+constexpr int kThreads = 256;
+
+// x^a and its first three derivatives: (f, f', f'', f''') ->
+// (x f, f + x f', 2 f' + x f'', 3 f'' + x f''').
+__device__ inline void pow4(double x, int a, double *p) {
+  double f0 = 1.0, f1 = 0.0, f2 = 0.0, f3 = 0.0;
+  for (int i = 0; i < a; ++i) {
+    const double n3 = 3.0 * f2 + x * f3;
+    const double n2 = 2.0 * f1 + x * f2;
+    const double n1 = f0 + x * f1;
+    f0 = x * f0;
+    f1 = n1;
+    f2 = n2;
+    f3 = n3;
+  }
+  p[0] = f0;
+  p[1] = f1;
+  p[2] = f2;
+  p[3] = f3;
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__global__ void __launch_bounds__(kThreads)
+ao_adjoint_kernel(int64_t P, int S, int A, int N, const double *__restrict__ coord,
+                  const double *__restrict__ nucl, const ShellDesc *__restrict__ shells,
+                  const int32_t *__restrict__ ao_abc, const double *__restrict__ ao_factor,
+                  const double *__restrict__ expo, const double *__restrict__ coef,
+                  const double *__restrict__ nucl_range, const double *__restrict__ G,
+                  double *__restrict__ partR, double *__restrict__ partA, int64_t Np) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t blk = blockIdx.x;
+  const int64_t p = blk * kTP + lane;
+  const bool inb = p < P;
+  const double rx = inb ? coord[p] : 0.0;
+  const double ry = inb ? coord[P + p] : 0.0;
+  const double rz = inb ? coord[2 * P + p] : 0.0;
+  double *myR = partR + blk * (int64_t)S * 3;
+  double *myA = partA + blk * Np;
+
+  for (int si = wave; si < S; si += kThreads / 64) {
+    const ShellDesc d = shells[si];
+    const double dx = rx - nucl[d.nucl];
+    const double dy = ry - nucl[N + d.nucl];
+    const double dz = rz - nucl[2 * N + d.nucl];
+    const double s = fma(dx, dx, fma(dy, dy, dz * dz));
+    const bool live = inb && !(s > kCutoff * nucl_range[d.nucl]);
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
+    for (int j = 0; j < d.prim_num; ++j) {
+      const double a = expo[d.prim_start + j];
+      const double as = a * s;
+      if (live && !(as > kCutoff)) {
+        const double v = coef[d.prim_start + j] * exp(-as);
+        g0 += v;
+        g1 -= a * v;
+        g2 += a * a * v;
+        g3 -= a * a * a * v;
+      }
+    }
+    double gR0 = 0.0, gR1 = 0.0, gR2 = 0.0, U0 = 0.0, U1 = 0.0, U2 = 0.0;
+    const int ncart = (d.l + 1) * (d.l + 2) / 2;
+    if (__any(live)) {
+      const double dd[3] = {dx, dy, dz};
+      const double gi[3] = {2.0 * dx * g1, 2.0 * dy * g1, 2.0 * dz * g1};
+      const double lapg = 4.0 * s * g2 + 6.0 * g1;
+      const double t3 = 2.0 * (10.0 * g2 + 4.0 * s * g3);
+      for (int kk = 0; kk < ncart; ++kk) {
+        const int k = d.ao_start + kk;
+        const int abc = ao_abc[k];
+        double X[4], Y[4], Z[4];
+        pow4(dx, abc & 0xff, X);
+        pow4(dy, (abc >> 8) & 0xff, Y);
+        pow4(dz, (abc >> 16) & 0xff, Z);
+        const double m0 = X[0] * Y[0] * Z[0];
+        const double Pi[3] = {X[1] * Y[0] * Z[0], X[0] * Y[1] * Z[0], X[0] * Y[0] * Z[1]};
+        const double lapP = X[2] * Y[0] * Z[0] + X[0] * Y[2] * Z[0] + X[0] * Y[0] * Z[2];
+        const double Pij[3][3] = {{X[2] * Y[0] * Z[0], X[1] * Y[1] * Z[0], X[1] * Y[0] * Z[1]},
+                                  {X[1] * Y[1] * Z[0], X[0] * Y[2] * Z[0], X[0] * Y[1] * Z[1]},
+                                  {X[1] * Y[0] * Z[1], X[0] * Y[1] * Z[1], X[0] * Y[0] * Z[2]}};
+        const double dlapP[3] = {X[3] * Y[0] * Z[0] + X[1] * Y[2] * Z[0] + X[1] * Y[0] * Z[2],
+                                 X[2] * Y[1] * Z[0] + X[0] * Y[3] * Z[0] + X[0] * Y[1] * Z[2],
+                                 X[2] * Y[0] * Z[1] + X[0] * Y[2] * Z[1] + X[0] * Y[0] * Z[3]};
+        const double f = ao_factor[k];
+        double w[5];
+        const double *Gp = G + (p * 5) * (int64_t)A + k;
+#pragma unroll
+        for (int c = 0; c < 5; ++c) w[c] = live ? Gp[(int64_t)c * A] * f : 0.0;
+        // r-gradient of the five components, contracted with w
+        double gR[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          double gij[3];
+#pragma unroll
+          for (int i = 0; i < 3; ++i) gij[i] = 4.0 * dd[i] * dd[q] * g2 + (i == q ? 2.0 * g1 : 0.0);
+          double acc = w[0] * (Pi[q] * g0 + m0 * gi[q]);
+          double cross = 0.0;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            acc += w[1 + i] * (Pij[i][q] * g0 + Pi[i] * gi[q] + Pi[q] * gi[i] + m0 * gij[i]);
+            cross += Pij[i][q] * gi[i] + Pi[i] * gij[i];
+          }
+          acc += w[4] * (dlapP[q] * g0 + lapP * gi[q] + Pi[q] * lapg + m0 * dd[q] * t3 + 2.0 * cross);
+          gR[q] = acc;
+        }
+        gR0 += gR[0];
+        gR1 += gR[1];
+        gR2 += gR[2];
+        const double md = Pi[0] * dx + Pi[1] * dy + Pi[2] * dz;
+        U0 += w[0] * m0 + w[1] * Pi[0] + w[2] * Pi[1] + w[3] * Pi[2] + w[4] * lapP;
+        U1 += 2.0 * m0 * (w[1] * dx + w[2] * dy + w[3] * dz) + w[4] * (6.0 * m0 + 4.0 * md);
+        U2 += 4.0 * s * w[4] * m0;
+      }
+    }
+    gR0 = wave_sum(gR0);
+    gR1 = wave_sum(gR1);
+    gR2 = wave_sum(gR2);
+    if (lane == 0) {
+      myR[(int64_t)d.shell * 3 + 0] = -gR0;
+      myR[(int64_t)d.shell * 3 + 1] = -gR1;
+      myR[(int64_t)d.shell * 3 + 2] = -gR2;
+    }
+    for (int j = 0; j < d.prim_num; ++j) {
+      const double a = expo[d.prim_start + j];
+      const double as = a * s;
+      double v = 0.0;
+      if (live && !(as > kCutoff)) {
+        const double e = coef[d.prim_start + j] * exp(-as);
+        v = -s * e * U0 - e * (1.0 - as) * U1 + e * a * (2.0 - as) * U2;
+      }
+      v = wave_sum(v);
+      if (lane == 0) myA[d.prim_start + j] = v;
+    }
+  }
+}
+
+// dR[n][q] = sum_b sum_{s of n} partR[b][s][q]; dA[j] = sum_b partA[b][j]
+__global__ void __launch_bounds__(kThreads)
+adjoint_reduce_kernel(int64_t nblk, int S, int N, int64_t Np, const int64_t *__restrict__ nucleus_index,
+                      const int64_t *__restrict__ nucleus_shell_num, const double *__restrict__ partR,
+                      const double *__restrict__ partA, double *__restrict__ dR, double *__restrict__ dA) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (dR != nullptr && t < 3 * (int64_t)N) {
+    const int64_t n = t / 3, q = t - 3 * n;
+    double acc = 0.0;
+    for (int64_t b = 0; b < nblk; ++b)
+      for (int64_t s = nucleus_index[n]; s < nucleus_index[n] + nucleus_shell_num[n]; ++s)
+        acc += partR[(b * S + s) * 3 + q];
+    dR[t] = acc;
+  }
+  if (dA != nullptr && t < Np) {
+    double acc = 0.0;
+    for (int64_t b = 0; b < nblk; ++b) acc += partA[b * Np + t];
+    dA[t] = acc;
+  }
+}
+
+}  // namespace
+
+hipError_t launch_ao_adjoint(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
+                             const double *nucl, const double *ao_factor, const double *G,
+                             double *partR, double *partA) {
+  if (P <= 0) return hipSuccess;
+  const int64_t blocks = (P + kTP - 1) / kTP;
+  hipLaunchKernelGGL(ao_adjoint_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, P,
+                     (int)b.shell_num, (int)b.ao_num, (int)b.nucl_num, coord, nucl, b.shells, b.ao_abc,
+                     ao_factor, b.expo, b.coef_norm, b.nucl_range, G, partR, partA, b.prim_num);
+  return hipGetLastError();
+}
+
+int64_t ao_adjoint_blocks(int64_t P) { return (P + kTP - 1) / kTP; }
+
+hipError_t launch_adjoint_reduce(hipStream_t s, const DeviceBasis &b, int64_t nblk,
+                                 const int64_t *nucleus_index, const int64_t *nucleus_shell_num,
+                                 const double *partR, const double *partA, double *dR, double *dA) {
+  const int64_t n = 3 * b.nucl_num > b.prim_num ? 3 * b.nucl_num : b.prim_num;
+  const int64_t g = (n + kThreads - 1) / kThreads;
+  hipLaunchKernelGGL(adjoint_reduce_kernel, dim3((unsigned)(g > 0 ? g : 1)), dim3(kThreads), 0, s, nblk,
+                     (int)b.shell_num, (int)b.nucl_num, b.prim_num, nucleus_index, nucleus_shell_num,
+                     partR, partA, dR, dA);
+  return hipGetLastError();
+}
+
+}  // namespace qmckl_amd

@@ -1,4 +1,4 @@
-// Shell/AO VGL kernels (FP64).
+// Shell/AO VGL kernels (FP64 arithmetic; FP64 or FP32 stores).
 //
 // Arithmetic of src/qmckl_ao_sycl.cpp: radial shell sums with the nucleus
 // screen r^2 > cutoff*range and the primitive screen alpha r^2 > cutoff
@@ -73,14 +73,17 @@ __device__ inline Mono monomial(int abc, double x, double y, double z) {
 
 // NCOMP = 5 (VGL) or 1 (value).  FROM_TABLE: radial parts are read from a
 // precomputed shell_vgl [P][5][S] instead of being summed from primitives.
-template <int NCOMP, bool FROM_TABLE>
+// OutT = double (the plain form) or float (the AO operand of the mixed MO
+// contraction: the FP64 value rounded once at the store).
+template <int NCOMP, bool FROM_TABLE, typename OutT>
 __global__ void __launch_bounds__(kThreads)
 ao_kernel(int64_t P, int S, int A, int N, const double *__restrict__ coord,
           const double *__restrict__ nucl, const ShellDesc *__restrict__ shells,
           const int32_t *__restrict__ ao_shell, const int32_t *__restrict__ ao_abc,
           const double *__restrict__ ao_factor, const double *__restrict__ expo,
           const double *__restrict__ coef, const double *__restrict__ nucl_range,
-          const double *__restrict__ shell_table, int S_table, double *__restrict__ out) {
+          const double *__restrict__ shell_table, int S_table, AoOut<OutT> ao) {
+  extern __shared__ unsigned char s_dyn[];  // [5 row tiles][nkb] nonzero flags
   __shared__ double rad[kShellChunk][NCOMP][kTP];
   __shared__ double pt[3][kTP];
   __shared__ double2 prim[kPrimLds];
@@ -92,6 +95,7 @@ ao_kernel(int64_t P, int S, int A, int N, const double *__restrict__ coord,
   const int64_t p0 = (int64_t)blockIdx.x * kTP;
   const int np = (int)((P - p0) < kTP ? (P - p0) : kTP);
 
+  __shared__ unsigned s_rowmax[kTP * NCOMP];  // float bits of max |value| per (point, component)
   if (tid < kTP) {
     const int64_t p = p0 + tid;
     const bool ok = tid < np;
@@ -99,6 +103,11 @@ ao_kernel(int64_t P, int S, int A, int N, const double *__restrict__ coord,
     pt[1][tid] = ok ? coord[P + p] : 0.0;
     pt[2][tid] = ok ? coord[2 * (int64_t)P + p] : 0.0;
   }
+  const bool track_flags = ao.flags != nullptr;
+  if (track_flags)
+    for (int i = tid; i < NCOMP * ao.nkb; i += kThreads) s_dyn[i] = 0;
+  if (ao.rowmax != nullptr)
+    for (int i = tid; i < kTP * NCOMP; i += kThreads) s_rowmax[i] = 0u;
 
   for (int s0 = 0; s0 < S; s0 += kShellChunk) {
     const int ns = (S - s0) < kShellChunk ? (S - s0) : kShellChunk;
@@ -183,19 +192,39 @@ ao_kernel(int64_t P, int S, int A, int N, const double *__restrict__ coord,
       const double z = pt[2][pl] - nucl[2 * N + n];
       const Mono m = monomial(ao_abc[k], x, y, z);
       const double f = ao_factor[k];
-      double *o = out + (p0 + pl) * (int64_t)NCOMP * A + k;
       const double R = rad[sl][0][pl];
-      if (NCOMP == 1) {
-        o[0] = m.v * R * f;
-      } else {
+      double v[NCOMP];
+      v[0] = m.v * R * f;
+      if (NCOMP == 5) {
         const double Rx = rad[sl][1 % NCOMP][pl], Ry = rad[sl][2 % NCOMP][pl];
         const double Rz = rad[sl][3 % NCOMP][pl], RL = rad[sl][4 % NCOMP][pl];
-        o[0] = m.v * R * f;
-        o[A] = (m.gx * R + m.v * Rx) * f;
-        o[2 * (int64_t)A] = (m.gy * R + m.v * Ry) * f;
-        o[3 * (int64_t)A] = (m.gz * R + m.v * Rz) * f;
-        o[4 * (int64_t)A] = (m.lap * R + m.v * RL + 2.0 * (m.gx * Rx + m.gy * Ry + m.gz * Rz)) * f;
+        v[1 % NCOMP] = (m.gx * R + m.v * Rx) * f;
+        v[2 % NCOMP] = (m.gy * R + m.v * Ry) * f;
+        v[3 % NCOMP] = (m.gz * R + m.v * Rz) * f;
+        v[4 % NCOMP] = (m.lap * R + m.v * RL + 2.0 * (m.gx * Rx + m.gy * Ry + m.gz * Rz)) * f;
       }
+      OutT *o = ao.out + ((p0 + pl) * NCOMP) * ao.ld + k;
+#pragma unroll
+      for (int c = 0; c < NCOMP; ++c) {
+        const OutT w = (OutT)v[c];
+        o[c * ao.ld] = w;
+        if (ao.rowmax != nullptr)
+          atomicMax(&s_rowmax[pl * NCOMP + c], __float_as_uint(fabsf((float)w)));
+        if (track_flags && w != (OutT)0) s_dyn[((pl * NCOMP + c) >> 6) * ao.nkb + (k >> 5)] = 1;
+      }
+    }
+  }
+  __syncthreads();
+  if (ao.rowmax != nullptr)
+    for (int i = tid; i < np * NCOMP; i += kThreads)
+      ao.rowmax[p0 * NCOMP + i] = __uint_as_float(s_rowmax[i]);
+  if (track_flags) {
+    // this workgroup owns rows [NCOMP*p0, NCOMP*(p0+np)): NCOMP whole 64-row tiles
+    const int64_t rt0 = (p0 * NCOMP) >> 6;
+    const int64_t rt_end = (NCOMP * (p0 + np) + 63) >> 6;
+    for (int i = tid; i < NCOMP * ao.nkb; i += kThreads) {
+      const int64_t rt = rt0 + i / ao.nkb;
+      if (rt < rt_end) ao.flags[rt * ao.nkb + (i % ao.nkb)] = s_dyn[i];
     }
   }
 }
@@ -237,41 +266,47 @@ shell_vgl_kernel(int64_t P, int S, int N, const double *__restrict__ coord,
 
 }  // namespace
 
-hipError_t launch_ao_vgl(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
-                         const double *nucl, const double *ao_factor, double *out, int ncomp) {
+template <int NCOMP, bool FROM_TABLE, typename OutT>
+hipError_t launch_ao(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
+                     const double *nucl, const double *ao_factor, const double *shell_table,
+                     int64_t S_table, AoOut<OutT> ao) {
   if (P <= 0) return hipSuccess;
   const int64_t blocks = (P + kTP - 1) / kTP;
   if (blocks > 0x7fffffff) return hipErrorInvalidValue;
-  if (ncomp == 5)
-    hipLaunchKernelGGL((ao_kernel<5, false>), dim3((unsigned)blocks), dim3(kThreads), 0, s, P,
-                       (int)b.shell_num, (int)b.ao_num, (int)b.nucl_num, coord, nucl, b.shells,
-                       b.ao_shell, b.ao_abc, ao_factor, b.expo, b.coef_norm, b.nucl_range,
-                       (const double *)nullptr, 0, out);
-  else
-    hipLaunchKernelGGL((ao_kernel<1, false>), dim3((unsigned)blocks), dim3(kThreads), 0, s, P,
-                       (int)b.shell_num, (int)b.ao_num, (int)b.nucl_num, coord, nucl, b.shells,
-                       b.ao_shell, b.ao_abc, ao_factor, b.expo, b.coef_norm, b.nucl_range,
-                       (const double *)nullptr, 0, out);
+  if (ao.flags != nullptr && (ao.nkb <= 0 || ao.nkb > kMaxFlagBlocks)) return hipErrorInvalidValue;
+  const size_t lds = ao.flags != nullptr ? (size_t)NCOMP * ao.nkb : 0;
+  hipLaunchKernelGGL((ao_kernel<NCOMP, FROM_TABLE, OutT>), dim3((unsigned)blocks), dim3(kThreads), lds,
+                     s, P, (int)b.shell_num, (int)b.ao_num, (int)b.nucl_num, coord, nucl, b.shells,
+                     b.ao_shell, b.ao_abc, ao_factor, b.expo, b.coef_norm, b.nucl_range, shell_table,
+                     (int)S_table, ao);
   return hipGetLastError();
+}
+
+hipError_t launch_ao_vgl(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
+                         const double *nucl, const double *ao_factor, double *out, int ncomp,
+                         uint8_t *flags, int nkb) {
+  AoOut<double> ao{out, ncomp == 5 ? b.ao_num : b.ao_num, flags, nkb, nullptr};
+  if (ncomp == 5)
+    return launch_ao<5, false, double>(s, b, P, coord, nucl, ao_factor, nullptr, 0, ao);
+  ao.flags = nullptr;
+  return launch_ao<1, false, double>(s, b, P, coord, nucl, ao_factor, nullptr, 0, ao);
+}
+
+hipError_t launch_ao_vgl_f32(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
+                             const double *nucl, const double *ao_factor, float *out, int64_t ld,
+                             float *rowmax, uint8_t *flags, int nkb) {
+  AoOut<float> ao{out, ld, flags, nkb, rowmax};
+  return launch_ao<5, false, float>(s, b, P, coord, nucl, ao_factor, nullptr, 0, ao);
 }
 
 hipError_t launch_ao_from_shell_table(hipStream_t s, const DeviceBasis &b, int64_t P,
                                       const double *coord, const double *nucl,
                                       const double *ao_factor, const double *shell_vgl,
                                       int64_t S_table, double *out, int ncomp) {
-  if (P <= 0) return hipSuccess;
-  const int64_t blocks = (P + kTP - 1) / kTP;
+  AoOut<double> ao{out, b.ao_num, nullptr, 0, nullptr};
   if (ncomp == 5)
-    hipLaunchKernelGGL((ao_kernel<5, true>), dim3((unsigned)blocks), dim3(kThreads), 0, s, P,
-                       (int)b.shell_num, (int)b.ao_num, (int)b.nucl_num, coord, nucl, b.shells,
-                       b.ao_shell, b.ao_abc, ao_factor, (const double *)nullptr,
-                       (const double *)nullptr, (const double *)nullptr, shell_vgl, (int)S_table, out);
-  else
-    hipLaunchKernelGGL((ao_kernel<1, true>), dim3((unsigned)blocks), dim3(kThreads), 0, s, P,
-                       (int)b.shell_num, (int)b.ao_num, (int)b.nucl_num, coord, nucl, b.shells,
-                       b.ao_shell, b.ao_abc, ao_factor, (const double *)nullptr,
-                       (const double *)nullptr, (const double *)nullptr, shell_vgl, (int)S_table, out);
-  return hipGetLastError();
+    return launch_ao<5, true, double>(s, b, P, coord, nucl, ao_factor, shell_vgl, S_table, ao);
+  return launch_ao<1, true, double>(s, b, P, coord, nucl, ao_factor, shell_vgl, S_table, ao);
 }
 
 hipError_t launch_shell_vgl(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,

@@ -29,16 +29,83 @@ qmckl_exit_code_device ready(qmckl_context_device context, Context *c, const cha
   return QMCKL_SUCCESS_DEVICE;
 }
 
-// mo_vgl for the current points into dst, from the context's ao_vgl.
+const char *gemm_timer_name(qmckl_precision_device form) {
+  switch (form) {
+    case QMCKL_PRECISION_FP32_DEVICE: return "mo_gemm_fp32";
+    case QMCKL_PRECISION_FP16X2_DEVICE: return "mo_gemm_fp16x2";
+    case QMCKL_PRECISION_BF16X2_DEVICE: return "mo_gemm_bf16x2";
+    case QMCKL_PRECISION_BF16X3_DEVICE: return "mo_gemm_bf16x3";
+    default: return "mo_gemm_f64";
+  }
+}
+
+// Mixed forms: the FP64 AO assembly stored as FP32 rows [5P][ld32] with row
+// maxima and nonzero flags, then the split MFMA contraction into dst.
+qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Context *c, double *dst,
+                                            const char *fn) {
+  if (!c->ao.provided || c->point.num <= 0 || c->point.coord == nullptr || !c->nucleus.provided)
+    return fail(context, QMCKL_NOT_PROVIDED_DEVICE, "qmckl_ao_basis", nullptr);
+  const qmckl_precision_device form = c->mo.precision;
+  const int64_t P = c->point.num, A = c->ao.ao_num, M = c->mo.mo_num, R = 5 * P;
+  const int64_t ld = round_up(A, 32);
+  const int nkb = (int)(ld / 32);
+  const int64_t ntile = (R + 63) / 64;
+  AoBasis &ao = c->ao;
+  MoBasis &mo = c->mo;
+  // operands
+  const bool fresh = ao.ao_f32 == nullptr || ao.ao_f32_cap < (int64_t)(sizeof(float) * R * ld);
+  if (!grow_bytes(c, reinterpret_cast<void **>(&ao.ao_f32), &ao.ao_f32_cap, (int64_t)sizeof(float) * R * ld) ||
+      !grow_bytes(c, reinterpret_cast<void **>(&ao.ao_f32_rowmax), &ao.ao_f32_rowmax_cap,
+                  (int64_t)sizeof(float) * R))
+    return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
+  uint8_t *flags = nullptr;
+  if (nkb <= kMaxFlagBlocks) {
+    if (!grow_bytes(c, reinterpret_cast<void **>(&ao.ao_f32_flags), &ao.ao_f32_flags_cap, ntile * nkb))
+      return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
+    flags = ao.ao_f32_flags;
+  }
+  hipError_t e = hipSuccess;
+  if (fresh)  // the padding columns [A, ld) are never written: zero them once
+    e = hipMemsetAsync(ao.ao_f32, 0, ao.ao_f32_cap, c->stream);
+  if (e == hipSuccess && (mo.coef_split == nullptr || mo.coef_split_form != form)) {
+    dfree(c, mo.coef_split);
+    mo.coef_split = static_cast<uint16_t *>(dmalloc(c, sizeof(uint16_t) * split_terms(form) * M * ld));
+    if (!drealloc(c, reinterpret_cast<void **>(&mo.coef_colinv), sizeof(float) * M) || !mo.coef_split)
+      return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
+    e = launch_split_coef(c->stream, form, M, A, mo.coefficient, ld, mo.coef_split, mo.coef_colinv);
+    if (e == hipSuccess) mo.coef_split_form = form;
+  }
+  if (e == hipSuccess) {
+    KernelTimer t(c, "ao_vgl_f32");
+    e = launch_ao_vgl_f32(c->stream, ao.dev, P, c->point.coord, c->nucleus.coord, ao.ao_factor,
+                          ao.ao_f32, ld, ao.ao_f32_rowmax, flags, flags ? nkb : 0);
+    t.stop();
+  }
+  if (e == hipSuccess) {
+    KernelTimer t(c, gemm_timer_name(form));
+    e = launch_mo_gemm_mixed(c->stream, form, R, A, M, ao.ao_f32, ld, ao.ao_f32_rowmax, mo.coef_split, ld,
+                             mo.coef_colinv, flags, flags ? nkb : 0, mo.flush_blocks, dst, M);
+    t.stop();
+  }
+  if (e == hipSuccess) e = sync(c);
+  return hip_check(context, e, fn);
+}
+
+// mo_vgl for the current points into dst: FP64 from the context's ao_vgl, or
+// one of the mixed forms.
 qmckl_exit_code_device compute_mo_vgl(qmckl_context_device context, Context *c, double *dst,
                                       const char *fn) {
+  if (c->mo.precision != QMCKL_PRECISION_FP64_DEVICE) return compute_mo_vgl_mixed(context, c, dst, fn);
   qmckl_exit_code_device rc = qmckl_provide_ao_basis_ao_vgl_device(context);
   if (rc != QMCKL_SUCCESS_DEVICE) return fail(context, QMCKL_NOT_PROVIDED_DEVICE, "qmckl_ao_basis", nullptr);
   const int64_t P = c->point.num, A = c->ao.ao_num, M = c->mo.mo_num;
+  const int nkb = (int)((A + 31) / 32);
+  const uint8_t *flags = nkb <= kMaxFlagBlocks ? c->ao.ao_vgl_flags : nullptr;
   hipError_t e;
   {
     KernelTimer t(c, "mo_gemm_f64");
-    e = launch_mo_gemm_f64(c->stream, 5 * P, A, M, c->ao.ao_vgl, A, c->mo.coefficient_t, M, dst, M);
+    e = launch_mo_gemm_f64(c->stream, 5 * P, A, M, c->ao.ao_vgl, A, c->mo.coefficient_t, M, dst, M,
+                           flags, flags ? nkb : 0);
     t.stop();
   }
   if (e == hipSuccess) e = sync(c);
@@ -95,6 +162,7 @@ qmckl_exit_code_device qmckl_finalize_mo_basis_device(qmckl_context_device conte
   if (e == hipSuccess) e = sync(c);
   if (e != hipSuccess) return hip_check(context, e, fn);
   c->mo.mo_vgl_date = c->mo.mo_value_date = 0;
+  c->mo.coef_split_form = QMCKL_PRECISION_FP64_DEVICE;  // rebuilt on next mixed use
   return QMCKL_SUCCESS_DEVICE;
 }
 
@@ -270,6 +338,30 @@ qmckl_exit_code_device qmckl_compute_mo_basis_mo_value_device(qmckl_context_devi
                                     coefficient_t, mo_num, mo_value, mo_num);
   if (e == hipSuccess) e = sync(c);
   return hip_check(context, e, fn);
+}
+
+qmckl_exit_code_device qmckl_set_mo_basis_precision_device(qmckl_context_device context,
+                                                           qmckl_precision_device form) {
+  Context *c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+  if (form < QMCKL_PRECISION_FP64_DEVICE || form > QMCKL_PRECISION_BF16X3_DEVICE)
+    return fail(context, QMCKL_INVALID_ARG_2_DEVICE, "qmckl_set_mo_basis_precision_device",
+                "unknown precision form");
+  if (form != c->mo.precision) {
+    c->mo.precision = form;
+    c->mo.mo_vgl_date = c->mo.mo_value_date = 0;
+  }
+  return QMCKL_SUCCESS_DEVICE;
+}
+
+qmckl_exit_code_device qmckl_get_mo_basis_precision_device(qmckl_context_device context,
+                                                           qmckl_precision_device *form) {
+  Context *c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+  if (!form)
+    return fail(context, QMCKL_INVALID_ARG_2_DEVICE, "qmckl_get_mo_basis_precision_device", "NULL pointer");
+  *form = c->mo.precision;
+  return QMCKL_SUCCESS_DEVICE;
 }
 
 }  // extern "C"

@@ -226,3 +226,32 @@ def test_lock_unlock_recursive(ctx):
     L.qmckl_lock_device(ctx.handle)
     L.qmckl_unlock_device(ctx.handle)
     L.qmckl_unlock_device(ctx.handle)
+
+
+def test_precision_form_setting(ctx):
+    L = lib()
+    f = ctypes.c_int32(-1)
+    ctx.call("qmckl_get_mo_basis_precision_device", ctypes.byref(f))
+    assert f.value == 0                       # FP64 plain by default
+    assert ctx.rc("qmckl_set_mo_basis_precision_device", 5) == 2
+    assert ctx.rc("qmckl_set_mo_basis_precision_device", -1) == 2
+    for form in (1, 2, 3, 4, 0):
+        ctx.call("qmckl_set_mo_basis_precision_device", form)
+        ctx.call("qmckl_get_mo_basis_precision_device", ctypes.byref(f))
+        assert f.value == form
+    assert L.qmckl_get_mo_basis_precision_device(ctx.handle, None) == 2
+
+
+def test_adjoint_validation(ctx):
+    assert ctx.rc("qmckl_get_mo_basis_mo_vgl_adjoint_device", None, 10, 1, 1, 1) == 2
+    assert ctx.rc("qmckl_get_mo_basis_mo_vgl_adjoint_device", 1234, 10, 1, 1, 1) == NOT_PROVIDED
+
+
+def test_kernel_timing_api(ctx):
+    n = ctypes.c_int64(5)
+    ms = ctypes.c_double(1.0)
+    ctx.call("qmckl_set_kernel_timing_device", True)
+    ctx.call("qmckl_get_kernel_timing_device", b"ao_vgl", ctypes.byref(n), ctypes.byref(ms))
+    assert (n.value, ms.value) == (0, 0.0)
+    ctx.call("qmckl_reset_kernel_timing_device")
+    assert ctx.rc("qmckl_get_kernel_timing_device", None, None, None) == 2

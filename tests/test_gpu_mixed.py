@@ -1,0 +1,119 @@
+"""Precision forms of the MO contraction on the GPU, through the C ABI,
+against the FP64 oracle.
+
+Error measure (written before tuning): for each (point, component) row,
+max_i |mo - mo_ref| / max_i |mo_ref|, maximised over rows, reported
+separately for value (c=0), gradient (c=1..3) and Laplacian (c=4).
+Bars: 1e-6 for FP32, FP16X2 and BF16X3; the two-term bf16 split (BF16X2)
+carries ~2^-17 relative representation error per operand and is held to
+1e-4 (its measured level is reported)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from qmckl_sycl_amd import systems as S
+from qmckl_sycl_amd.context import Context, DeviceArray, lib
+
+from helpers import say
+
+pytestmark = pytest.mark.gpu
+
+FORMS = {"fp32": 1, "fp16x2": 2, "bf16x2": 3, "bf16x3": 4}
+BAR = {"fp32": 1e-6, "fp16x2": 1e-6, "bf16x2": 1e-4, "bf16x3": 1e-6}
+
+
+def row_err(got, ref):
+    d = np.abs(got - ref)
+    rm = np.abs(ref).max(axis=2, keepdims=True)
+    e = (d / np.where(rm > 0, rm, 1.0)).max(axis=(0, 2))
+    return {"value": float(e[0]), "gradient": float(e[1:4].max()), "laplacian": float(e[4])}
+
+
+def run_form(ctx, form, P):
+    ctx.call("qmckl_set_mo_basis_precision_device", FORMS[form])
+    out = ctx.mo_vgl(P)
+    f = ctypes.c_int32(-1)
+    ctx.call("qmckl_get_mo_basis_precision_device", ctypes.byref(f))
+    assert f.value == FORMS[form]
+    return out
+
+
+@pytest.fixture(scope="module")
+def chb():
+    return S.chbrclf()
+
+
+@pytest.mark.parametrize("form", list(FORMS))
+def test_chbrclf_forms(chb, form):
+    ref = S.chbrclf_mo_reference()
+    with Context(0) as c:
+        c.set_system(chb)
+        got = run_form(c, form, 136)
+    e = row_err(got, ref)
+    say(f"chbrclf {form}: {e}")
+    assert max(e.values()) <= BAR[form], e
+
+
+@pytest.mark.parametrize("form", list(FORMS))
+def test_chbrclf_sized_forms(oracle_lib, form):
+    s = S.chbrclf_sized()
+    with Context(0) as c:
+        c.set_system(s)
+        got = run_form(c, form, s.point_num)
+    ref = oracle_lib.mo_vgl(s)
+    e = row_err(got, ref)
+    say(f"chbrclf_sized {form}: {e}")
+    assert max(e.values()) <= BAR[form], e
+
+
+def test_water64_forms_sampled(oracle_lib):
+    """64-water, 128 walkers (81,920 points): every form on the full batch,
+    checked on 512 sampled points."""
+    s = S.water64(walk_num=128)
+    rng = np.random.default_rng(1)
+    idx = np.sort(rng.choice(s.point_num, size=512, replace=False))
+    coord = np.ascontiguousarray(s.coord_T()[:, idx])
+    ref = oracle_lib.mo_vgl_from_ao(s.mo_coef, oracle_lib.ao_vgl(s, coord))
+    errs = {}
+    with Context(0) as c:
+        c.set_system(s)
+        for form in FORMS:
+            got = run_form(c, form, s.point_num)[idx]
+            errs[form] = row_err(got, ref)
+            say(f"water64 {form}: {errs[form]}")
+            assert np.all(np.isfinite(got))
+        c.call("qmckl_set_mo_basis_precision_device", 0)
+        got = c.mo_vgl(s.point_num)[idx]
+        errs["fp64"] = row_err(got, ref)
+        say(f"water64 fp64: {errs['fp64']}")
+    for form, e in errs.items():
+        assert max(e.values()) <= BAR.get(form, 1e-12), (form, e)
+
+
+def test_switching_forms_recomputes(chb):
+    with Context(0) as c:
+        c.set_system(chb)
+        a = c.mo_vgl(136)
+        b = run_form(c, "fp16x2", 136)
+        assert not np.array_equal(a, b)
+        c.call("qmckl_set_mo_basis_precision_device", 0)
+        assert np.array_equal(c.mo_vgl(136), a)
+        assert c.rc("qmckl_set_mo_basis_precision_device", 9) == 2
+
+
+def test_block_skipping_is_exact(oracle_lib):
+    """The FP64 contraction with screened blocks skipped (provider path) is
+    bit-identical to the dense low-level launcher on the same ao_vgl."""
+    s = S.water64(walk_num=2)
+    with Context(0) as c:
+        c.set_system(s)
+        P, A, M = s.point_num, s.ao_num, s.mo_num
+        mo = c.mo_vgl(P)
+        ao = DeviceArray(c, (P, 5, A))
+        c.call("qmckl_get_ao_basis_ao_vgl_device", ao.ptr, ao.size)
+        ct = DeviceArray.from_numpy(c, np.ascontiguousarray(s.mo_coef.T))
+        out = DeviceArray(c, (P, 5, M))
+        c.check(lib().qmckl_compute_mo_basis_mo_vgl_device(c.handle, A, M, P, ct.ptr, ao.ptr, out.ptr))
+        dense = out.numpy()
+    assert np.array_equal(mo, dense)

@@ -86,6 +86,9 @@ void free_tables(Context *c, DeviceBasis &b) {
   dfree(c, b.nucl_range);
   dfree(c, b.nucl_max_l);
   dfree(c, b.nucl_prim_index);
+  dfree(c, b.chunks);
+  b.chunks = nullptr;
+  b.nchunks = 0;
   b.shells = nullptr;
   b.ao_shell = b.ao_abc = b.nucl_max_l = nullptr;
   b.coef_norm = b.nucl_range = nullptr;
@@ -169,8 +172,24 @@ std::string build_basis_tables(Context *c, DeviceBasis &b, int64_t N, int64_t S,
   b.nucl_range = h2d_new(c, nucl_range);
   b.nucl_max_l = h2d_new(c, nucl_max_l);
   b.nucl_prim_index = h2d_new(c, nucl_prim_index);
+  // chunks of whole shells for the AO kernel: <= kChunkShells shells and
+  // <= kChunkAos AOs each
+  std::vector<int32_t> chunks{0};
+  int ns = 0, na = 0;
+  for (size_t i = 0; i < shells.size(); ++i) {
+    const int nc = (shells[i].l + 1) * (shells[i].l + 2) / 2;
+    if (ns > 0 && (ns == kChunkShells || na + nc > kChunkAos)) {
+      chunks.push_back((int32_t)i);
+      ns = na = 0;
+    }
+    ++ns;
+    na += nc;
+  }
+  chunks.push_back((int32_t)shells.size());
+  b.nchunks = (int)chunks.size() - 1;
+  b.chunks = h2d_new(c, chunks);
   if (!b.shells || !b.ao_shell || !b.ao_abc || !b.coef_norm || !b.nucl_range || !b.nucl_max_l ||
-      !b.nucl_prim_index)
+      !b.nucl_prim_index || !b.chunks)
     return "device allocation failed";
   if (sync(c) != hipSuccess) return "copy to device failed";
   return "";

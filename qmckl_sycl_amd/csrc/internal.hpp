@@ -25,6 +25,8 @@ constexpr uint64_t kInvalidTag = 0xDEADBEEFull;  // include/qmckl_types.hpp:13
 constexpr double kCutoff = 27.631021115928547;   // -ln(1e-12), src/qmckl_ao_sycl.cpp:22
 constexpr int kMaxL = 6;                          // highest shell angular momentum on the device
 constexpr int kMaxCart = (kMaxL + 1) * (kMaxL + 2) / 2;
+constexpr int kChunkShells = 16;  // shells per AO-kernel chunk (LDS radial tile)
+constexpr int kChunkAos = 64;     // AOs per AO-kernel chunk (one per lane)
 
 // Device-side description of one shell, in AO order (nucleus-major, shells
 // of a nucleus in order), built at finalize.
@@ -60,6 +62,10 @@ struct DeviceBasis {
   double *nucl_range = nullptr;    // [N]
   int32_t *nucl_max_l = nullptr;   // [N]
   int64_t *nucl_prim_index = nullptr;  // [N+1]
+  // AO-kernel chunks: positions in `shells` where chunk c starts, [nchunks+1];
+  // a chunk holds at most kChunkShells shells and kChunkAos AOs
+  int32_t *chunks = nullptr;
+  int nchunks = 0;
   int max_shell_prims = 0;         // host copy: largest prim count of a shell
   int max_l = 0;                   // host copy
 };

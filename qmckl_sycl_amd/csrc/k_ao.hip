@@ -6,22 +6,24 @@
 // (291-331), and the product-rule assembly times ao_factor (367-422).
 // Screened entries are exact zeros.
 //
-// Work decomposition (one workgroup = 64 points x all AOs, 4 wavefronts):
-//   for each chunk of kShellChunk shells (in AO order):
-//     stage the chunk's shell descriptors and its (exponent, normalised
-//     coefficient) pairs in LDS;
-//     phase 1: each wavefront takes whole shells, lane = point, so the
-//       primitive loop is wavefront-uniform (LDS broadcast reads) and a
-//       primitive or shell screened for all 64 points costs no exp: the
-//       branch is skipped when every lane's predicate is false.  Radial
-//       R, dR/dx, dR/dy, dR/dz, lap R land in LDS [shell][5][point];
-//     phase 2: thread = (point, AO of the chunk), AO fastest, forms the
-//       monomial and its derivatives from the AO's (a,b,c) and writes the
-//       five components: consecutive lanes store consecutive AOs of one
-//       [p][c][*] row, so every store is a contiguous run.
+// Work decomposition (one workgroup = 64 points x all AOs, 4 wavefronts).
+// The shells, in AO order, are cut at finalize into chunks of at most 16
+// shells and 64 AOs.  For each chunk:
+//   - its shell descriptors and (exponent, normalised coefficient) pairs are
+//     staged in LDS;
+//   - phase 1: each wavefront takes whole shells, lane = point, so the
+//     primitive loop is wavefront-uniform (LDS broadcast reads) and a
+//     primitive screened for all 64 points costs no exp (the branch is
+//     skipped when no lane's predicate holds).  R, dR/dx, dR/dy, dR/dz and
+//     lap R land in LDS [shell][5][point];
+//   - phase 2: lane = AO of the chunk (its shell, exponents (a,b,c), factor
+//     and nucleus are loaded once), wavefront w walks points w, w+4, ...;
+//     every store of a wavefront is one contiguous run of a [p][c][*] row.
+//     Row maxima (for the FP32 operand of the mixed contraction) are reduced
+//     across the wavefront, which owns its points, so no atomics are needed.
 // r^2 is fma(x,x,fma(y,y,z*z)), as in the CPU oracle, so the screens decide
 // identically.  The kernel is bound by the HBM write of ao_vgl:
-// 40 B per (point, AO) plus 24 B per point.
+// 40 B (FP64) or 20 B (FP32) per (point, AO) plus 24 B per point.
 #include <hip/hip_runtime.h>
 
 #include "internal.hpp"
@@ -30,10 +32,10 @@ namespace qmckl_amd {
 
 namespace {
 
-constexpr int kTP = 64;          // points per workgroup
-constexpr int kThreads = 256;    // 4 wavefronts
-constexpr int kShellChunk = 16;  // shells per LDS chunk
-constexpr int kPrimLds = 384;    // staged primitives per chunk
+constexpr int kTP = 64;            // points per workgroup
+constexpr int kThreads = 256;      // 4 wavefronts
+constexpr int kPrimLds = 384;      // staged primitives per chunk
+constexpr int kRS = 5 * kTP + 2;   // LDS stride between shells (bank spread)
 
 struct Mono {
   double v, gx, gy, gz, lap;
@@ -55,8 +57,7 @@ __device__ inline void pow3(double x, int a, double &f0, double &f1, double &f2)
   f2 = p2;
 }
 
-__device__ inline Mono monomial(int abc, double x, double y, double z) {
-  const int a = abc & 0xff, b = (abc >> 8) & 0xff, c = (abc >> 16) & 0xff;
+__device__ inline Mono monomial(int a, int b, int c, double x, double y, double z) {
   double X0, X1, X2, Y0, Y1, Y2, Z0, Z1, Z2;
   pow3(x, a, X0, X1, X2);
   pow3(y, b, Y0, Y1, Y2);
@@ -71,47 +72,55 @@ __device__ inline Mono monomial(int abc, double x, double y, double z) {
   return m;
 }
 
+__device__ inline float wave_max(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
 // NCOMP = 5 (VGL) or 1 (value).  FROM_TABLE: radial parts are read from a
 // precomputed shell_vgl [P][5][S] instead of being summed from primitives.
 // OutT = double (the plain form) or float (the AO operand of the mixed MO
 // contraction: the FP64 value rounded once at the store).
 template <int NCOMP, bool FROM_TABLE, typename OutT>
 __global__ void __launch_bounds__(kThreads)
-ao_kernel(int64_t P, int S, int A, int N, const double *__restrict__ coord,
-          const double *__restrict__ nucl, const ShellDesc *__restrict__ shells,
+ao_kernel(int64_t P, int N, const double *__restrict__ coord, const double *__restrict__ nucl,
+          const ShellDesc *__restrict__ shells, const int32_t *__restrict__ chunks, int nchunks,
           const int32_t *__restrict__ ao_shell, const int32_t *__restrict__ ao_abc,
           const double *__restrict__ ao_factor, const double *__restrict__ expo,
           const double *__restrict__ coef, const double *__restrict__ nucl_range,
           const double *__restrict__ shell_table, int S_table, AoOut<OutT> ao) {
-  extern __shared__ unsigned char s_dyn[];  // [5 row tiles][nkb] nonzero flags
-  __shared__ double rad[kShellChunk][NCOMP][kTP];
+  extern __shared__ unsigned char s_flag[];  // [NCOMP row tiles][nkb] nonzero flags
+  __shared__ double rad[kChunkShells * kRS];
   __shared__ double pt[3][kTP];
   __shared__ double2 prim[kPrimLds];
-  __shared__ ShellDesc sd[kShellChunk];
-  __shared__ int prim_off[kShellChunk + 1];
+  __shared__ ShellDesc sd[kChunkShells];
+  __shared__ int prim_off[kChunkShells + 1];
+  __shared__ float s_rowmax[kTP * NCOMP];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int64_t p0 = (int64_t)blockIdx.x * kTP;
   const int np = (int)((P - p0) < kTP ? (P - p0) : kTP);
+  const bool track_flags = ao.flags != nullptr;
+  const bool track_max = ao.rowmax != nullptr;
 
-  __shared__ unsigned s_rowmax[kTP * NCOMP];  // float bits of max |value| per (point, component)
   if (tid < kTP) {
     const int64_t p = p0 + tid;
     const bool ok = tid < np;
     pt[0][tid] = ok ? coord[p] : 0.0;
     pt[1][tid] = ok ? coord[P + p] : 0.0;
-    pt[2][tid] = ok ? coord[2 * (int64_t)P + p] : 0.0;
+    pt[2][tid] = ok ? coord[2 * P + p] : 0.0;
   }
-  const bool track_flags = ao.flags != nullptr;
   if (track_flags)
-    for (int i = tid; i < NCOMP * ao.nkb; i += kThreads) s_dyn[i] = 0;
-  if (ao.rowmax != nullptr)
-    for (int i = tid; i < kTP * NCOMP; i += kThreads) s_rowmax[i] = 0u;
+    for (int i = tid; i < NCOMP * ao.nkb; i += kThreads) s_flag[i] = 0;
+  if (track_max)
+    for (int i = tid; i < kTP * NCOMP; i += kThreads) s_rowmax[i] = 0.0f;
 
-  for (int s0 = 0; s0 < S; s0 += kShellChunk) {
-    const int ns = (S - s0) < kShellChunk ? (S - s0) : kShellChunk;
-    __syncthreads();  // previous chunk's phase 2 is done with rad/sd
+  for (int ch = 0; ch < nchunks; ++ch) {
+    const int s0 = chunks[ch];
+    const int ns = chunks[ch + 1] - s0;
+    __syncthreads();  // the previous chunk is done with rad/sd/prim
     if (tid < ns) sd[tid] = shells[s0 + tid];
     __syncthreads();
     if (tid == 0) {
@@ -173,58 +182,70 @@ ao_kernel(int64_t P, int S, int A, int N, const double *__restrict__ coord,
         }
       }
 #pragma unroll
-      for (int c = 0; c < NCOMP; ++c) rad[sl][c][lane] = t[c];
+      for (int c = 0; c < NCOMP; ++c) rad[sl * kRS + c * kTP + lane] = t[c];
     }
     __syncthreads();
 
-    // phase 2: AOs of the chunk, thread = (point, AO), AO fastest
+    // phase 2: lane = AO of the chunk, wavefront walks its points
     const int ka = sd[0].ao_start;
     const int kb = sd[ns - 1].ao_start + (sd[ns - 1].l + 1) * (sd[ns - 1].l + 2) / 2;
-    const int nk = kb - ka;
-    const int work = np * nk;
-    for (int i = tid; i < work; i += kThreads) {
-      const int pl = i / nk;
-      const int k = ka + (i - pl * nk);
-      const int sl = ao_shell[k] - s0;
+    const bool act = lane < kb - ka;
+    const int k = ka + lane;
+    int sl = 0, ea = 0, eb = 0, ec = 0;
+    double f = 0.0, nx = 0.0, ny = 0.0, nz = 0.0;
+    if (act) {
+      sl = ao_shell[k] - s0;
+      const int abc = ao_abc[k];
+      ea = abc & 0xff;
+      eb = (abc >> 8) & 0xff;
+      ec = (abc >> 16) & 0xff;
+      f = ao_factor[k];
       const int n = sd[sl].nucl;
-      const double x = pt[0][pl] - nucl[n];
-      const double y = pt[1][pl] - nucl[N + n];
-      const double z = pt[2][pl] - nucl[2 * N + n];
-      const Mono m = monomial(ao_abc[k], x, y, z);
-      const double f = ao_factor[k];
-      const double R = rad[sl][0][pl];
+      nx = nucl[n];
+      ny = nucl[N + n];
+      nz = nucl[2 * N + n];
+    }
+    const double *rs = rad + sl * kRS;
+    for (int pl = wave; pl < np; pl += kThreads / 64) {
       double v[NCOMP];
-      v[0] = m.v * R * f;
-      if (NCOMP == 5) {
-        const double Rx = rad[sl][1 % NCOMP][pl], Ry = rad[sl][2 % NCOMP][pl];
-        const double Rz = rad[sl][3 % NCOMP][pl], RL = rad[sl][4 % NCOMP][pl];
-        v[1 % NCOMP] = (m.gx * R + m.v * Rx) * f;
-        v[2 % NCOMP] = (m.gy * R + m.v * Ry) * f;
-        v[3 % NCOMP] = (m.gz * R + m.v * Rz) * f;
-        v[4 % NCOMP] = (m.lap * R + m.v * RL + 2.0 * (m.gx * Rx + m.gy * Ry + m.gz * Rz)) * f;
+#pragma unroll
+      for (int c = 0; c < NCOMP; ++c) v[c] = 0.0;
+      if (act) {
+        const Mono m = monomial(ea, eb, ec, pt[0][pl] - nx, pt[1][pl] - ny, pt[2][pl] - nz);
+        const double R = rs[pl];
+        v[0] = m.v * R * f;
+        if (NCOMP == 5) {
+          const double Rx = rs[1 * kTP + pl], Ry = rs[2 * kTP + pl];
+          const double Rz = rs[3 * kTP + pl], RL = rs[4 * kTP + pl];
+          v[1 % NCOMP] = (m.gx * R + m.v * Rx) * f;
+          v[2 % NCOMP] = (m.gy * R + m.v * Ry) * f;
+          v[3 % NCOMP] = (m.gz * R + m.v * Rz) * f;
+          v[4 % NCOMP] = (m.lap * R + m.v * RL + 2.0 * (m.gx * Rx + m.gy * Ry + m.gz * Rz)) * f;
+        }
       }
       OutT *o = ao.out + ((p0 + pl) * NCOMP) * ao.ld + k;
 #pragma unroll
       for (int c = 0; c < NCOMP; ++c) {
         const OutT w = (OutT)v[c];
-        o[c * ao.ld] = w;
-        if (ao.rowmax != nullptr)
-          atomicMax(&s_rowmax[pl * NCOMP + c], __float_as_uint(fabsf((float)w)));
-        if (track_flags && w != (OutT)0) s_dyn[((pl * NCOMP + c) >> 6) * ao.nkb + (k >> 5)] = 1;
+        if (act) o[c * ao.ld] = w;
+        if (track_flags && w != (OutT)0) s_flag[((pl * NCOMP + c) >> 6) * ao.nkb + (k >> 5)] = 1;
+        if (track_max) {
+          const float mx = wave_max(fabsf((float)w));
+          if (lane == 0) s_rowmax[pl * NCOMP + c] = fmaxf(s_rowmax[pl * NCOMP + c], mx);
+        }
       }
     }
   }
   __syncthreads();
-  if (ao.rowmax != nullptr)
-    for (int i = tid; i < np * NCOMP; i += kThreads)
-      ao.rowmax[p0 * NCOMP + i] = __uint_as_float(s_rowmax[i]);
+  if (track_max)
+    for (int i = tid; i < np * NCOMP; i += kThreads) ao.rowmax[p0 * NCOMP + i] = s_rowmax[i];
   if (track_flags) {
-    // this workgroup owns rows [NCOMP*p0, NCOMP*(p0+np)): NCOMP whole 64-row tiles
+    // this workgroup owns rows [NCOMP*p0, NCOMP*(p0+np)): whole 64-row tiles
     const int64_t rt0 = (p0 * NCOMP) >> 6;
     const int64_t rt_end = (NCOMP * (p0 + np) + 63) >> 6;
     for (int i = tid; i < NCOMP * ao.nkb; i += kThreads) {
       const int64_t rt = rt0 + i / ao.nkb;
-      if (rt < rt_end) ao.flags[rt * ao.nkb + (i % ao.nkb)] = s_dyn[i];
+      if (rt < rt_end) ao.flags[rt * ao.nkb + (i % ao.nkb)] = s_flag[i];
     }
   }
 }
@@ -272,20 +293,20 @@ hipError_t launch_ao(hipStream_t s, const DeviceBasis &b, int64_t P, const doubl
                      int64_t S_table, AoOut<OutT> ao) {
   if (P <= 0) return hipSuccess;
   const int64_t blocks = (P + kTP - 1) / kTP;
-  if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+  if (blocks > 0x7fffffff || b.chunks == nullptr || b.nchunks <= 0) return hipErrorInvalidValue;
   if (ao.flags != nullptr && (ao.nkb <= 0 || ao.nkb > kMaxFlagBlocks)) return hipErrorInvalidValue;
   const size_t lds = ao.flags != nullptr ? (size_t)NCOMP * ao.nkb : 0;
   hipLaunchKernelGGL((ao_kernel<NCOMP, FROM_TABLE, OutT>), dim3((unsigned)blocks), dim3(kThreads), lds,
-                     s, P, (int)b.shell_num, (int)b.ao_num, (int)b.nucl_num, coord, nucl, b.shells,
-                     b.ao_shell, b.ao_abc, ao_factor, b.expo, b.coef_norm, b.nucl_range, shell_table,
-                     (int)S_table, ao);
+                     s, P, (int)b.nucl_num, coord, nucl, b.shells, b.chunks, b.nchunks, b.ao_shell,
+                     b.ao_abc, ao_factor, b.expo, b.coef_norm, b.nucl_range, shell_table, (int)S_table,
+                     ao);
   return hipGetLastError();
 }
 
 hipError_t launch_ao_vgl(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
                          const double *nucl, const double *ao_factor, double *out, int ncomp,
                          uint8_t *flags, int nkb) {
-  AoOut<double> ao{out, ncomp == 5 ? b.ao_num : b.ao_num, flags, nkb, nullptr};
+  AoOut<double> ao{out, b.ao_num, flags, nkb, nullptr};
   if (ncomp == 5)
     return launch_ao<5, false, double>(s, b, P, coord, nucl, ao_factor, nullptr, 0, ao);
   ao.flags = nullptr;

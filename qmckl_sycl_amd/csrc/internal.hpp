@@ -39,16 +39,14 @@ struct ShellDesc {
   int32_t shell;       // index in the user's shell arrays
 };
 
-// Output of the AO kernel: values at out[(p*ncomp + c)*ld + k]; optional
-// nonzero flags per (64-row tile of the (p,c) rows, 32-AO block) and optional
-// max |value| per (p,c) row.
+// Output of the AO kernel: values at out[(p*ncomp + c)*ld + k] and optional
+// nonzero flags per (64-row tile of the (p,c) rows, 32-AO block).
 template <typename OutT>
 struct AoOut {
   OutT *out;
   int64_t ld;
   uint8_t *flags;  // [ceil(ncomp*P/64)][nkb] or nullptr
   int nkb;
-  float *rowmax;   // [ncomp*P] or nullptr
 };
 
 // Basis tables on the device (owned by the context, rebuilt by finalize).
@@ -121,12 +119,10 @@ struct AoBasis {
   // nonzero flags of ao_vgl per (64 rows of [5P], 32 AOs), written with it
   uint8_t *ao_vgl_flags = nullptr;
   int64_t ao_vgl_flags_cap = 0;
-  // FP32 AO operand of the mixed contraction: [5P][ld32], ld32 = A rounded
-  // up to 32 (padding zero), its row maxima and nonzero flags
+  // FP32 AO operand of the mixed contraction: [5P][ld], ld = A rounded up
+  // to 64 (padding zero), and its nonzero flags
   float *ao_f32 = nullptr;
   int64_t ao_f32_cap = 0;
-  float *ao_f32_rowmax = nullptr;
-  int64_t ao_f32_rowmax_cap = 0;
   uint8_t *ao_f32_flags = nullptr;
   int64_t ao_f32_flags_cap = 0;
   int32_t uninitialized = (1 << 14) - 1;
@@ -144,9 +140,8 @@ struct MoBasis {
   int64_t mo_value_cap = 0;
   uint64_t mo_value_date = 0;
   qmckl_precision_device precision = QMCKL_PRECISION_FP64_DEVICE;
-  int flush_blocks = 2;             // 32-wide K blocks per FP32 partial sum
-  // MO coefficients prepared for the mixed forms: [terms][M][ld32] 16-bit
-  // split terms (or [M][ld32] FP32), inverse column scales [M]
+  // MO coefficients prepared for the mixed forms: [terms][M][ld] 16-bit
+  // split terms (or [M][ld] FP32), inverse column scales [M]
   uint16_t *coef_split = nullptr;
   float *coef_colinv = nullptr;
   qmckl_precision_device coef_split_form = QMCKL_PRECISION_FP64_DEVICE;
@@ -251,7 +246,7 @@ hipError_t launch_ao_vgl(hipStream_t s, const DeviceBasis &b, int64_t P, const d
                          int ncomp, uint8_t *flags = nullptr, int nkb = 0);
 hipError_t launch_ao_vgl_f32(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
                              const double *nucl, const double *ao_factor, float *out, int64_t ld,
-                             float *rowmax, uint8_t *flags, int nkb);
+                             uint8_t *flags, int nkb);
 hipError_t launch_ao_from_shell_table(hipStream_t s, const DeviceBasis &b, int64_t P,
                                       const double *coord, const double *nucl,
                                       const double *ao_factor, const double *shell_vgl,
@@ -277,9 +272,9 @@ int split_terms(int form);
 hipError_t launch_split_coef(hipStream_t s, int form, int64_t M, int64_t K, const double *coef,
                              int64_t ldb, uint16_t *B, float *colinv);
 hipError_t launch_mo_gemm_mixed(hipStream_t s, int form, int64_t R, int64_t K, int64_t Ncol,
-                                const float *A, int64_t lda, const float *rowmax, const void *B,
-                                int64_t ldb, const float *colinv, const uint8_t *flags, int nkb,
-                                int flush, double *C, int64_t ldc);
+                                const float *A, int64_t lda, const void *B, int64_t ldb,
+                                const float *colinv, const uint8_t *flags, int nkb, double *C,
+                                int64_t ldc);
 inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 hipError_t launch_strided_copy(hipStream_t s, int64_t rows, int64_t cols, const double *in,
                                int64_t ld_in, double *out, int64_t ld_out);

@@ -19,8 +19,6 @@
 //   - phase 2: lane = AO of the chunk (its shell, exponents (a,b,c), factor
 //     and nucleus are loaded once), wavefront w walks points w, w+4, ...;
 //     every store of a wavefront is one contiguous run of a [p][c][*] row.
-//     Row maxima (for the FP32 operand of the mixed contraction) are reduced
-//     across the wavefront, which owns its points, so no atomics are needed.
 // r^2 is fma(x,x,fma(y,y,z*z)), as in the CPU oracle, so the screens decide
 // identically.  The kernel is bound by the HBM write of ao_vgl:
 // 40 B (FP64) or 20 B (FP32) per (point, AO) plus 24 B per point.
@@ -72,12 +70,6 @@ __device__ inline Mono monomial(int a, int b, int c, double x, double y, double 
   return m;
 }
 
-__device__ inline float wave_max(float v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-  return v;
-}
-
 // NCOMP = 5 (VGL) or 1 (value).  FROM_TABLE: radial parts are read from a
 // precomputed shell_vgl [P][5][S] instead of being summed from primitives.
 // OutT = double (the plain form) or float (the AO operand of the mixed MO
@@ -96,14 +88,12 @@ ao_kernel(int64_t P, int N, const double *__restrict__ coord, const double *__re
   __shared__ double2 prim[kPrimLds];
   __shared__ ShellDesc sd[kChunkShells];
   __shared__ int prim_off[kChunkShells + 1];
-  __shared__ float s_rowmax[kTP * NCOMP];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int64_t p0 = (int64_t)blockIdx.x * kTP;
   const int np = (int)((P - p0) < kTP ? (P - p0) : kTP);
   const bool track_flags = ao.flags != nullptr;
-  const bool track_max = ao.rowmax != nullptr;
 
   if (tid < kTP) {
     const int64_t p = p0 + tid;
@@ -114,8 +104,6 @@ ao_kernel(int64_t P, int N, const double *__restrict__ coord, const double *__re
   }
   if (track_flags)
     for (int i = tid; i < NCOMP * ao.nkb; i += kThreads) s_flag[i] = 0;
-  if (track_max)
-    for (int i = tid; i < kTP * NCOMP; i += kThreads) s_rowmax[i] = 0.0f;
 
   for (int ch = 0; ch < nchunks; ++ch) {
     const int s0 = chunks[ch];
@@ -229,16 +217,10 @@ ao_kernel(int64_t P, int N, const double *__restrict__ coord, const double *__re
         const OutT w = (OutT)v[c];
         if (act) o[c * ao.ld] = w;
         if (track_flags && w != (OutT)0) s_flag[((pl * NCOMP + c) >> 6) * ao.nkb + (k >> 5)] = 1;
-        if (track_max) {
-          const float mx = wave_max(fabsf((float)w));
-          if (lane == 0) s_rowmax[pl * NCOMP + c] = fmaxf(s_rowmax[pl * NCOMP + c], mx);
-        }
       }
     }
   }
   __syncthreads();
-  if (track_max)
-    for (int i = tid; i < np * NCOMP; i += kThreads) ao.rowmax[p0 * NCOMP + i] = s_rowmax[i];
   if (track_flags) {
     // this workgroup owns rows [NCOMP*p0, NCOMP*(p0+np)): whole 64-row tiles
     const int64_t rt0 = (p0 * NCOMP) >> 6;
@@ -306,7 +288,7 @@ hipError_t launch_ao(hipStream_t s, const DeviceBasis &b, int64_t P, const doubl
 hipError_t launch_ao_vgl(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
                          const double *nucl, const double *ao_factor, double *out, int ncomp,
                          uint8_t *flags, int nkb) {
-  AoOut<double> ao{out, b.ao_num, flags, nkb, nullptr};
+  AoOut<double> ao{out, b.ao_num, flags, nkb};
   if (ncomp == 5)
     return launch_ao<5, false, double>(s, b, P, coord, nucl, ao_factor, nullptr, 0, ao);
   ao.flags = nullptr;
@@ -315,8 +297,8 @@ hipError_t launch_ao_vgl(hipStream_t s, const DeviceBasis &b, int64_t P, const d
 
 hipError_t launch_ao_vgl_f32(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
                              const double *nucl, const double *ao_factor, float *out, int64_t ld,
-                             float *rowmax, uint8_t *flags, int nkb) {
-  AoOut<float> ao{out, ld, flags, nkb, rowmax};
+                             uint8_t *flags, int nkb) {
+  AoOut<float> ao{out, ld, flags, nkb};
   return launch_ao<5, false, float>(s, b, P, coord, nucl, ao_factor, nullptr, 0, ao);
 }
 
@@ -324,7 +306,7 @@ hipError_t launch_ao_from_shell_table(hipStream_t s, const DeviceBasis &b, int64
                                       const double *coord, const double *nucl,
                                       const double *ao_factor, const double *shell_vgl,
                                       int64_t S_table, double *out, int ncomp) {
-  AoOut<double> ao{out, b.ao_num, nullptr, 0, nullptr};
+  AoOut<double> ao{out, b.ao_num, nullptr, 0};
   if (ncomp == 5)
     return launch_ao<5, true, double>(s, b, P, coord, nucl, ao_factor, shell_vgl, S_table, ao);
   return launch_ao<1, true, double>(s, b, P, coord, nucl, ao_factor, shell_vgl, S_table, ao);

@@ -4,25 +4,21 @@
 // (src/qmckl_mo_sycl.cpp:23-50), with the AO operand stored in FP32
 // (rounded once from the FP64 assembly) and the MO coefficients pre-split.
 // Forms:
-//   FP16X2: each operand is scaled by a power of two (AO per (p,c) row from
-//           its max |value|, C per MO from its max |coefficient|) and split
-//           into fp16 hi + lo; three products hi*hi + hi*lo + lo*hi.
-//   BF16X2: bf16 hi + lo, the same three products (no scaling needed).
+//   FP16X2: each operand is scaled by a power of two and split into fp16
+//           hi + lo; three products hi*hi + hi*lo + lo*hi.  The AO scale is
+//           per (row, 64-wide K stage), chosen in the kernel from the FP32
+//           values; the C scale is per MO, from its max |coefficient|.
+//   BF16X2: bf16 hi + lo, the same three products.
 //   BF16X3: bf16 hi + mid + lo, six products (hh, hm, mh, hl, lh, mm).
 //   FP32:   v_mfma_f32_16x16x4_f32 on the FP32 operands.
-// The MFMA accumulates in FP32 over `flush` consecutive 32-wide K blocks
-// and is then added into FP64 accumulators, which bounds the FP32
-// accumulation error to that of short partial sums.  K blocks whose AO tile
-// is all zero for the workgroup's 64 rows (screened AOs) are skipped using
+// The MFMA accumulates in FP32 over 64 consecutive K and is then added into
+// FP64 accumulators, which bounds the FP32 accumulation error to that of
+// short partial sums.  Power-of-two scales are exact.  K blocks whose AO
+// tile is all zero for the workgroup's rows (screened AOs) are skipped using
 // the nonzero flags the AO kernel writes.
 //
-// Tiling: 256 threads = 2 x 2 wavefronts; a wavefront owns 32 rows x
-// 16*CBW columns (2 x CBW tiles of 16x16), the workgroup 64 rows x 32*CBW.
-// Per K block the 64x32 FP32 AO slab is loaded once (two 16-byte loads per
-// thread), scaled and split into LDS; the C slab (32*CBW MOs x 32 AOs, K
-// contiguous per MO) is copied into LDS.  16x16x32 operands: lane l holds
-// A[l%16][8(l/16)..8(l/16)+7] and B[8(l/16)..+7][l%16] and receives
-// D[4(l/16)+j][l%16], j = 0..3.
+// 16x16x32 operands: lane l holds A[l%16][8(l/16)..8(l/16)+7] and
+// B[8(l/16)..+7][l%16] and receives D[4(l/16)+j][l%16], j = 0..3.
 #include <hip/hip_runtime.h>
 
 #include "internal.hpp"
@@ -40,10 +36,11 @@ constexpr int kBM = 64;
 constexpr int kKB = 32;
 constexpr int kPad = 8;  // halves of padding per LDS row
 
-// power-of-two scale bringing max |x| into [2^13, 2^14)
+// power-of-two scale bringing max |x| into [2^13, 2^14); the exponent is
+// clamped so that the scale and its inverse stay normal floats
 __device__ inline float pow2_scale(float m) {
   if (!(m > 0.0f) || !isfinite(m)) return 1.0f;
-  return ldexpf(1.0f, 13 - ilogbf(m));
+  return ldexpf(1.0f, min(100, max(-100, 13 - ilogbf(m))));
 }
 
 template <int SP>
@@ -91,130 +88,233 @@ __device__ inline f4 mfma(const uint4 &a, const uint4 &b, f4 c) {
 }
 
 // Split GEMM (FP16X2, BF16X2, BF16X3).
-//   A   [R][lda] FP32 AO rows, lda a multiple of 32, entries past K zero
-//   rowmax [R] max |A row| (scale source)
-//   Bs  [T][Ncol][ldb] split C terms (pre-scaled by colscale), K contiguous
+//   A      [R][lda] FP32 AO rows, lda a multiple of 64, entries past K zero
+//   Bs     [T][Ncol][ldb] split C terms (pre-scaled by their column scale),
+//          K contiguous, ldb a multiple of 64
 //   colinv [Ncol] inverse column scales
+//   flags  [ceil(R/64)][nkb] nonzero flags per (64-row tile, 32-wide K
+//          block), or null
+// Workgroup tile 128 rows x 32*CBW columns, 2 x 2 wavefronts of
+// 64 x 16*CBW (4 x CBW MFMA tiles each, one wavefront per SIMD).  K advances
+// in stages of 64; the stages whose four flags are all zero are dropped from
+// the workgroup's stage list up front.  Per stage:
+//   - the FP32 A slab (128 x 64) and the split C slab were loaded into
+//     registers while the previous stage computed;
+//   - each A row is scaled by the power of two that brings its largest
+//     |entry| in the stage into [2^13, 2^14) and split into LDS;
+//   - 2 x 4 x CBW x (3 or 6) MFMAs accumulate in FP32 over the 64 K, and the
+//     stage's sums are unscaled (exactly) into FP64 accumulators.
+// Workgroup ids are laid out so that the col_tiles workgroups sharing a row
+// tile land on one XCD (ids i and i+8 go to the same XCD) and read the FP32
+// rows from that XCD's L2.
+constexpr int kSM = 128;          // rows per workgroup
+constexpr int kStage = 64;        // K per stage
+constexpr int kLDS = kStage + 8;  // LDS row length in halves (bank spread)
+constexpr int kMaxStages = kMaxFlagBlocks / 2;
+
+template <int SP>
+__device__ inline void split8(const float (&v)[8], uint4 (&out)[Terms<SP>::n]) {
+  if constexpr (SP == QMCKL_AMD_SPLIT_FP16X2) {
+    h8 hi, lo;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const _Float16 h = (_Float16)v[e];
+      hi[e] = h;
+      lo[e] = (_Float16)(v[e] - (float)h);
+    }
+    out[0] = __builtin_bit_cast(uint4, hi);
+    out[1] = __builtin_bit_cast(uint4, lo);
+  } else {
+    b8 hi, mid, lo;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const __bf16 h = (__bf16)v[e];
+      const float r = v[e] - (float)h;
+      const __bf16 m = (__bf16)r;
+      hi[e] = h;
+      mid[e] = m;
+      if constexpr (SP == QMCKL_AMD_SPLIT_BF16X3) lo[e] = (__bf16)(r - (float)m);
+    }
+    out[0] = __builtin_bit_cast(uint4, hi);
+    out[1] = __builtin_bit_cast(uint4, mid);
+    if constexpr (SP == QMCKL_AMD_SPLIT_BF16X3) out[2] = __builtin_bit_cast(uint4, lo);
+  }
+}
+
 template <int SP, int CBW>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads), amdgpu_waves_per_eu(1, 1)))
 gemm_split_kernel(int64_t R, int64_t K, int64_t Ncol, const float *__restrict__ A, int64_t lda,
-                  const float *__restrict__ rowmax, const uint16_t *__restrict__ Bs, int64_t ldb,
-                  const float *__restrict__ colinv, const uint8_t *__restrict__ flags, int nkb,
-                  int flush, double *__restrict__ C, int64_t ldc) {
+                  const uint16_t *__restrict__ Bs, int64_t ldb, const float *__restrict__ colinv,
+                  const uint8_t *__restrict__ flags, int nkb, int64_t row_tiles, int col_tiles,
+                  double *__restrict__ C, int64_t ldc) {
   constexpr int T = Terms<SP>::n;
   constexpr int BN = 32 * CBW;
-  constexpr int LDR = kKB + kPad;  // LDS row length in halves
-  __shared__ uint16_t sA[T][kBM * LDR];
-  __shared__ uint16_t sB[T][BN * LDR];
+  constexpr int BL = T * BN * 8;                          // 16-byte pieces of a C slab
+  constexpr int NBL = (BL + kThreads - 1) / kThreads;     // per thread
+  __shared__ uint16_t sA[T][kSM * kLDS];
+  __shared__ uint16_t sB[T][BN * kLDS];
+  __shared__ float s_inv[kSM];
+  __shared__ int s_list[kMaxStages];
+  __shared__ int s_count;
+
+  const int64_t wg = blockIdx.x;
+  const int64_t rt = (wg >> 3) / col_tiles * 8 + (wg & 7);
+  const int ct = (int)((wg >> 3) % col_tiles);
+  if (rt >= row_tiles) return;  // padding of the grid to whole groups of 8
+  const int64_t row0 = rt * kSM, col0 = (int64_t)ct * BN;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
   const int li = lane & 15, lk = lane >> 4;
-  const int64_t row0 = (int64_t)blockIdx.x * kBM;
-  const int64_t col0 = (int64_t)blockIdx.y * BN;
-  const int64_t nblk = (K + kKB - 1) / kKB;
+  const int nst = (int)((K + kStage - 1) / kStage);
   const int64_t Bterm = Ncol * ldb;
 
-  // this thread's A load slot: row r_ld, 8 consecutive k at seg
-  const int r_ld = tid >> 2, seg = (tid & 3) * 8;
-  const int64_t grow = row0 + r_ld;
-  const float sc = grow < R ? pow2_scale(rowmax[grow]) : 1.0f;
-
-  f4 d[2][CBW];
-  double acc[2][CBW][4];
+  // stage list
+  int count = nst;
+  if (flags != nullptr) {
+    if (wave == 0) {
+      int base = 0;
+      for (int s0 = 0; s0 < nst; s0 += 64) {
+        const int st = s0 + lane;
+        bool live = false;
+        if (st < nst) {
 #pragma unroll
-  for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-    for (int cb = 0; cb < CBW; ++cb) {
-      d[rb][cb] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[rb][cb][j] = 0.0;
-    }
-
-  int pending = 0;
-  for (int64_t kb = 0; kb < nblk; ++kb) {
-    if (flags != nullptr && flags[(int64_t)blockIdx.x * nkb + kb] == 0) continue;
-    const int64_t k0 = kb * kKB;
-    // A slab: FP32 -> scaled split terms
-    {
-      float v[8];
-      if (grow < R) {
-        const float4 *src = reinterpret_cast<const float4 *>(A + grow * lda + k0 + seg);
-        const float4 x0 = src[0], x1 = src[1];
-        v[0] = x0.x; v[1] = x0.y; v[2] = x0.z; v[3] = x0.w;
-        v[4] = x1.x; v[5] = x1.y; v[6] = x1.z; v[7] = x1.w;
-      } else {
-#pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = 0.f;
+          for (int r = 0; r < 2; ++r) {
+            const int64_t t64 = rt * 2 + r;
+            if (t64 * 64 < R) {
+              const uint8_t *f = flags + t64 * nkb;
+              live = live || f[2 * st] != 0 || (2 * st + 1 < nkb && f[2 * st + 1] != 0);
+            }
+          }
+        }
+        const uint64_t m = __ballot(live);
+        if (live) s_list[base + __popcll(m & ((1ull << lane) - 1ull))] = st;
+        base += __popcll(m);
       }
-#pragma unroll
-      for (int j = 0; j < 8; ++j)
-        split_store<SP>(v[j] * sc, &sA[0][r_ld * LDR + seg + j], kBM * LDR);
-    }
-    // C slab: BN columns x 32 k of each term, 16 bytes per copy
-    for (int i = tid; i < T * BN * 4; i += kThreads) {
-      const int t = i / (BN * 4), rem = i - t * BN * 4;
-      const int c = rem >> 2, s8 = (rem & 3) * 8;
-      const int64_t gc = col0 + c;
-      uint4 val = {0u, 0u, 0u, 0u};
-      if (gc < Ncol) val = *reinterpret_cast<const uint4 *>(Bs + t * Bterm + gc * ldb + k0 + s8);
-      *reinterpret_cast<uint4 *>(&sB[t][c * LDR + s8]) = val;
+      if (lane == 0) s_count = base;
     }
     __syncthreads();
-    uint4 a[T][2];
+    count = s_count;
+  }
+
+  // this thread's A slot: row ar of the tile, K half ah of the stage
+  const int ar = tid >> 1, ah = (tid & 1) * 32;
+  const bool arow_ok = row0 + ar < R;
+  const float *Arow = A + (arow_ok ? row0 + ar : 0) * lda + ah;
+
+  float4 ra[8];
+  uint4 rb[NBL];
+  auto load = [&](int s) {
+    const int64_t k0 = (int64_t)(flags != nullptr ? s_list[s] : s) * kStage;
+    const float4 *src = reinterpret_cast<const float4 *>(Arow + k0);
 #pragma unroll
-    for (int t = 0; t < T; ++t)
+    for (int q = 0; q < 8; ++q) ra[q] = arow_ok ? src[q] : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-      for (int rb = 0; rb < 2; ++rb)
-        a[t][rb] = *reinterpret_cast<const uint4 *>(&sA[t][(wr * 32 + rb * 16 + li) * LDR + 8 * lk]);
+    for (int q = 0; q < NBL; ++q) {
+      const int idx = tid + q * kThreads;
+      uint4 v = {0u, 0u, 0u, 0u};
+      if (idx < BL) {
+        const int t = idx / (BN * 8), rem = idx - t * (BN * 8);
+        const int64_t gc = col0 + (rem >> 3);
+        if (gc < Ncol) v = *reinterpret_cast<const uint4 *>(Bs + t * Bterm + gc * ldb + k0 + (rem & 7) * 8);
+      }
+      rb[q] = v;
+    }
+  };
+
+  f4 d[4][CBW];
+  double acc[4][CBW][4];
 #pragma unroll
-    for (int cb = 0; cb < CBW; ++cb) {
-      uint4 b[T];
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[r][cb][j] = 0.0;
+
+  if (count > 0) load(0);
+  for (int s = 0; s < count; ++s) {
+    __syncthreads();  // every wavefront is done reading the previous stage
+    {
+      float m = 0.f;
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        m = fmaxf(fmaxf(m, fmaxf(fabsf(ra[q].x), fabsf(ra[q].y))), fmaxf(fabsf(ra[q].z), fabsf(ra[q].w)));
+      m = fmaxf(m, __shfl_xor(m, 1, 64));
+      const float sc = pow2_scale(m);
+      if ((tid & 1) == 0) s_inv[ar] = 1.0f / sc;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float v[8] = {ra[2 * q].x * sc,     ra[2 * q].y * sc,     ra[2 * q].z * sc,
+                            ra[2 * q].w * sc,     ra[2 * q + 1].x * sc, ra[2 * q + 1].y * sc,
+                            ra[2 * q + 1].z * sc, ra[2 * q + 1].w * sc};
+        uint4 o[T];
+        split8<SP>(v, o);
+#pragma unroll
+        for (int t = 0; t < T; ++t) *reinterpret_cast<uint4 *>(&sA[t][ar * kLDS + ah + 8 * q]) = o[t];
+      }
+#pragma unroll
+      for (int q = 0; q < NBL; ++q) {
+        const int idx = tid + q * kThreads;
+        if (idx < BL) {
+          const int t = idx / (BN * 8), rem = idx - t * (BN * 8);
+          *reinterpret_cast<uint4 *>(&sB[t][(rem >> 3) * kLDS + (rem & 7) * 8]) = rb[q];
+        }
+      }
+    }
+    __syncthreads();
+    if (s + 1 < count) load(s + 1);  // in flight while this stage computes
+
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      uint4 a[T][4];
 #pragma unroll
       for (int t = 0; t < T; ++t)
-        b[t] = *reinterpret_cast<const uint4 *>(&sB[t][(wc * 16 * CBW + cb * 16 + li) * LDR + 8 * lk]);
 #pragma unroll
-      for (int rb = 0; rb < 2; ++rb) {
-        f4 x = d[rb][cb];
-        x = mfma<SP>(a[0][rb], b[0], x);
-        x = mfma<SP>(a[0][rb], b[1], x);
-        x = mfma<SP>(a[1][rb], b[0], x);
-        if constexpr (T == 3) {
-          x = mfma<SP>(a[0][rb], b[T - 1], x);
-          x = mfma<SP>(a[T - 1][rb], b[0], x);
-          x = mfma<SP>(a[1][rb], b[1], x);
+        for (int r = 0; r < 4; ++r)
+          a[t][r] = *reinterpret_cast<const uint4 *>(&sA[t][(wr * 64 + r * 16 + li) * kLDS + ks * 32 + 8 * lk]);
+#pragma unroll
+      for (int cb = 0; cb < CBW; ++cb) {
+        uint4 b[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t)
+          b[t] = *reinterpret_cast<const uint4 *>(&sB[t][(wc * 16 * CBW + cb * 16 + li) * kLDS + ks * 32 + 8 * lk]);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          f4 x = ks == 0 ? f4{0.f, 0.f, 0.f, 0.f} : d[r][cb];
+          x = mfma<SP>(a[0][r], b[0], x);
+          x = mfma<SP>(a[0][r], b[1], x);
+          x = mfma<SP>(a[1][r], b[0], x);
+          if constexpr (T == 3) {
+            x = mfma<SP>(a[0][r], b[2], x);
+            x = mfma<SP>(a[2][r], b[0], x);
+            x = mfma<SP>(a[1][r], b[1], x);
+          }
+          d[r][cb] = x;
         }
-        d[rb][cb] = x;
       }
     }
-    __syncthreads();
-    if (++pending >= flush) {
 #pragma unroll
-      for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-        for (int cb = 0; cb < CBW; ++cb) {
-#pragma unroll
-          for (int j = 0; j < 4; ++j) acc[rb][cb][j] += (double)d[rb][cb][j];
-          d[rb][cb] = f4{0.f, 0.f, 0.f, 0.f};
-        }
-      pending = 0;
-    }
-  }
-#pragma unroll
-  for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-    for (int cb = 0; cb < CBW; ++cb) {
-      const int64_t col = col0 + wc * 16 * CBW + cb * 16 + li;
-      if (col >= Ncol) continue;
-      const double ci = (double)colinv[col];
+    for (int r = 0; r < 4; ++r)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int64_t row = row0 + wr * 32 + rb * 16 + 4 * lk + j;
-        if (row >= R) continue;
-        const double ri = 1.0 / (double)pow2_scale(rowmax[row]);
-        C[row * ldc + col] = (acc[rb][cb][j] + (double)d[rb][cb][j]) * ri * ci;
+        const double inv = (double)s_inv[wr * 64 + r * 16 + 4 * lk + j];
+#pragma unroll
+        for (int cb = 0; cb < CBW; ++cb) acc[r][cb][j] = fma((double)d[r][cb][j], inv, acc[r][cb][j]);
       }
-    }
+  }
+#pragma unroll
+  for (int cb = 0; cb < CBW; ++cb) {
+    const int64_t col = col0 + wc * 16 * CBW + cb * 16 + li;
+    if (col >= Ncol) continue;
+    const double ci = (double)colinv[col];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t row = row0 + wr * 64 + r * 16 + 4 * lk + j;
+        if (row < R) C[row * ldc + col] = acc[r][cb][j] * ci;
+      }
+  }
 }
 
 // FP32 form: v_mfma_f32_16x16x4_f32, lane l holds A[l%16][l/16], B[l/16][l%16].
@@ -341,31 +441,37 @@ split_coef_kernel(int64_t M, int64_t K, const double *__restrict__ coef, int64_t
 
 template <int SP, int CBW>
 hipError_t launch_split(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const float *A, int64_t lda,
-                        const float *rowmax, const void *B, int64_t ldb, const float *colinv,
-                        const uint8_t *flags, int nkb, int flush, double *C, int64_t ldc) {
+                        const void *B, int64_t ldb, const float *colinv, const uint8_t *flags, int nkb,
+                        double *C, int64_t ldc) {
   constexpr int BN = 32 * CBW;
-  const int64_t gx = (R + kBM - 1) / kBM, gy = (Ncol + BN - 1) / BN;
-  if (gx > 0x7fffffff || gy > 65535) return hipErrorInvalidValue;
-  if constexpr (SP == QMCKL_AMD_SPLIT_FP32)
+  if constexpr (SP == QMCKL_AMD_SPLIT_FP32) {
+    const int64_t gx = (R + kBM - 1) / kBM, gy = (Ncol + BN - 1) / BN;
+    if (gx > 0x7fffffff || gy > 65535) return hipErrorInvalidValue;
     hipLaunchKernelGGL(gemm_f32_kernel<CBW>, dim3((unsigned)gx, (unsigned)gy), dim3(kThreads), 0, s, R,
-                       K, Ncol, A, lda, (const float *)B, ldb, flags, nkb, flush, C, ldc);
-  else
-    hipLaunchKernelGGL((gemm_split_kernel<SP, CBW>), dim3((unsigned)gx, (unsigned)gy), dim3(kThreads), 0, s,
-                       R, K, Ncol, A, lda, rowmax, (const uint16_t *)B, ldb, colinv, flags, nkb, flush, C,
+                       K, Ncol, A, lda, (const float *)B, ldb, flags, nkb, 2, C, ldc);
+  } else {
+    const int64_t row_tiles = (R + kSM - 1) / kSM;
+    const int64_t col_tiles = (Ncol + BN - 1) / BN;
+    const int64_t nwg = (row_tiles + 7) / 8 * 8 * col_tiles;
+    if (nwg > 0x7fffffff || (lda % kStage) != 0 || (ldb % kStage) != 0) return hipErrorInvalidValue;
+    if (flags != nullptr && (nkb > kMaxFlagBlocks || nkb * 32 < K)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((gemm_split_kernel<SP, CBW>), dim3((unsigned)nwg), dim3(kThreads), 0, s, R, K, Ncol,
+                       A, lda, (const uint16_t *)B, ldb, colinv, flags, nkb, row_tiles, (int)col_tiles, C,
                        ldc);
+  }
   return hipGetLastError();
 }
 
 template <int SP>
 hipError_t launch_form(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const float *A, int64_t lda,
-                       const float *rowmax, const void *B, int64_t ldb, const float *colinv,
-                       const uint8_t *flags, int nkb, int flush, double *C, int64_t ldc) {
+                       const void *B, int64_t ldb, const float *colinv, const uint8_t *flags, int nkb,
+                       double *C, int64_t ldc) {
   const int64_t nb = (Ncol + 31) / 32;  // 32-column groups
-  if (nb <= 1) return launch_split<SP, 1>(s, R, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, flush, C, ldc);
-  if (nb <= 2) return launch_split<SP, 2>(s, R, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, flush, C, ldc);
-  if (nb <= 3) return launch_split<SP, 3>(s, R, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, flush, C, ldc);
-  if (nb <= 4) return launch_split<SP, 4>(s, R, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, flush, C, ldc);
-  return launch_split<SP, 5>(s, R, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, flush, C, ldc);
+  if (nb <= 1) return launch_split<SP, 1>(s, R, K, Ncol, A, lda, B, ldb, colinv, flags, nkb, C, ldc);
+  if (nb <= 2) return launch_split<SP, 2>(s, R, K, Ncol, A, lda, B, ldb, colinv, flags, nkb, C, ldc);
+  if (nb <= 3) return launch_split<SP, 3>(s, R, K, Ncol, A, lda, B, ldb, colinv, flags, nkb, C, ldc);
+  if (nb <= 4) return launch_split<SP, 4>(s, R, K, Ncol, A, lda, B, ldb, colinv, flags, nkb, C, ldc);
+  return launch_split<SP, 5>(s, R, K, Ncol, A, lda, B, ldb, colinv, flags, nkb, C, ldc);
 }
 
 }  // namespace
@@ -402,21 +508,20 @@ hipError_t launch_split_coef(hipStream_t s, int form, int64_t M, int64_t K, cons
 }
 
 hipError_t launch_mo_gemm_mixed(hipStream_t s, int form, int64_t R, int64_t K, int64_t Ncol,
-                                const float *A, int64_t lda, const float *rowmax, const void *B,
-                                int64_t ldb, const float *colinv, const uint8_t *flags, int nkb,
-                                int flush, double *C, int64_t ldc) {
+                                const float *A, int64_t lda, const void *B, int64_t ldb,
+                                const float *colinv, const uint8_t *flags, int nkb, double *C,
+                                int64_t ldc) {
   if (R <= 0 || Ncol <= 0) return hipSuccess;
-  if (K <= 0 || (lda % kKB) != 0 || (ldb % kKB) != 0 || lda < K || ldb < K || flush < 1)
-    return hipErrorInvalidValue;
+  if (K <= 0 || (lda % 64) != 0 || (ldb % 64) != 0 || lda < K || ldb < K) return hipErrorInvalidValue;
   switch (form) {
     case QMCKL_AMD_SPLIT_FP16X2:
-      return launch_form<QMCKL_AMD_SPLIT_FP16X2>(s, R, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, flush, C, ldc);
+      return launch_form<QMCKL_AMD_SPLIT_FP16X2>(s, R, K, Ncol, A, lda, B, ldb, colinv, flags, nkb, C, ldc);
     case QMCKL_AMD_SPLIT_BF16X2:
-      return launch_form<QMCKL_AMD_SPLIT_BF16X2>(s, R, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, flush, C, ldc);
+      return launch_form<QMCKL_AMD_SPLIT_BF16X2>(s, R, K, Ncol, A, lda, B, ldb, colinv, flags, nkb, C, ldc);
     case QMCKL_AMD_SPLIT_BF16X3:
-      return launch_form<QMCKL_AMD_SPLIT_BF16X3>(s, R, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, flush, C, ldc);
+      return launch_form<QMCKL_AMD_SPLIT_BF16X3>(s, R, K, Ncol, A, lda, B, ldb, colinv, flags, nkb, C, ldc);
     case QMCKL_AMD_SPLIT_FP32:
-      return launch_form<QMCKL_AMD_SPLIT_FP32>(s, R, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, flush, C, ldc);
+      return launch_form<QMCKL_AMD_SPLIT_FP32>(s, R, K, Ncol, A, lda, B, ldb, colinv, flags, nkb, C, ldc);
     default:
       return hipErrorInvalidValue;
   }

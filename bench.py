@@ -170,6 +170,18 @@ def cpu_baseline(s, target_s=10.0):
                       f"(ao_vgl then per-point mo_vgl loop), {t:.2f} s"}
 
 
+def copy_calibration(mib):
+    """A plain copy kernel of known size: reads and writes mib MiB each."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_each_way": n * 4, "launches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,11 +193,17 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the FP64 plain comparison run")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (the reference "
+                         "kernel for scaling HBM traffic counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
 
     from qmckl_sycl_amd.context import Context
     s = make_system(args.config, args.walkers)

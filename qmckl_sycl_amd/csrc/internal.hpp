@@ -268,14 +268,28 @@ int64_t ao_adjoint_blocks(int64_t P);
 hipError_t launch_adjoint_reduce(hipStream_t s, const DeviceBasis &b, int64_t nblk,
                                  const int64_t *nucleus_index, const int64_t *nucleus_shell_num,
                                  const double *partR, const double *partA, double *dR, double *dA);
+inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 int split_terms(int form);
+// MOs per workgroup of the split contraction: the smallest of 64, 128,
+// 192, 256, 320 that covers mo_num, at most 320 (192 for the three-term
+// form: register budget); more MOs take several workgroup columns.
+inline int split_wg_cols(int form, int64_t M) {
+  const int cap = form == QMCKL_AMD_SPLIT_BF16X3 ? 192 : 320;
+  const int64_t w = (M + 63) / 64 * 64;
+  return (int)(w < 64 ? 64 : (w > cap ? cap : w));
+}
+// MOs the prepared coefficient buffer holds: M rounded up to the workgroup
+// width of the split contraction (zero columns), M for the FP32 form.
+inline int64_t split_cols_padded(int form, int64_t M) {
+  if (form == QMCKL_AMD_SPLIT_FP32) return M;
+  return round_up(M, split_wg_cols(form, M));
+}
 hipError_t launch_split_coef(hipStream_t s, int form, int64_t M, int64_t K, const double *coef,
                              int64_t ldb, uint16_t *B, float *colinv);
 hipError_t launch_mo_gemm_mixed(hipStream_t s, int form, int64_t R, int64_t K, int64_t Ncol,
                                 const float *A, int64_t lda, const void *B, int64_t ldb,
                                 const float *colinv, const uint8_t *flags, int nkb, double *C,
                                 int64_t ldc);
-inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 hipError_t launch_strided_copy(hipStream_t s, int64_t rows, int64_t cols, const double *in,
                                int64_t ld_in, double *out, int64_t ld_out);
 

@@ -89,25 +89,33 @@ __device__ inline f4 mfma(const uint4 &a, const uint4 &b, f4 c) {
 
 // Split GEMM (FP16X2, BF16X2, BF16X3).
 //   A      [R][lda] FP32 AO rows, lda a multiple of 64, entries past K zero
-//   Bs     [T][Ncol][ldb] split C terms (pre-scaled by their column scale),
-//          K contiguous, ldb a multiple of 64
+//   Bs     [Npad][T][ldb] split C terms (pre-scaled by their column scale),
+//          K contiguous, ldb a multiple of 64; Npad = Ncol rounded up to the
+//          workgroup width, rows past Ncol zero
 //   colinv [Ncol] inverse column scales
 //   flags  [ceil(R/64)][nkb] nonzero flags per (64-row tile, 32-wide K
 //          block), or null
-// Workgroup tile 128 rows x 32*CBW columns, 2 x 2 wavefronts of
-// 64 x 16*CBW (4 x CBW MFMA tiles each, one wavefront per SIMD).  K advances
-// in stages of 64; the stages whose four flags are all zero are dropped from
-// the workgroup's stage list up front.  Per stage:
-//   - the FP32 A slab (128 x 64) and the split C slab were loaded into
-//     registers while the previous stage computed;
-//   - each A row is scaled by the power of two that brings its largest
-//     |entry| in the stage into [2^13, 2^14) and split into LDS;
+// Workgroup tile 64 rows x 64*CBW columns (all MOs for mo_num <= 320, so
+// each FP32 AO row is read from HBM once); wavefront w owns columns
+// [16*CBW*w, 16*CBW*(w+1)) for all 64 rows: 4 x CBW MFMA tiles, one
+// wavefront per SIMD.  K advances in stages of 64; stages whose two flags
+// are zero are dropped from the workgroup's stage list up front.  Per stage:
+//   - the FP32 A slab (64 x 64, loaded into registers while the previous
+//     stage computed) is scaled per row by the power of two that brings the
+//     row's largest |entry| in the stage into [2^13, 2^14) and split into
+//     LDS, shared by the four wavefronts;
+//   - each wavefront reads its C terms straight from global memory (L2) into
+//     registers in the MFMA operand layout (no LDS round trip: no other
+//     wavefront uses them); the next stage's half-stage of C is requested as
+//     soon as the current one has been consumed;
 //   - 2 x 4 x CBW x (3 or 6) MFMAs accumulate in FP32 over the 64 K, and the
 //     stage's sums are unscaled (exactly) into FP64 accumulators.
-// Workgroup ids are laid out so that the col_tiles workgroups sharing a row
-// tile land on one XCD (ids i and i+8 go to the same XCD) and read the FP32
-// rows from that XCD's L2.
-constexpr int kSM = 128;          // rows per workgroup
+// Measured on MI355X at 409,600 x 4,160 x 320: the loop is bound by the
+// C operand traffic through each CU's L1 (64 rows share each C byte) and by
+// the FP64 accumulators' register footprint, which keeps the tile at 64
+// rows and one wavefront per SIMD; v_mfma_f32_32x32x16 tiles and a
+// software-pipelined single-barrier loop were both measured slower here.
+constexpr int kRM = 64;           // rows per workgroup
 constexpr int kStage = 64;        // K per stage
 constexpr int kLDS = kStage + 8;  // LDS row length in halves (bank spread)
 constexpr int kMaxStages = kMaxFlagBlocks / 2;
@@ -145,48 +153,28 @@ template <int SP, int CBW>
 __global__ void __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads), amdgpu_waves_per_eu(1, 1)))
 gemm_split_kernel(int64_t R, int64_t K, int64_t Ncol, const float *__restrict__ A, int64_t lda,
                   const uint16_t *__restrict__ Bs, int64_t ldb, const float *__restrict__ colinv,
-                  const uint8_t *__restrict__ flags, int nkb, int64_t row_tiles, int col_tiles,
-                  double *__restrict__ C, int64_t ldc) {
+                  const uint8_t *__restrict__ flags, int nkb, double *__restrict__ C, int64_t ldc) {
   constexpr int T = Terms<SP>::n;
-  constexpr int BN = 32 * CBW;
-  constexpr int BL = T * BN * 8;                          // 16-byte pieces of a C slab
-  constexpr int NBL = (BL + kThreads - 1) / kThreads;     // per thread
-  __shared__ uint16_t sA[T][kSM * kLDS];
-  __shared__ uint16_t sB[T][BN * kLDS];
-  __shared__ float s_inv[kSM];
+  __shared__ uint16_t sA[T][kRM * kLDS];
+  __shared__ float s_inv[kRM];
   __shared__ int s_list[kMaxStages];
   __shared__ int s_count;
 
-  const int64_t wg = blockIdx.x;
-  const int64_t rt = (wg >> 3) / col_tiles * 8 + (wg & 7);
-  const int ct = (int)((wg >> 3) % col_tiles);
-  if (rt >= row_tiles) return;  // padding of the grid to whole groups of 8
-  const int64_t row0 = rt * kSM, col0 = (int64_t)ct * BN;
-
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = wave >> 1, wc = wave & 1;
   const int li = lane & 15, lk = lane >> 4;
+  const int64_t row0 = (int64_t)blockIdx.x * kRM;
+  const int64_t col0 = (int64_t)blockIdx.y * 64 * CBW + wave * 16 * CBW;  // this wavefront's first column
   const int nst = (int)((K + kStage - 1) / kStage);
-  const int64_t Bterm = Ncol * ldb;
 
   // stage list
   int count = nst;
   if (flags != nullptr) {
     if (wave == 0) {
+      const uint8_t *f = flags + (int64_t)blockIdx.x * nkb;
       int base = 0;
       for (int s0 = 0; s0 < nst; s0 += 64) {
         const int st = s0 + lane;
-        bool live = false;
-        if (st < nst) {
-#pragma unroll
-          for (int r = 0; r < 2; ++r) {
-            const int64_t t64 = rt * 2 + r;
-            if (t64 * 64 < R) {
-              const uint8_t *f = flags + t64 * nkb;
-              live = live || f[2 * st] != 0 || (2 * st + 1 < nkb && f[2 * st + 1] != 0);
-            }
-          }
-        }
+        const bool live = st < nst && (f[2 * st] != 0 || (2 * st + 1 < nkb && f[2 * st + 1] != 0));
         const uint64_t m = __ballot(live);
         if (live) s_list[base + __popcll(m & ((1ull << lane) - 1ull))] = st;
         base += __popcll(m);
@@ -197,29 +185,29 @@ gemm_split_kernel(int64_t R, int64_t K, int64_t Ncol, const float *__restrict__ 
     count = s_count;
   }
 
-  // this thread's A slot: row ar of the tile, K half ah of the stage
-  const int ar = tid >> 1, ah = (tid & 1) * 32;
-  const bool arow_ok = row0 + ar < R;
-  const float *Arow = A + (arow_ok ? row0 + ar : 0) * lda + ah;
+  // A slot: row ar of the tile, 16 consecutive K at aq; 16 consecutive lanes
+  // take 16 consecutive rows, so the LDS stores below do not conflict.
+  // Rows past R read row R-1 (their results are not stored).
+  const int ar = wave * 16 + li, aq = lk * 16;
+  const float *Ap = A + (row0 + ar < R ? row0 + ar : R - 1) * lda + aq;
+  // C operand: lane holds B[8*lk..8*lk+7][li] of each 16-column tile
+  const uint16_t *Bp = Bs + (col0 + li) * T * ldb + 8 * lk;
+  const int64_t cb_stride = 16 * T * ldb;
 
-  float4 ra[8];
-  uint4 rb[NBL];
-  auto load = [&](int s) {
-    const int64_t k0 = (int64_t)(flags != nullptr ? s_list[s] : s) * kStage;
-    const float4 *src = reinterpret_cast<const float4 *>(Arow + k0);
+  float4 ra[4];
+  uint4 bq[2][CBW][T];
+  auto stage_k0 = [&](int s) { return (int64_t)(flags != nullptr ? s_list[s] : s) * kStage; };
+  auto load_a = [&](int64_t k0) {
+    const float4 *src = reinterpret_cast<const float4 *>(Ap + k0);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) ra[q] = arow_ok ? src[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int q = 0; q < 4; ++q) ra[q] = src[q];
+  };
+  auto load_b = [&](int64_t k0, int ks) {
 #pragma unroll
-    for (int q = 0; q < NBL; ++q) {
-      const int idx = tid + q * kThreads;
-      uint4 v = {0u, 0u, 0u, 0u};
-      if (idx < BL) {
-        const int t = idx / (BN * 8), rem = idx - t * (BN * 8);
-        const int64_t gc = col0 + (rem >> 3);
-        if (gc < Ncol) v = *reinterpret_cast<const uint4 *>(Bs + t * Bterm + gc * ldb + k0 + (rem & 7) * 8);
-      }
-      rb[q] = v;
-    }
+    for (int cb = 0; cb < CBW; ++cb)
+#pragma unroll
+      for (int t = 0; t < T; ++t)
+        bq[ks][cb][t] = *reinterpret_cast<const uint4 *>(Bp + cb * cb_stride + t * ldb + k0 + ks * 32);
   };
 
   f4 d[4][CBW];
@@ -231,38 +219,38 @@ gemm_split_kernel(int64_t R, int64_t K, int64_t Ncol, const float *__restrict__ 
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[r][cb][j] = 0.0;
 
-  if (count > 0) load(0);
+  if (count > 0) {
+    const int64_t k0 = stage_k0(0);
+    load_a(k0);
+    load_b(k0, 0);
+    load_b(k0, 1);
+  }
   for (int s = 0; s < count; ++s) {
-    __syncthreads();  // every wavefront is done reading the previous stage
+    __syncthreads();  // every wavefront is done reading the previous A slab
     {
       float m = 0.f;
 #pragma unroll
-      for (int q = 0; q < 8; ++q)
+      for (int q = 0; q < 4; ++q)
         m = fmaxf(fmaxf(m, fmaxf(fabsf(ra[q].x), fabsf(ra[q].y))), fmaxf(fabsf(ra[q].z), fabsf(ra[q].w)));
-      m = fmaxf(m, __shfl_xor(m, 1, 64));
+      m = fmaxf(m, __shfl_xor(m, 16, 64));
+      m = fmaxf(m, __shfl_xor(m, 32, 64));
       const float sc = pow2_scale(m);
-      if ((tid & 1) == 0) s_inv[ar] = 1.0f / sc;
+      if (lk == 0) s_inv[ar] = 1.0f / sc;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const float v[8] = {ra[2 * q].x * sc,     ra[2 * q].y * sc,     ra[2 * q].z * sc,
-                            ra[2 * q].w * sc,     ra[2 * q + 1].x * sc, ra[2 * q + 1].y * sc,
-                            ra[2 * q + 1].z * sc, ra[2 * q + 1].w * sc};
+      for (int h = 0; h < 2; ++h) {
+        const float v[8] = {ra[2 * h].x * sc,     ra[2 * h].y * sc,     ra[2 * h].z * sc,
+                            ra[2 * h].w * sc,     ra[2 * h + 1].x * sc, ra[2 * h + 1].y * sc,
+                            ra[2 * h + 1].z * sc, ra[2 * h + 1].w * sc};
         uint4 o[T];
         split8<SP>(v, o);
 #pragma unroll
-        for (int t = 0; t < T; ++t) *reinterpret_cast<uint4 *>(&sA[t][ar * kLDS + ah + 8 * q]) = o[t];
-      }
-#pragma unroll
-      for (int q = 0; q < NBL; ++q) {
-        const int idx = tid + q * kThreads;
-        if (idx < BL) {
-          const int t = idx / (BN * 8), rem = idx - t * (BN * 8);
-          *reinterpret_cast<uint4 *>(&sB[t][(rem >> 3) * kLDS + (rem & 7) * 8]) = rb[q];
-        }
+        for (int t = 0; t < T; ++t) *reinterpret_cast<uint4 *>(&sA[t][ar * kLDS + aq + 8 * h]) = o[t];
       }
     }
     __syncthreads();
-    if (s + 1 < count) load(s + 1);  // in flight while this stage computes
+    const bool more = s + 1 < count;
+    const int64_t k1 = more ? stage_k0(s + 1) : 0;
+    if (more) load_a(k1);  // in flight while this stage computes
 
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks) {
@@ -271,47 +259,49 @@ gemm_split_kernel(int64_t R, int64_t K, int64_t Ncol, const float *__restrict__ 
       for (int t = 0; t < T; ++t)
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          a[t][r] = *reinterpret_cast<const uint4 *>(&sA[t][(wr * 64 + r * 16 + li) * kLDS + ks * 32 + 8 * lk]);
+          a[t][r] = *reinterpret_cast<const uint4 *>(&sA[t][(r * 16 + li) * kLDS + ks * 32 + 8 * lk]);
 #pragma unroll
       for (int cb = 0; cb < CBW; ++cb) {
-        uint4 b[T];
+        const uint4 *b = bq[ks][cb];
+        f4 x[4];
 #pragma unroll
-        for (int t = 0; t < T; ++t)
-          b[t] = *reinterpret_cast<const uint4 *>(&sB[t][(wc * 16 * CBW + cb * 16 + li) * kLDS + ks * 32 + 8 * lk]);
+        for (int r = 0; r < 4; ++r) x[r] = mfma<SP>(a[0][r], b[0], ks == 0 ? f4{0.f, 0.f, 0.f, 0.f} : d[r][cb]);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          f4 x = ks == 0 ? f4{0.f, 0.f, 0.f, 0.f} : d[r][cb];
-          x = mfma<SP>(a[0][r], b[0], x);
-          x = mfma<SP>(a[0][r], b[1], x);
-          x = mfma<SP>(a[1][r], b[0], x);
-          if constexpr (T == 3) {
-            x = mfma<SP>(a[0][r], b[2], x);
-            x = mfma<SP>(a[2][r], b[0], x);
-            x = mfma<SP>(a[1][r], b[1], x);
-          }
-          d[r][cb] = x;
+        for (int r = 0; r < 4; ++r) x[r] = mfma<SP>(a[0][r], b[1], x[r]);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) x[r] = mfma<SP>(a[1][r], b[0], x[r]);
+        if constexpr (T == 3) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) x[r] = mfma<SP>(a[0][r], b[2], x[r]);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) x[r] = mfma<SP>(a[2][r], b[0], x[r]);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) x[r] = mfma<SP>(a[1][r], b[1], x[r]);
         }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) d[r][cb] = x[r];
       }
+      if (more) load_b(k1, ks);  // this half-stage's C registers are free again
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const double inv = (double)s_inv[wr * 64 + r * 16 + 4 * lk + j];
+        const double inv = (double)s_inv[r * 16 + 4 * lk + j];
 #pragma unroll
         for (int cb = 0; cb < CBW; ++cb) acc[r][cb][j] = fma((double)d[r][cb][j], inv, acc[r][cb][j]);
       }
   }
 #pragma unroll
   for (int cb = 0; cb < CBW; ++cb) {
-    const int64_t col = col0 + wc * 16 * CBW + cb * 16 + li;
+    const int64_t col = col0 + cb * 16 + li;
     if (col >= Ncol) continue;
     const double ci = (double)colinv[col];
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int64_t row = row0 + wr * 64 + r * 16 + 4 * lk + j;
+        const int64_t row = row0 + r * 16 + 4 * lk + j;
         if (row < R) C[row * ldc + col] = acc[r][cb][j] * ci;
       }
   }
@@ -411,16 +401,19 @@ gemm_f32_kernel(int64_t R, int64_t K, int64_t Ncol, const float *__restrict__ A,
 }
 
 // One workgroup per MO i: column scale from max_k |C[i][k]| and the split
-// terms of C[i][k] * scale, zero past K.
+// terms of C[i][k] * scale at Bs[(i*T + t)*ldb + k], zero past K and for the
+// padding MOs i >= M (FP32 form: the scaled-by-one values at Bs[i*ldb + k]).
 template <int SP>
 __global__ void __launch_bounds__(kThreads)
 split_coef_kernel(int64_t M, int64_t K, const double *__restrict__ coef, int64_t ldb,
                   uint16_t *__restrict__ Bs, float *__restrict__ colinv) {
   __shared__ float red[kThreads];
   const int64_t i = blockIdx.x;
-  const double *row = coef + i * K;
+  const bool real = i < M;
+  const double *row = coef + (real ? i : 0) * K;
   float m = 0.f;
-  for (int64_t k = threadIdx.x; k < K; k += kThreads) m = fmaxf(m, fabsf((float)row[k]));
+  if (real)
+    for (int64_t k = threadIdx.x; k < K; k += kThreads) m = fmaxf(m, fabsf((float)row[k]));
   red[threadIdx.x] = m;
   __syncthreads();
   for (int s = kThreads / 2; s > 0; s >>= 1) {
@@ -428,37 +421,38 @@ split_coef_kernel(int64_t M, int64_t K, const double *__restrict__ coef, int64_t
     __syncthreads();
   }
   const float sc = SP == QMCKL_AMD_SPLIT_FP32 ? 1.0f : pow2_scale(red[0]);
-  if (threadIdx.x == 0) colinv[i] = 1.0f / sc;
+  if (threadIdx.x == 0 && real) colinv[i] = 1.0f / sc;
   for (int64_t k = threadIdx.x; k < ldb; k += kThreads) {
-    const float x = k < K ? (float)row[k] * sc : 0.0f;
+    const float x = real && k < K ? (float)row[k] * sc : 0.0f;
     if constexpr (SP == QMCKL_AMD_SPLIT_FP32) {
       reinterpret_cast<float *>(Bs)[i * ldb + k] = x;
     } else {
-      split_store<SP>(x, Bs + i * ldb + k, (int)(M * ldb));
+      split_store<SP>(x, Bs + i * Terms<SP>::n * ldb + k, (int)ldb);
     }
   }
+}
+
+template <int CBW>
+hipError_t launch_f32(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const float *A, int64_t lda,
+                      const void *B, int64_t ldb, const uint8_t *flags, int nkb, double *C, int64_t ldc) {
+  constexpr int BN = 32 * CBW;
+  const int64_t gx = (R + kBM - 1) / kBM, gy = (Ncol + BN - 1) / BN;
+  if (gx > 0x7fffffff || gy > 65535) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(gemm_f32_kernel<CBW>, dim3((unsigned)gx, (unsigned)gy), dim3(kThreads), 0, s, R, K,
+                     Ncol, A, lda, (const float *)B, ldb, flags, nkb, 2, C, ldc);
+  return hipGetLastError();
 }
 
 template <int SP, int CBW>
 hipError_t launch_split(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const float *A, int64_t lda,
                         const void *B, int64_t ldb, const float *colinv, const uint8_t *flags, int nkb,
                         double *C, int64_t ldc) {
-  constexpr int BN = 32 * CBW;
-  if constexpr (SP == QMCKL_AMD_SPLIT_FP32) {
-    const int64_t gx = (R + kBM - 1) / kBM, gy = (Ncol + BN - 1) / BN;
-    if (gx > 0x7fffffff || gy > 65535) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(gemm_f32_kernel<CBW>, dim3((unsigned)gx, (unsigned)gy), dim3(kThreads), 0, s, R,
-                       K, Ncol, A, lda, (const float *)B, ldb, flags, nkb, 2, C, ldc);
-  } else {
-    const int64_t row_tiles = (R + kSM - 1) / kSM;
-    const int64_t col_tiles = (Ncol + BN - 1) / BN;
-    const int64_t nwg = (row_tiles + 7) / 8 * 8 * col_tiles;
-    if (nwg > 0x7fffffff || (lda % kStage) != 0 || (ldb % kStage) != 0) return hipErrorInvalidValue;
-    if (flags != nullptr && (nkb > kMaxFlagBlocks || nkb * 32 < K)) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((gemm_split_kernel<SP, CBW>), dim3((unsigned)nwg), dim3(kThreads), 0, s, R, K, Ncol,
-                       A, lda, (const uint16_t *)B, ldb, colinv, flags, nkb, row_tiles, (int)col_tiles, C,
-                       ldc);
-  }
+  const int64_t gx = (R + kRM - 1) / kRM, gy = (Ncol + 64 * CBW - 1) / (64 * CBW);
+  if (gx > 0x7fffffff || gy > 65535 || (lda % kStage) != 0 || (ldb % kStage) != 0)
+    return hipErrorInvalidValue;
+  if (flags != nullptr && (nkb > kMaxFlagBlocks || (int64_t)nkb * 32 < K)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((gemm_split_kernel<SP, CBW>), dim3((unsigned)gx, (unsigned)gy), dim3(kThreads), 0, s,
+                     R, K, Ncol, A, lda, (const uint16_t *)B, ldb, colinv, flags, nkb, C, ldc);
   return hipGetLastError();
 }
 
@@ -466,12 +460,24 @@ template <int SP>
 hipError_t launch_form(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const float *A, int64_t lda,
                        const void *B, int64_t ldb, const float *colinv, const uint8_t *flags, int nkb,
                        double *C, int64_t ldc) {
-  const int64_t nb = (Ncol + 31) / 32;  // 32-column groups
-  if (nb <= 1) return launch_split<SP, 1>(s, R, K, Ncol, A, lda, B, ldb, colinv, flags, nkb, C, ldc);
-  if (nb <= 2) return launch_split<SP, 2>(s, R, K, Ncol, A, lda, B, ldb, colinv, flags, nkb, C, ldc);
-  if (nb <= 3) return launch_split<SP, 3>(s, R, K, Ncol, A, lda, B, ldb, colinv, flags, nkb, C, ldc);
-  if (nb <= 4) return launch_split<SP, 4>(s, R, K, Ncol, A, lda, B, ldb, colinv, flags, nkb, C, ldc);
-  return launch_split<SP, 5>(s, R, K, Ncol, A, lda, B, ldb, colinv, flags, nkb, C, ldc);
+  if constexpr (SP == QMCKL_AMD_SPLIT_FP32) {
+    const int64_t nb = (Ncol + 31) / 32;  // 32-column groups
+    if (nb <= 1) return launch_f32<1>(s, R, K, Ncol, A, lda, B, ldb, flags, nkb, C, ldc);
+    if (nb <= 2) return launch_f32<2>(s, R, K, Ncol, A, lda, B, ldb, flags, nkb, C, ldc);
+    if (nb <= 3) return launch_f32<3>(s, R, K, Ncol, A, lda, B, ldb, flags, nkb, C, ldc);
+    if (nb <= 4) return launch_f32<4>(s, R, K, Ncol, A, lda, B, ldb, flags, nkb, C, ldc);
+    return launch_f32<5>(s, R, K, Ncol, A, lda, B, ldb, flags, nkb, C, ldc);
+  } else {
+    const int cbw = split_wg_cols(SP, Ncol) / 64;
+    if (cbw == 1) return launch_split<SP, 1>(s, R, K, Ncol, A, lda, B, ldb, colinv, flags, nkb, C, ldc);
+    if (cbw == 2) return launch_split<SP, 2>(s, R, K, Ncol, A, lda, B, ldb, colinv, flags, nkb, C, ldc);
+    if (cbw == 3) return launch_split<SP, 3>(s, R, K, Ncol, A, lda, B, ldb, colinv, flags, nkb, C, ldc);
+    if constexpr (Terms<SP>::n == 2) {
+      if (cbw == 4) return launch_split<SP, 4>(s, R, K, Ncol, A, lda, B, ldb, colinv, flags, nkb, C, ldc);
+      if (cbw == 5) return launch_split<SP, 5>(s, R, K, Ncol, A, lda, B, ldb, colinv, flags, nkb, C, ldc);
+    }
+    return hipErrorInvalidValue;
+  }
 }
 
 }  // namespace
@@ -489,18 +495,19 @@ int split_terms(int form) {
 hipError_t launch_split_coef(hipStream_t s, int form, int64_t M, int64_t K, const double *coef,
                              int64_t ldb, uint16_t *B, float *colinv) {
   if (M <= 0) return hipSuccess;
+  const unsigned g = (unsigned)split_cols_padded(form, M);
   switch (form) {
     case QMCKL_AMD_SPLIT_FP16X2:
-      hipLaunchKernelGGL(split_coef_kernel<QMCKL_AMD_SPLIT_FP16X2>, dim3((unsigned)M), dim3(kThreads), 0, s, M, K, coef, ldb, B, colinv);
+      hipLaunchKernelGGL(split_coef_kernel<QMCKL_AMD_SPLIT_FP16X2>, dim3(g), dim3(kThreads), 0, s, M, K, coef, ldb, B, colinv);
       break;
     case QMCKL_AMD_SPLIT_BF16X2:
-      hipLaunchKernelGGL(split_coef_kernel<QMCKL_AMD_SPLIT_BF16X2>, dim3((unsigned)M), dim3(kThreads), 0, s, M, K, coef, ldb, B, colinv);
+      hipLaunchKernelGGL(split_coef_kernel<QMCKL_AMD_SPLIT_BF16X2>, dim3(g), dim3(kThreads), 0, s, M, K, coef, ldb, B, colinv);
       break;
     case QMCKL_AMD_SPLIT_BF16X3:
-      hipLaunchKernelGGL(split_coef_kernel<QMCKL_AMD_SPLIT_BF16X3>, dim3((unsigned)M), dim3(kThreads), 0, s, M, K, coef, ldb, B, colinv);
+      hipLaunchKernelGGL(split_coef_kernel<QMCKL_AMD_SPLIT_BF16X3>, dim3(g), dim3(kThreads), 0, s, M, K, coef, ldb, B, colinv);
       break;
     case QMCKL_AMD_SPLIT_FP32:
-      hipLaunchKernelGGL(split_coef_kernel<QMCKL_AMD_SPLIT_FP32>, dim3((unsigned)M), dim3(kThreads), 0, s, M, K, coef, ldb, B, colinv);
+      hipLaunchKernelGGL(split_coef_kernel<QMCKL_AMD_SPLIT_FP32>, dim3(g), dim3(kThreads), 0, s, M, K, coef, ldb, B, colinv);
       break;
     default: return hipErrorInvalidValue;
   }

@@ -67,7 +67,8 @@ qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Contex
     e = hipMemsetAsync(ao.ao_f32, 0, ao.ao_f32_cap, c->stream);
   if (e == hipSuccess && (mo.coef_split == nullptr || mo.coef_split_form != form)) {
     dfree(c, mo.coef_split);
-    mo.coef_split = static_cast<uint16_t *>(dmalloc(c, sizeof(uint16_t) * split_terms(form) * M * ld));
+    mo.coef_split = static_cast<uint16_t *>(
+        dmalloc(c, sizeof(uint16_t) * split_terms(form) * split_cols_padded(form, M) * ld));
     if (!drealloc(c, reinterpret_cast<void **>(&mo.coef_colinv), sizeof(float) * M) || !mo.coef_split)
       return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
     e = launch_split_coef(c->stream, form, M, A, mo.coefficient, ld, mo.coef_split, mo.coef_colinv);

@@ -9,16 +9,20 @@
 // Work decomposition (one workgroup = 64 points x all AOs, 4 wavefronts).
 // The shells, in AO order, are cut at finalize into chunks of at most 16
 // shells and 64 AOs.  For each chunk:
-//   - its shell descriptors and (exponent, normalised coefficient) pairs are
-//     staged in LDS;
-//   - phase 1: each wavefront takes whole shells, lane = point, so the
-//     primitive loop is wavefront-uniform (LDS broadcast reads) and a
+//   - phase 1: each wavefront takes whole shells, lane = point.  The shell
+//     index is wavefront-uniform, so the shell descriptor, the nucleus and
+//     the (exponent, coefficient) pairs come through scalar loads, and a
 //     primitive screened for all 64 points costs no exp (the branch is
 //     skipped when no lane's predicate holds).  R, dR/dx, dR/dy, dR/dz and
-//     lap R land in LDS [shell][5][point];
+//     lap R land in LDS [shell][component][point];
 //   - phase 2: lane = AO of the chunk (its shell, exponents (a,b,c), factor
 //     and nucleus are loaded once), wavefront w walks points w, w+4, ...;
-//     every store of a wavefront is one contiguous run of a [p][c][*] row.
+//     every store of a wavefront is one contiguous run of a [p][c][*] row,
+//     through a pointer advanced by a constant stride.  Nonzero flags are
+//     gathered with wavefront ballots into a scalar bit mask per chunk and
+//     written to LDS once per chunk and wavefront.
+// Two barriers per chunk: phase 1 -> phase 2, and phase 2 -> the next
+// chunk's phase 1 (which overwrites the radial tile).
 // r^2 is fma(x,x,fma(y,y,z*z)), as in the CPU oracle, so the screens decide
 // identically.  The kernel is bound by the HBM write of ao_vgl:
 // 40 B (FP64) or 20 B (FP32) per (point, AO) plus 24 B per point.
@@ -32,7 +36,6 @@ namespace {
 
 constexpr int kTP = 64;            // points per workgroup
 constexpr int kThreads = 256;      // 4 wavefronts
-constexpr int kPrimLds = 384;      // staged primitives per chunk
 constexpr int kRS = 5 * kTP + 2;   // LDS stride between shells (bank spread)
 
 struct Mono {
@@ -85,12 +88,9 @@ ao_kernel(int64_t P, int N, const double *__restrict__ coord, const double *__re
   extern __shared__ unsigned char s_flag[];  // [NCOMP row tiles][nkb] nonzero flags
   __shared__ double rad[kChunkShells * kRS];
   __shared__ double pt[3][kTP];
-  __shared__ double2 prim[kPrimLds];
-  __shared__ ShellDesc sd[kChunkShells];
-  __shared__ int prim_off[kChunkShells + 1];
 
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int64_t p0 = (int64_t)blockIdx.x * kTP;
   const int np = (int)((P - p0) < kTP ? (P - p0) : kTP);
   const bool track_flags = ao.flags != nullptr;
@@ -104,34 +104,18 @@ ao_kernel(int64_t P, int N, const double *__restrict__ coord, const double *__re
   }
   if (track_flags)
     for (int i = tid; i < NCOMP * ao.nkb; i += kThreads) s_flag[i] = 0;
+  __syncthreads();
+
+  const double px = pt[0][lane], py = pt[1][lane], pz = pt[2][lane];
+  const int64_t ostep = (int64_t)(kThreads / 64) * NCOMP * ao.ld;  // stride between a wavefront's points
 
   for (int ch = 0; ch < nchunks; ++ch) {
     const int s0 = chunks[ch];
     const int ns = chunks[ch + 1] - s0;
-    __syncthreads();  // the previous chunk is done with rad/sd/prim
-    if (tid < ns) sd[tid] = shells[s0 + tid];
-    __syncthreads();
-    if (tid == 0) {
-      int off = 0;
-      for (int i = 0; i < ns; ++i) {
-        prim_off[i] = off;
-        off += sd[i].prim_num;
-      }
-      prim_off[ns] = off;
-    }
-    __syncthreads();
-    const bool staged = !FROM_TABLE && prim_off[ns] <= kPrimLds;
-    if (staged) {
-      for (int i = 0; i < ns; ++i) {
-        const int base = sd[i].prim_start, cnt = sd[i].prim_num, o = prim_off[i];
-        for (int j = tid; j < cnt; j += kThreads) prim[o + j] = make_double2(expo[base + j], coef[base + j]);
-      }
-    }
-    __syncthreads();
 
     // phase 1: radial parts, wavefront per shell, lane per point
     for (int sl = wave; sl < ns; sl += kThreads / 64) {
-      const ShellDesc d = sd[sl];
+      const ShellDesc d = shells[s0 + sl];
       double t[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
       if (FROM_TABLE) {
         if (lane < np) {
@@ -140,24 +124,18 @@ ao_kernel(int64_t P, int N, const double *__restrict__ coord, const double *__re
           for (int c = 0; c < NCOMP; ++c) t[c] = row[(int64_t)c * S_table];
         }
       } else {
-        const double x = pt[0][lane] - nucl[d.nucl];
-        const double y = pt[1][lane] - nucl[N + d.nucl];
-        const double z = pt[2][lane] - nucl[2 * N + d.nucl];
+        const double x = px - nucl[d.nucl];
+        const double y = py - nucl[N + d.nucl];
+        const double z = pz - nucl[2 * N + d.nucl];
         const double r2 = fma(x, x, fma(y, y, z * z));
         const bool live = lane < np && !(r2 > kCutoff * nucl_range[d.nucl]);
+        const double *ex = expo + d.prim_start;
+        const double *cf = coef + d.prim_start;
         for (int j = 0; j < d.prim_num; ++j) {
-          double a, cn;
-          if (staged) {
-            const double2 pr = prim[prim_off[sl] + j];
-            a = pr.x;
-            cn = pr.y;
-          } else {
-            a = expo[d.prim_start + j];
-            cn = coef[d.prim_start + j];
-          }
+          const double a = ex[j];
           const double ar2 = a * r2;
           if (live && !(ar2 > kCutoff)) {
-            const double v = cn * exp(-ar2);
+            const double v = cf[j] * exp(-ar2);
             t[0] += v;
             if (NCOMP == 5) {
               const double two_a = -2.0 * a * v;
@@ -175,52 +153,63 @@ ao_kernel(int64_t P, int N, const double *__restrict__ coord, const double *__re
     __syncthreads();
 
     // phase 2: lane = AO of the chunk, wavefront walks its points
-    const int ka = sd[0].ao_start;
-    const int kb = sd[ns - 1].ao_start + (sd[ns - 1].l + 1) * (sd[ns - 1].l + 2) / 2;
-    const bool act = lane < kb - ka;
-    const int k = ka + lane;
-    int sl = 0, ea = 0, eb = 0, ec = 0;
-    double f = 0.0, nx = 0.0, ny = 0.0, nz = 0.0;
-    if (act) {
-      sl = ao_shell[k] - s0;
-      const int abc = ao_abc[k];
-      ea = abc & 0xff;
-      eb = (abc >> 8) & 0xff;
-      ec = (abc >> 16) & 0xff;
-      f = ao_factor[k];
-      const int n = sd[sl].nucl;
-      nx = nucl[n];
-      ny = nucl[N + n];
-      nz = nucl[2 * N + n];
-    }
+    const ShellDesc first = shells[s0], last = shells[s0 + ns - 1];
+    const int ka = first.ao_start;
+    const int nao = last.ao_start + (last.l + 1) * (last.l + 2) / 2 - ka;
+    const bool act = lane < nao;
+    const int k = ka + (act ? lane : 0);
+    const int sl = ao_shell[k] - s0;
+    const int abc = ao_abc[k];
+    const int ea = abc & 0xff, eb = (abc >> 8) & 0xff, ec = (abc >> 16) & 0xff;
+    const double f = ao_factor[k];
+    const int n = shells[s0 + sl].nucl;
+    const double nx = nucl[n], ny = nucl[N + n], nz = nucl[2 * N + n];
     const double *rs = rad + sl * kRS;
-    for (int pl = wave; pl < np; pl += kThreads / 64) {
-      double v[NCOMP];
-#pragma unroll
-      for (int c = 0; c < NCOMP; ++c) v[c] = 0.0;
+    // lanes of the chunk's (up to three) 32-AO flag blocks
+    const int kb0 = ka >> 5;
+    const int cut0 = 32 * (kb0 + 1) - ka, cut1 = cut0 + 32;
+    const uint64_t act_mask = nao >= 64 ? ~0ull : ((1ull << nao) - 1ull);
+    const uint64_t bm0 = act_mask & ((1ull << cut0) - 1ull);
+    const uint64_t bm1 = act_mask & ~((1ull << cut0) - 1ull) & (cut1 >= 64 ? ~0ull : ((1ull << cut1) - 1ull));
+    const uint64_t bm2 = act_mask & (cut1 >= 64 ? 0ull : ~((1ull << cut1) - 1ull));
+    uint32_t fmask = 0;  // bit 3*tile + block: some entry of (row tile, block) is nonzero
+
+    OutT *o = ao.out + (p0 + wave) * NCOMP * ao.ld + k;
+    for (int pl = wave; pl < np; pl += kThreads / 64, o += ostep) {
+      const Mono m = monomial(ea, eb, ec, pt[0][pl] - nx, pt[1][pl] - ny, pt[2][pl] - nz);
+      const double R = rs[pl];
+      OutT w[NCOMP];
+      w[0] = (OutT)(m.v * R * f);
+      if (NCOMP == 5) {
+        const double Rx = rs[1 * kTP + pl], Ry = rs[2 * kTP + pl];
+        const double Rz = rs[3 * kTP + pl], RL = rs[4 * kTP + pl];
+        w[1 % NCOMP] = (OutT)((m.gx * R + m.v * Rx) * f);
+        w[2 % NCOMP] = (OutT)((m.gy * R + m.v * Ry) * f);
+        w[3 % NCOMP] = (OutT)((m.gz * R + m.v * Rz) * f);
+        w[4 % NCOMP] = (OutT)((m.lap * R + m.v * RL + 2.0 * (m.gx * Rx + m.gy * Ry + m.gz * Rz)) * f);
+      }
       if (act) {
-        const Mono m = monomial(ea, eb, ec, pt[0][pl] - nx, pt[1][pl] - ny, pt[2][pl] - nz);
-        const double R = rs[pl];
-        v[0] = m.v * R * f;
-        if (NCOMP == 5) {
-          const double Rx = rs[1 * kTP + pl], Ry = rs[2 * kTP + pl];
-          const double Rz = rs[3 * kTP + pl], RL = rs[4 * kTP + pl];
-          v[1 % NCOMP] = (m.gx * R + m.v * Rx) * f;
-          v[2 % NCOMP] = (m.gy * R + m.v * Ry) * f;
-          v[3 % NCOMP] = (m.gz * R + m.v * Rz) * f;
-          v[4 % NCOMP] = (m.lap * R + m.v * RL + 2.0 * (m.gx * Rx + m.gy * Ry + m.gz * Rz)) * f;
+#pragma unroll
+        for (int c = 0; c < NCOMP; ++c) o[c * ao.ld] = w[c];
+      }
+      if (track_flags) {
+#pragma unroll
+        for (int c = 0; c < NCOMP; ++c) {
+          const uint64_t nzm = __ballot(w[c] != (OutT)0);
+          const int tile = (pl * NCOMP + c) >> 6;
+          fmask |= ((nzm & bm0) ? 1u : 0u) << (3 * tile);
+          fmask |= ((nzm & bm1) ? 2u : 0u) << (3 * tile);
+          fmask |= ((nzm & bm2) ? 4u : 0u) << (3 * tile);
         }
       }
-      OutT *o = ao.out + ((p0 + pl) * NCOMP) * ao.ld + k;
-#pragma unroll
-      for (int c = 0; c < NCOMP; ++c) {
-        const OutT w = (OutT)v[c];
-        if (act) o[c * ao.ld] = w;
-        if (track_flags && w != (OutT)0) s_flag[((pl * NCOMP + c) >> 6) * ao.nkb + (k >> 5)] = 1;
-      }
     }
+    if (track_flags && lane == 0 && fmask != 0) {
+      for (int tile = 0; tile < NCOMP; ++tile)
+        for (int b = 0; b < 3; ++b)
+          if (((fmask >> (3 * tile + b)) & 1u) && kb0 + b < ao.nkb) s_flag[tile * ao.nkb + kb0 + b] = 1;
+    }
+    __syncthreads();  // phase 2 is done with the radial tile
   }
-  __syncthreads();
   if (track_flags) {
     // this workgroup owns rows [NCOMP*p0, NCOMP*(p0+np)): whole 64-row tiles
     const int64_t rt0 = (p0 * NCOMP) >> 6;

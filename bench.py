@@ -111,6 +111,56 @@ def run_form(ctx, s, form, steps, warmup):
     return (t1 - t0) / steps, kernels, mo
 
 
+def bench_batch(forms, steps, warmup, scale=1.0):
+    """Batching form: the 48-molecule ragged batch (2-64 nuclei, 30-1,000 AOs,
+    16-512 walkers) through qmckl_get_mo_basis_mo_vgl_batch_device, one
+    context per molecule, timed per form; next to it the same molecules one
+    inplace getter call after another (each call synchronises)."""
+    import torch
+    from qmckl_sycl_amd import systems
+    from qmckl_sycl_amd.context import Context, DeviceArray, batch_mo_vgl, lib
+    L = lib()
+    mols = systems.ragged_batch(48, scale=scale)
+    points = int(sum(m.point_num for m in mols))
+    ctxs, outs = [], []
+    for m in mols:
+        c = Context(0)
+        c.set_system(m)
+        ctxs.append(c)
+        outs.append(DeviceArray(c, (m.point_num, 5, m.mo_num)))
+    res = {"molecules": len(mols), "points": points,
+           "ao_range": [int(min(m.ao_num for m in mols)), int(max(m.ao_num for m in mols))],
+           "points_range": [int(min(m.point_num for m in mols)), int(max(m.point_num for m in mols))]}
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    try:
+        for form in forms:
+            for c in ctxs:
+                c.check(L.qmckl_set_mo_basis_precision_device(c.handle, FORM_IDS[form]))
+            sec = timed(lambda: batch_mo_vgl(ctxs, outs))
+            res[form] = {"ms_per_step": sec * 1e3, "value": points / sec}
+
+            def one_by_one():
+                for c, o in zip(ctxs, outs):
+                    c.check(L.qmckl_get_mo_basis_mo_vgl_inplace_device(c.handle, o.ptr, o.size))
+            sec = timed(one_by_one)
+            res[form]["one_call_per_molecule"] = {"ms_per_step": sec * 1e3, "value": points / sec}
+            log("batch", form, json.dumps(res[form]))
+    finally:
+        for c in ctxs:
+            c.destroy()
+    return res
+
+
 def roofline(kernels, s):
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
     kind, work = kernel_work(dom, s)
@@ -193,6 +243,10 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the FP64 plain comparison run")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the 48-molecule ragged batch measurement")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only run the 48-molecule ragged batch measurement")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (the reference "
                          "kernel for scaling HBM traffic counters) and exit")
@@ -203,6 +257,10 @@ def main():
         sys.exit(2)
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
+        return
+    if args.batch_only:
+        print(json.dumps({"batch_48_ragged": bench_batch(["plain", args.form], args.steps, args.warmup)}),
+              flush=True)
         return
 
     from qmckl_sycl_amd.context import Context
@@ -238,6 +296,8 @@ def main():
                              "kernels": results["plain"]["kernels"],
                              "roofline": roofline(results["plain"]["kernels"], s)}
         out["speedup_vs_plain_fp64"] = head["value"] / results["plain"]["value"]
+    if not args.no_batch:
+        out["batch_48_ragged"] = bench_batch(["plain", args.form], max(2, args.steps // 3), 1)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(s, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -361,6 +361,24 @@ qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_adjoint_device(qmckl_context_de
                                                                 double *d_exponent,
                                                                 double *d_mo_coefficient);
 
+/* ---- batching form (new; no counterpart in the reference) -----------------
+ * Evaluates mo_vgl for count contexts (one molecule each, on one device) in
+ * one submission: for each i it does what
+ * qmckl_get_mo_basis_mo_vgl_inplace_device(contexts[i], mo_vgl[i],
+ * size_max[i]) does (touch, then distances, AOs and MOs in the context's
+ * precision form, written to mo_vgl[i] [P_i][5][M_i]), but the work of every
+ * context is enqueued on that context's own stream before any of them is
+ * waited for, so molecules of different sizes run concurrently on the GPU.
+ * Returns when every output is complete.  contexts, mo_vgl and size_max are
+ * host arrays of count entries (mo_vgl[i] a device pointer).  Errors: count
+ * <= 0 -> ARG_1; a NULL array -> ARG_2/3/4; an invalid context ->
+ * INVALID_CONTEXT; contexts on different devices -> ARG_2; otherwise each
+ * context's own code (recorded in that context), the first one returned. */
+qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_batch_device(int64_t count,
+                                                              const qmckl_context_device *contexts,
+                                                              double *const *mo_vgl,
+                                                              const int64_t *size_max);
+
 /* ---- kernel timing (new; no counterpart in the reference) ----------------
  * When enabled, every kernel the context launches is bracketed by HIP events
  * on the context's stream and its elapsed time is accumulated under the

@@ -252,3 +252,18 @@ class Context:
 
     def mo_value(self, P):
         return self._get_array("qmckl_get_mo_basis_mo_value_device", (P, self.mo_num()))
+
+
+def batch_mo_vgl(contexts, outputs):
+    """qmckl_get_mo_basis_mo_vgl_batch_device over Context objects and their
+    DeviceArray outputs ([P][5][M] each); raises on the returned code."""
+    n = len(contexts)
+    if n != len(outputs):
+        raise ValueError("one output per context")
+    handles = (ctypes.c_int64 * n)(*[c.handle for c in contexts])
+    ptrs = (ctypes.c_void_p * n)(*[o.ptr for o in outputs])
+    sizes = (ctypes.c_int64 * n)(*[o.size for o in outputs])
+    rc = lib().qmckl_get_mo_basis_mo_vgl_batch_device(n, handles, ptrs, sizes)
+    if rc != 0:
+        raise QmcklError(rc, "qmckl_get_mo_basis_mo_vgl_batch_device")
+    return rc

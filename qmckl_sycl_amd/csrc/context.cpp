@@ -7,15 +7,32 @@
 // stream on its device; the memory table is a hash map keyed by pointer.
 #include <cstdio>
 #include <cstdlib>
+#include <mutex>
 #include <new>
+#include <unordered_set>
 
 #include "internal.hpp"
 
 namespace qmckl_amd {
 
+// Live contexts: a handle is dereferenced only if create returned it and
+// destroy has not retired it, so a stale or arbitrary handle is reported as
+// an invalid context instead of being read.
+namespace {
+std::mutex g_live_mutex;
+std::unordered_set<const Context *> &live_contexts() {
+  static std::unordered_set<const Context *> live;
+  return live;
+}
+}  // namespace
+
 Context *ctx_of(qmckl_context_device context) {
   if (context == QMCKL_NULL_CONTEXT_DEVICE) return nullptr;
   Context *c = reinterpret_cast<Context *>(context);
+  {
+    std::lock_guard<std::mutex> g(g_live_mutex);
+    if (live_contexts().count(c) == 0) return nullptr;
+  }
   if (c->tag != kValidTag) return nullptr;
   return c;
 }
@@ -82,6 +99,7 @@ bool drealloc(Context *c, void **slot, size_t bytes) {
 
 hipError_t sync(Context *c) {
   if (!has_device(c)) return hipErrorNoDevice;
+  if (c->defer_sync) return hipSuccess;
   return hipStreamSynchronize(c->stream);
 }
 
@@ -216,12 +234,20 @@ qmckl_context_device qmckl_context_create_device(int device_id) {
     }
     c->device_id = device_id;
   }
+  {
+    std::lock_guard<std::mutex> g(g_live_mutex);
+    live_contexts().insert(c);
+  }
   return reinterpret_cast<qmckl_context_device>(c);
 }
 
 qmckl_exit_code_device qmckl_context_destroy_device(qmckl_context_device context) {
   Context *c = ctx_of(context);
   if (c == nullptr) return QMCKL_INVALID_CONTEXT_DEVICE;
+  {
+    std::lock_guard<std::mutex> g(g_live_mutex);
+    live_contexts().erase(c);
+  }
   if (c->stream) {
     (void)hipSetDevice(c->device_id);
     (void)hipStreamSynchronize(c->stream);

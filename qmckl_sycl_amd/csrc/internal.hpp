@@ -171,6 +171,9 @@ struct Context {
   std::unordered_map<void *, size_t> host_allocs;
   uint64_t date = 0;
   bool timing = false;
+  // set while a batch enqueues this context's work: the compute chain's
+  // stream synchronisations are skipped and the batch waits once at the end
+  bool defer_sync = false;
   struct TimerAcc {
     int64_t launches = 0;
     double total_ms = 0.0;

@@ -363,4 +363,52 @@ qmckl_exit_code_device qmckl_get_mo_basis_precision_device(qmckl_context_device 
   return QMCKL_SUCCESS_DEVICE;
 }
 
+// Batching form: each context's chain is enqueued on its own stream with the
+// chain's synchronisations deferred, then every stream is waited for once.
+qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_batch_device(int64_t count,
+                                                              const qmckl_context_device *contexts,
+                                                              double *const *mo_vgl,
+                                                              const int64_t *size_max) {
+  const char *fn = "qmckl_get_mo_basis_mo_vgl_batch_device";
+  if (count <= 0) return QMCKL_INVALID_ARG_1_DEVICE;
+  if (contexts == nullptr) return QMCKL_INVALID_ARG_2_DEVICE;
+  if (mo_vgl == nullptr) return QMCKL_INVALID_ARG_3_DEVICE;
+  if (size_max == nullptr) return QMCKL_INVALID_ARG_4_DEVICE;
+  Context *c0 = ctx_of(contexts[0]);
+  if (!c0) return QMCKL_INVALID_CONTEXT_DEVICE;
+  for (int64_t i = 0; i < count; ++i) {
+    Context *c = ctx_of(contexts[i]);
+    if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+    if (c->device_id != c0->device_id)
+      return fail(contexts[i], QMCKL_INVALID_ARG_2_DEVICE, fn, "contexts on different devices");
+    if (mo_vgl[i] == nullptr) return fail(contexts[i], QMCKL_INVALID_ARG_3_DEVICE, fn, "NULL pointer");
+    if (size_max[i] < 5 * c->mo.mo_num * c->point.num)
+      return fail(contexts[i], QMCKL_INVALID_ARG_4_DEVICE, fn, "output array too small");
+  }
+  for (int64_t i = 0; i < count; ++i) {
+    qmckl_exit_code_device rc = ready(contexts[i], ctx_of(contexts[i]), fn);
+    if (rc != QMCKL_SUCCESS_DEVICE) return rc;
+  }
+  qmckl_exit_code_device first = QMCKL_SUCCESS_DEVICE;
+  int64_t enqueued = 0;
+  for (; enqueued < count; ++enqueued) {
+    Context *c = ctx_of(contexts[enqueued]);
+    c->defer_sync = true;
+    qmckl_exit_code_device rc = qmckl_context_touch_device(contexts[enqueued]);
+    if (rc == QMCKL_SUCCESS_DEVICE) rc = compute_mo_vgl(contexts[enqueued], c, mo_vgl[enqueued], fn);
+    c->defer_sync = false;
+    if (rc != QMCKL_SUCCESS_DEVICE) {
+      first = rc;
+      ++enqueued;
+      break;
+    }
+  }
+  for (int64_t i = 0; i < enqueued; ++i) {
+    Context *c = ctx_of(contexts[i]);
+    const qmckl_exit_code_device rc = hip_check(contexts[i], sync(c), fn);
+    if (first == QMCKL_SUCCESS_DEVICE) first = rc;
+  }
+  return first;
+}
+
 }  // extern "C"

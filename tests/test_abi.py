@@ -255,3 +255,35 @@ def test_kernel_timing_api(ctx):
     assert (n.value, ms.value) == (0, 0.0)
     ctx.call("qmckl_reset_kernel_timing_device")
     assert ctx.rc("qmckl_get_kernel_timing_device", None, None, None) == 2
+
+
+def test_batch_validation(ctx):
+    L = lib()
+    f = L.qmckl_get_mo_basis_mo_vgl_batch_device
+    one = (ctypes.c_int64 * 1)(ctx.handle)
+    ptr = (ctypes.c_void_p * 1)(1234)
+    size = (ctypes.c_int64 * 1)(10)
+    assert f(0, one, ptr, size) == 1
+    assert f(-3, one, ptr, size) == 1
+    assert f(1, None, ptr, size) == 2
+    assert f(1, one, None, size) == 3
+    assert f(1, one, ptr, None) == 4
+    bad = (ctypes.c_int64 * 2)(ctx.handle, 77)
+    ptr2 = (ctypes.c_void_p * 2)(1234, 1234)
+    size2 = (ctypes.c_int64 * 2)(10, 10)
+    assert f(2, bad, ptr2, size2) == INVALID_CONTEXT
+    nullout = (ctypes.c_void_p * 1)(None)
+    assert f(1, one, nullout, size) == 3
+    assert ctx.last_error()[1] == "qmckl_get_mo_basis_mo_vgl_batch_device"
+    # an unprovided basis is reported as the single-context getter reports it
+    assert f(1, one, ptr, size) == NOT_PROVIDED
+
+
+def test_destroyed_context_is_invalid():
+    c = Context(NO_DEVICE)
+    h = c.handle
+    c.destroy()
+    L = lib()
+    n = ctypes.c_int64(0)
+    assert L.qmckl_get_nucleus_num_device(h, ctypes.byref(n)) == INVALID_CONTEXT
+    assert L.qmckl_context_destroy_device(h) == INVALID_CONTEXT

@@ -1,0 +1,89 @@
+"""Batching form (ragged molecules, one submission) on the GPU through the
+C ABI: qmckl_get_mo_basis_mo_vgl_batch_device over many contexts must give,
+for every molecule, exactly what that molecule's own inplace getter gives,
+and the FP64 results must match the oracle."""
+import numpy as np
+import pytest
+
+from qmckl_sycl_amd import systems as S
+from qmckl_sycl_amd.context import Context, DeviceArray, QmcklError, batch_mo_vgl
+
+from helpers import RTOL_VGL, assert_vgl_close, say
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def molecules():
+    # the 48-molecule ragged batch, walker counts scaled down for test time
+    return S.ragged_batch(48, scale=0.04)
+
+
+def setup(mols, forms=None):
+    ctxs, outs = [], []
+    for i, m in enumerate(mols):
+        c = Context(0)
+        c.set_system(m)
+        if forms is not None:
+            c.call("qmckl_set_mo_basis_precision_device", forms[i % len(forms)])
+        ctxs.append(c)
+        outs.append(DeviceArray(c, (m.point_num, 5, m.mo_num)))
+    return ctxs, outs
+
+
+def single(c, m):
+    out = DeviceArray(c, (m.point_num, 5, m.mo_num))
+    c.call("qmckl_get_mo_basis_mo_vgl_inplace_device", out.ptr, out.size)
+    return out.numpy()
+
+
+def test_batch_equals_single_fp64(molecules, oracle_lib):
+    say("points per molecule:", [m.point_num for m in molecules])
+    say("AOs per molecule:", [m.ao_num for m in molecules])
+    ctxs, outs = setup(molecules)
+    try:
+        batch_mo_vgl(ctxs, outs)
+        got = [o.numpy() for o in outs]
+        for i, (c, m) in enumerate(zip(ctxs, molecules)):
+            ref = single(c, m)
+            assert np.array_equal(got[i], ref), m.name
+        # FP64 parity with the oracle on the smallest, the largest and two
+        # others (all their points)
+        order = np.argsort([m.point_num * m.ao_num for m in molecules])
+        for i in (order[0], order[len(order) // 3], order[2 * len(order) // 3], order[-1]):
+            m = molecules[i]
+            ref, mag = oracle_lib.mo_vgl(m, with_mag=True)
+            assert_vgl_close(got[i], ref, mag, what=f"batch {m.name} P={m.point_num} A={m.ao_num}")
+    finally:
+        for c in ctxs:
+            c.destroy()
+
+
+def test_batch_mixed_forms_and_moving_points(molecules):
+    mols = molecules[:24]
+    ctxs, outs = setup(mols, forms=[0, 2, 3, 1])
+    try:
+        rng = np.random.default_rng(5)
+        for step in range(2):
+            batch_mo_vgl(ctxs, outs)
+            for c, m, o in zip(ctxs, mols, outs):
+                assert np.array_equal(o.numpy(), single(c, m)), (step, m.name)
+            # move every electron and run again: the batch re-stamps each context
+            for c, m in zip(ctxs, mols):
+                c.set_electron_coord(m.elec_coord + rng.normal(0, 0.05, m.elec_coord.shape))
+    finally:
+        for c in ctxs:
+            c.destroy()
+
+
+def test_batch_rejects_small_output(molecules):
+    mols = molecules[:3]
+    ctxs, outs = setup(mols)
+    try:
+        short = DeviceArray(ctxs[1], (1,))
+        with pytest.raises(QmcklError) as e:
+            batch_mo_vgl(ctxs, [outs[0], short, outs[2]])
+        assert e.value.code == 4
+    finally:
+        for c in ctxs:
+            c.destroy()

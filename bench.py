@@ -161,19 +161,103 @@ def bench_batch(forms, steps, warmup, scale=1.0):
     return res
 
 
+# device kernel name (as the profiler lists it) of each timed kernel
+KERNEL_SYMBOL = {"en_distance": "en_distance_kernel", "ao_vgl": "ao_kernel<5, false, double>",
+                 "ao_vgl_f32": "ao_kernel<5, false, float>", "mo_gemm_f64": "gemm_f64_kernel<",
+                 "mo_gemm_fp32": "gemm_f32_kernel<", "mo_gemm_fp16x2": "gemm_split_kernel<2,",
+                 "mo_gemm_bf16x2": "gemm_split_kernel<3,", "mo_gemm_bf16x3": "gemm_split_kernel<4,"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_{workload}.json")
+
+
+def measured_traffic(s, kernel):
+    """HBM bytes per launch of a kernel on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE measurement (bench.py --traffic), or
+    None when there is none."""
+    path = TRAFFIC_FILE.format(workload=f"{s.name}_{s.walk_num}w")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = rec.get("kernels", {}).get(kernel)
+    return None if k is None else k.get("bytes_per_launch")
+
+
 def roofline(kernels, s):
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
     kind, work = kernel_work(dom, s)
     avg_s = kernels[dom]["avg_ms"] / 1e3
+    traffic = measured_traffic(s, dom)
     if kind == "mfma":
         prec, peak = GEMM_PEAK[dom]
         ach = work / avg_s / 1e12
         return {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
-                "frac": ach / peak, "traffic": None, "kernel": dom, "peak_dtype": prec,
+                "frac": ach / peak, "traffic": traffic, "kernel": dom, "peak_dtype": prec,
                 "mfma_products_per_fma": PRODUCTS[dom]}
     ach = work / avg_s / 1e9
     return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": dom}
+            "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom}
+
+
+def _pmc_per_kernel(counter, argv):
+    """Runs rocprofv3 --pmc <counter> over a child python process running
+    this file with argv; returns {kernel name: mean value per dispatch}."""
+    import csv
+    import subprocess
+    import tempfile
+    out = tempfile.mkdtemp(prefix="qmckl_pmc_")
+    cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "run", "--",
+           sys.executable, os.path.abspath(__file__)] + argv
+    log("running", " ".join(cmd))
+    subprocess.run(cmd, check=True, timeout=300, stdout=sys.stderr, stderr=sys.stderr)
+    vals = {}
+    for dirpath, _, files in os.walk(out):
+        for fn in files:
+            if fn.endswith("counter_collection.csv"):
+                with open(os.path.join(dirpath, fn)) as f:
+                    for r in csv.DictReader(f):
+                        if r["Counter_Name"] == counter:
+                            vals.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in vals.items()}
+
+
+def traffic_profile(args):
+    """Measures HBM bytes per launch of each timed kernel of the workload:
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch) in two profiler runs, each
+    scaled by what the same counter reads for a plain 1 GiB device copy.
+    Writes profiles/hbm_traffic_<workload>.json and prints it."""
+    s = make_system(args.config, args.walkers)
+    run = ["--steps", "2", "--warmup", "0", "--no-cpu", "--no-batch", "--form", args.form,
+           "--config", args.config] + (["--walkers", str(args.walkers)] if args.walkers else [])
+    cal = ["--copy-calibration", "1024"]
+    copy_bytes = 1024 * (1 << 20)
+    factors, per = {}, {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        c = _pmc_per_kernel(counter, cal)
+        copy_kib = max(v for k, v in c.items() if "copy" in k.lower() or "elementwise" in k)
+        factors[counter] = copy_bytes / (copy_kib * 1024.0)
+        per[counter] = _pmc_per_kernel(counter, run)
+    kernels = {}
+    for timer, sym in KERNEL_SYMBOL.items():
+        rec = {}
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            hits = [v for k, v in per[counter].items() if sym in k.replace("(anonymous namespace)::", "")]
+            if hits:
+                rec[counter.lower() + "_bytes"] = sum(hits) / len(hits) * 1024.0 * factors[counter]
+        if len(rec) == 2:
+            rec["bytes_per_launch"] = rec["fetch_size_bytes"] + rec["write_size_bytes"]
+            kind, work = kernel_work(timer, s)
+            if kind == "hbm":
+                rec["algorithmic_bytes_per_launch"] = work
+            kernels[timer] = rec
+    out = {"workload": f"{s.name}_{s.walk_num}w", "form": args.form,
+           "counter_scale": factors,
+           "command": "python bench.py --traffic " + " ".join(run[6:]),
+           "kernels": kernels}
+    path = TRAFFIC_FILE.format(workload=out["workload"])
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out), flush=True)
 
 
 def errors_vs_oracle(s, mos, n=256, seed=0):
@@ -247,6 +331,9 @@ def main():
                     help="skip the 48-molecule ragged batch measurement")
     ap.add_argument("--batch-only", action="store_true",
                     help="only run the 48-molecule ragged batch measurement")
+    ap.add_argument("--traffic", action="store_true",
+                    help="measure HBM bytes per launch of each kernel with rocprofv3 FETCH_SIZE / "
+                         "WRITE_SIZE runs (calibrated on a plain copy) and write them under profiles/")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (the reference "
                          "kernel for scaling HBM traffic counters) and exit")
@@ -257,6 +344,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
+        return
+    if args.traffic:
+        traffic_profile(args)
         return
     if args.batch_only:
         print(json.dumps({"batch_48_ragged": bench_batch(["plain", args.form], args.steps, args.warmup)}),

@@ -119,10 +119,12 @@ struct AoBasis {
   // nonzero flags of ao_vgl per (64 rows of [5P], 32 AOs), written with it
   uint8_t *ao_vgl_flags = nullptr;
   int64_t ao_vgl_flags_cap = 0;
-  // FP32 AO operand of the mixed contraction: [5P][ld], ld = A rounded up
-  // to 64 (padding zero), and its nonzero flags
+  // FP32 AO operand of the mixed contraction (blocked layout, AOs padded to
+  // ld = A rounded up to 64), its row maxima and nonzero flags
   float *ao_f32 = nullptr;
   int64_t ao_f32_cap = 0;
+  float *ao_f32_rowmax = nullptr;
+  int64_t ao_f32_rowmax_cap = 0;
   uint8_t *ao_f32_flags = nullptr;
   int64_t ao_f32_flags_cap = 0;
   int32_t uninitialized = (1 << 14) - 1;
@@ -247,9 +249,13 @@ hipError_t launch_shell_vgl(hipStream_t s, const DeviceBasis &b, int64_t P, cons
 hipError_t launch_ao_vgl(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
                          const double *nucl, const double *ao_factor, double *ao_vgl,
                          int ncomp, uint8_t *flags = nullptr, int nkb = 0);
-hipError_t launch_ao_vgl_f32(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
-                             const double *nucl, const double *ao_factor, float *out, int64_t ld,
-                             uint8_t *flags, int nkb);
+// AO operand of the mixed contraction, FP32 in the blocked layout
+// buf[c][64-point tile][AO/4][point in tile][4] (AOs zero-padded to ld, a
+// multiple of 64), row maxima rowmax[c][point] and, if flags is not null,
+// nonzero flags [5 * tiles][nkb] with nkb = ld / 32.
+hipError_t launch_ao_rows_f32(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
+                              const double *nucl, const double *ao_factor, int64_t ld, float *buf,
+                              float *rowmax, uint8_t *flags, int nkb);
 hipError_t launch_ao_from_shell_table(hipStream_t s, const DeviceBasis &b, int64_t P,
                                       const double *coord, const double *nucl,
                                       const double *ao_factor, const double *shell_vgl,
@@ -289,10 +295,12 @@ inline int64_t split_cols_padded(int form, int64_t M) {
 }
 hipError_t launch_split_coef(hipStream_t s, int form, int64_t M, int64_t K, const double *coef,
                              int64_t ldb, uint16_t *B, float *colinv);
-hipError_t launch_mo_gemm_mixed(hipStream_t s, int form, int64_t R, int64_t K, int64_t Ncol,
-                                const float *A, int64_t lda, const void *B, int64_t ldb,
-                                const float *colinv, const uint8_t *flags, int nkb, double *C,
-                                int64_t ldc);
+// Mixed MO contraction of the blocked AO operand (launch_ao_rows_f32) of P
+// points with the prepared coefficients into mo_vgl C [P][5][Ncol] (ldc).
+hipError_t launch_mo_gemm_mixed(hipStream_t s, int form, int64_t P, int64_t K, int64_t Ncol,
+                                const float *A, int64_t lda, const float *rowmax, const void *B,
+                                int64_t ldb, const float *colinv, const uint8_t *flags, int nkb,
+                                double *C, int64_t ldc);
 hipError_t launch_strided_copy(hipStream_t s, int64_t rows, int64_t cols, const double *in,
                                int64_t ld_in, double *out, int64_t ld_out);
 

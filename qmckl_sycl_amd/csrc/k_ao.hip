@@ -284,13 +284,6 @@ hipError_t launch_ao_vgl(hipStream_t s, const DeviceBasis &b, int64_t P, const d
   return launch_ao<1, false, double>(s, b, P, coord, nucl, ao_factor, nullptr, 0, ao);
 }
 
-hipError_t launch_ao_vgl_f32(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
-                             const double *nucl, const double *ao_factor, float *out, int64_t ld,
-                             uint8_t *flags, int nkb) {
-  AoOut<float> ao{out, ld, flags, nkb};
-  return launch_ao<5, false, float>(s, b, P, coord, nucl, ao_factor, nullptr, 0, ao);
-}
-
 hipError_t launch_ao_from_shell_table(hipStream_t s, const DeviceBasis &b, int64_t P,
                                       const double *coord, const double *nucl,
                                       const double *ao_factor, const double *shell_vgl,

@@ -1,0 +1,256 @@
+// AO operand of the mixed-precision MO contraction, one lane per point.
+//
+// Same arithmetic as the FP64 AO kernel (src/qmckl_ao_sycl.cpp: radial
+// shell sums with the nucleus screen r^2 > cutoff*range and the primitive
+// screen alpha r^2 > cutoff, 40-95; Cartesian monomials x^a y^b z^c in order
+// of descending a then b, 291-331; product-rule assembly times ao_factor,
+// 367-422), evaluated in FP64 and rounded once to FP32 at the store.
+//
+// The operand is internal to the contraction, so its layout is chosen for
+// the two kernels that touch it:
+//   buf[c][pt][kq][pl][4]   c = VGL component (5), pt = 64-point tile,
+//                           kq = AO index / 4, pl = point in tile,
+//                           the 4 consecutive AOs 4kq .. 4kq+3
+// i.e. the GEMM row (c, point) of a 64-row tile has its AOs in 16-byte
+// pieces, 64 of which (one per point of the tile) form a contiguous
+// kilobyte for each group of 4 AOs.
+// AOs past ao_num up to ld (a multiple of 64) are written as zeros.
+// Also written: rowmax[c][point] = max |entry| of the row (each lane owns
+// its five rows, so no reduction is needed), and nonzero flags per
+// (row tile, 32-AO block) gathered with wavefront ballots.
+//
+// Work decomposition: a wavefront owns 64 points and walks all shells in
+// AO order; the shell loop is wavefront-uniform (shell data, nucleus and
+// primitives come through scalar loads; a primitive screened for all 64
+// points costs no exp), and for l <= 3 the shell's Cartesian components
+// are unrolled at compile time, so each AO costs a few FP64 multiplies on
+// powers computed once per (point, shell).  No LDS, no
+// barriers: the four wavefronts of a workgroup are independent.
+#include <hip/hip_runtime.h>
+
+#include "internal.hpp"
+
+namespace qmckl_amd {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+// Running state of one lane: the AO counter of the wavefront, row maxima
+// and nonzero bits of the current 32-AO block.  Each value is stored as it
+// is produced: the four 4-byte stores of a 16-byte piece come from four
+// consecutive AOs of the same lane, so L2 assembles whole lines before they
+// reach HBM.
+struct RowWriter {
+  float mx[5];
+  uint32_t nz;       // bit c: some AO of the current 32-block is nonzero in row c
+  int k;             // next AO index (wavefront-uniform)
+  float *base;       // &buf[0][pt][0][lane][0]
+  int64_t cstride;   // floats between components
+  uint8_t *flags;    // flags + pt * nkb, component stride fstride
+  int64_t fstride;
+  int lane;
+
+  __device__ __attribute__((always_inline)) void push(const double (&v)[5]) {
+    float *o = base + (int64_t)(k >> 2) * 256 + (k & 3);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      const float w = (float)v[c];
+      o[c * cstride] = w;
+      mx[c] = fmaxf(mx[c], fabsf(w));
+      nz |= (w != 0.0f ? 1u : 0u) << c;
+    }
+    if ((k & 31) == 31) {
+      if (flags != nullptr) {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+          const bool any = __ballot((nz >> c) & 1u) != 0;
+          if (lane == 0) flags[c * fstride + (k >> 5)] = any ? 1 : 0;
+        }
+      }
+      nz = 0;
+    }
+    ++k;
+  }
+};
+
+// Evaluates the l = L Cartesian AOs of one shell for this lane's point and
+// pushes them in the reference order (a descending, then b descending).
+template <int L>
+__device__ __attribute__((always_inline)) inline void shell_aos(RowWriter &w,
+                                                                const double *__restrict__ ao_factor,
+                                                                double x, double y, double z,
+                                                                const double (&R)[5]) {
+  double px[L + 1], py[L + 1], pz[L + 1];
+  px[0] = py[0] = pz[0] = 1.0;
+#pragma unroll
+  for (int e = 1; e <= L; ++e) {
+    px[e] = px[e - 1] * x;
+    py[e] = py[e - 1] * y;
+    pz[e] = pz[e - 1] * z;
+  }
+  const int k0 = w.k;
+  int i = 0;
+#pragma unroll
+  for (int a = L; a >= 0; --a) {
+#pragma unroll
+    for (int b = L - a; b >= 0; --b, ++i) {
+      const int c = L - a - b;
+      const double X0 = px[a], Y0 = py[b], Z0 = pz[c];
+      const double X1 = a > 0 ? a * px[a > 0 ? a - 1 : 0] : 0.0;
+      const double Y1 = b > 0 ? b * py[b > 0 ? b - 1 : 0] : 0.0;
+      const double Z1 = c > 0 ? c * pz[c > 0 ? c - 1 : 0] : 0.0;
+      const double X2 = a > 1 ? a * (a - 1) * px[a > 1 ? a - 2 : 0] : 0.0;
+      const double Y2 = b > 1 ? b * (b - 1) * py[b > 1 ? b - 2 : 0] : 0.0;
+      const double Z2 = c > 1 ? c * (c - 1) * pz[c > 1 ? c - 2 : 0] : 0.0;
+      const double yz = Y0 * Z0;
+      const double m = X0 * yz;
+      const double gx = X1 * yz, gy = X0 * Y1 * Z0, gz = X0 * Y0 * Z1;
+      const double lap = X2 * yz + X0 * (Y2 * Z0 + Y0 * Z2);
+      const double f = ao_factor[k0 + i];
+      double v[5];
+      v[0] = m * R[0] * f;
+      v[1] = (gx * R[0] + m * R[1]) * f;
+      v[2] = (gy * R[0] + m * R[2]) * f;
+      v[3] = (gz * R[0] + m * R[3]) * f;
+      v[4] = (lap * R[0] + m * R[4] + 2.0 * (gx * R[1] + gy * R[2] + gz * R[3])) * f;
+      w.push(v);
+    }
+  }
+}
+
+// (x^e, e x^(e-1), e(e-1) x^(e-2)) by the recurrence
+// (f, f', f'') -> (x f, f + x f', 2 f' + x f'').
+__device__ inline void pow3(double x, int e, double &f0, double &f1, double &f2) {
+  double p0 = 1.0, p1 = 0.0, p2 = 0.0;
+  for (int i = 0; i < e; ++i) {
+    const double n2 = 2.0 * p1 + x * p2;
+    const double n1 = p0 + x * p1;
+    p0 = x * p0;
+    p1 = n1;
+    p2 = n2;
+  }
+  f0 = p0;
+  f1 = p1;
+  f2 = p2;
+}
+
+// Shells of l > 3 (rare): the same AOs with the exponents as (wavefront-
+// uniform) loop counts instead of unrolled code, which keeps the register
+// footprint of the kernel at that of l <= 3.
+__device__ __attribute__((always_inline)) inline void shell_aos_any(RowWriter &w, const double *__restrict__ ao_factor,
+                                                        double x, double y, double z, const double (&R)[5],
+                                                        int l) {
+  const int k0 = w.k;
+  int i = 0;
+  for (int a = l; a >= 0; --a) {
+    for (int b = l - a; b >= 0; --b, ++i) {
+      const int c = l - a - b;
+      double X0, X1, X2, Y0, Y1, Y2, Z0, Z1, Z2;
+      pow3(x, a, X0, X1, X2);
+      pow3(y, b, Y0, Y1, Y2);
+      pow3(z, c, Z0, Z1, Z2);
+      const double yz = Y0 * Z0;
+      const double m = X0 * yz;
+      const double gx = X1 * yz, gy = X0 * Y1 * Z0, gz = X0 * Y0 * Z1;
+      const double lap = X2 * yz + X0 * (Y2 * Z0 + Y0 * Z2);
+      const double f = ao_factor[k0 + i];
+      double v[5];
+      v[0] = m * R[0] * f;
+      v[1] = (gx * R[0] + m * R[1]) * f;
+      v[2] = (gy * R[0] + m * R[2]) * f;
+      v[3] = (gz * R[0] + m * R[3]) * f;
+      v[4] = (lap * R[0] + m * R[4] + 2.0 * (gx * R[1] + gy * R[2] + gz * R[3])) * f;
+      w.push(v);
+    }
+  }
+}
+
+__global__ void __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads), amdgpu_waves_per_eu(4)))
+ao_rows_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord,
+               const double *__restrict__ nucl, const ShellDesc *__restrict__ shells,
+               const double *__restrict__ ao_factor, const double *__restrict__ expo,
+               const double *__restrict__ coef, const double *__restrict__ nucl_range, int64_t A,
+               int64_t ld, int64_t npt, float *__restrict__ buf, float *__restrict__ rowmax,
+               uint8_t *__restrict__ flags, int nkb) {
+  const int lane = threadIdx.x & 63;
+  const int64_t pt = (int64_t)blockIdx.x * (kThreads / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (pt >= npt) return;  // wavefront-uniform
+  const int64_t p = pt * 64 + lane;
+  const bool inside = p < P;
+  const double px = inside ? coord[p] : 0.0;
+  const double py = inside ? coord[P + p] : 0.0;
+  const double pz = inside ? coord[2 * P + p] : 0.0;
+
+  RowWriter w;
+#pragma unroll
+  for (int c = 0; c < 5; ++c) w.mx[c] = 0.0f;
+  w.nz = 0;
+  w.k = 0;
+  w.cstride = npt * (ld / 4) * 256;
+  w.base = buf + pt * (ld / 4) * 256 + lane * 4;
+  w.flags = flags != nullptr ? flags + pt * nkb : nullptr;
+  w.fstride = npt * nkb;
+  w.lane = lane;
+
+  for (int64_t s = 0; s < S; ++s) {
+    const ShellDesc d = shells[s];
+    const double x = px - nucl[d.nucl];
+    const double y = py - nucl[N + d.nucl];
+    const double z = pz - nucl[2 * N + d.nucl];
+    const double r2 = fma(x, x, fma(y, y, z * z));
+    const bool live = inside && !(r2 > kCutoff * nucl_range[d.nucl]);
+    double R[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const double *ex = expo + d.prim_start;
+    const double *cf = coef + d.prim_start;
+    for (int j = 0; j < d.prim_num; ++j) {
+      const double a = ex[j];
+      const double ar2 = a * r2;
+      if (live && !(ar2 > kCutoff)) {
+        const double v = cf[j] * exp(-ar2);
+        const double two_a = -2.0 * a * v;
+        R[0] += v;
+        R[1] += two_a * x;
+        R[2] += two_a * y;
+        R[3] += two_a * z;
+        R[4] += two_a * (3.0 - 2.0 * ar2);
+      }
+    }
+    switch (d.l) {
+      case 0: shell_aos<0>(w, ao_factor, x, y, z, R); break;
+      case 1: shell_aos<1>(w, ao_factor, x, y, z, R); break;
+      case 2: shell_aos<2>(w, ao_factor, x, y, z, R); break;
+      case 3: shell_aos<3>(w, ao_factor, x, y, z, R); break;
+      default: shell_aos_any(w, ao_factor, x, y, z, R, d.l); break;
+    }
+  }
+  // zero AOs up to ld (the padding the contraction reads, and the last 32-block's flags)
+  const double zero[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  while (w.k < ld) w.push(zero);
+  if (inside) {
+#pragma unroll
+    for (int c = 0; c < 5; ++c) rowmax[(c * npt + pt) * 64 + lane] = w.mx[c];
+  } else {
+#pragma unroll
+    for (int c = 0; c < 5; ++c) rowmax[(c * npt + pt) * 64 + lane] = 0.0f;
+  }
+}
+
+}  // namespace
+
+hipError_t launch_ao_rows_f32(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
+                              const double *nucl, const double *ao_factor, int64_t ld, float *buf,
+                              float *rowmax, uint8_t *flags, int nkb) {
+  if (P <= 0) return hipSuccess;
+  if (b.max_l > kMaxL || ld % 64 != 0 || ld < b.ao_num) return hipErrorInvalidValue;
+  if (flags != nullptr && (int64_t)nkb * 32 != ld) return hipErrorInvalidValue;
+  const int64_t npt = (P + 63) / 64;
+  const int64_t blocks = (npt + kThreads / 64 - 1) / (kThreads / 64);
+  if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(ao_rows_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, P, (int)b.nucl_num,
+                     b.shell_num, coord, nucl, b.shells, ao_factor, b.expo, b.coef_norm, b.nucl_range,
+                     b.ao_num, ld, npt, buf, rowmax, flags, nkb);
+  return hipGetLastError();
+}
+
+}  // namespace qmckl_amd

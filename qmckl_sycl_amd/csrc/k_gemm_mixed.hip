@@ -151,9 +151,10 @@ __device__ inline void split8(const float (&v)[8], uint4 (&out)[Terms<SP>::n]) {
 
 template <int SP, int CBW>
 __global__ void __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads), amdgpu_waves_per_eu(1, 1)))
-gemm_split_kernel(int64_t R, int64_t K, int64_t Ncol, const float *__restrict__ A, int64_t lda,
-                  const uint16_t *__restrict__ Bs, int64_t ldb, const float *__restrict__ colinv,
-                  const uint8_t *__restrict__ flags, int nkb, double *__restrict__ C, int64_t ldc) {
+gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *__restrict__ A,
+                  int64_t lda, const float *__restrict__ rowmax, const uint16_t *__restrict__ Bs,
+                  int64_t ldb, const float *__restrict__ colinv, const uint8_t *__restrict__ flags, int nkb,
+                  double *__restrict__ C, int64_t ldc) {
   constexpr int T = Terms<SP>::n;
   __shared__ uint16_t sA[T][kRM * kLDS];
   __shared__ float s_inv[kRM];
@@ -162,7 +163,8 @@ gemm_split_kernel(int64_t R, int64_t K, int64_t Ncol, const float *__restrict__ 
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  const int64_t row0 = (int64_t)blockIdx.x * kRM;
+  const int64_t rt = blockIdx.x;                  // row tile: component c, points 64*pt ..
+  const int64_t comp = rt / npt, p0 = (rt - comp * npt) * 64;
   const int64_t col0 = (int64_t)blockIdx.y * 64 * CBW + wave * 16 * CBW;  // this wavefront's first column
   const int nst = (int)((K + kStage - 1) / kStage);
 
@@ -170,7 +172,7 @@ gemm_split_kernel(int64_t R, int64_t K, int64_t Ncol, const float *__restrict__ 
   int count = nst;
   if (flags != nullptr) {
     if (wave == 0) {
-      const uint8_t *f = flags + (int64_t)blockIdx.x * nkb;
+      const uint8_t *f = flags + rt * nkb;
       int base = 0;
       for (int s0 = 0; s0 < nst; s0 += 64) {
         const int st = s0 + lane;
@@ -185,11 +187,15 @@ gemm_split_kernel(int64_t R, int64_t K, int64_t Ncol, const float *__restrict__ 
     count = s_count;
   }
 
-  // A slot: row ar of the tile, 16 consecutive K at aq; 16 consecutive lanes
-  // take 16 consecutive rows, so the LDS stores below do not conflict.
-  // Rows past R read row R-1 (their results are not stored).
-  const int ar = wave * 16 + li, aq = lk * 16;
-  const float *Ap = A + (row0 + ar < R ? row0 + ar : R - 1) * lda + aq;
+  // A slot: row ar = lane of the tile, 16 consecutive K at aq = 16*wave,
+  // read as four 16-byte pieces of the blocked operand (consecutive lanes,
+  // consecutive pieces); 16 consecutive lanes take 16 consecutive rows, so
+  // the LDS stores below do not conflict.  Each row has one power-of-two
+  // scale (from its max |entry| over all K).
+  const int ar = lane, aq = wave * 16;
+  const float *Ap = A + rt * (lda / 4) * 256 + (aq / 4) * 256 + ar * 4;
+  const float sc = pow2_scale(rowmax[rt * 64 + ar]);
+  if (wave == 0) s_inv[ar] = 1.0f / sc;
   // C operand: lane holds B[8*lk..8*lk+7][li] of each 16-column tile
   const uint16_t *Bp = Bs + (col0 + li) * T * ldb + 8 * lk;
   const int64_t cb_stride = 16 * T * ldb;
@@ -198,9 +204,9 @@ gemm_split_kernel(int64_t R, int64_t K, int64_t Ncol, const float *__restrict__ 
   uint4 bq[2][CBW][T];
   auto stage_k0 = [&](int s) { return (int64_t)(flags != nullptr ? s_list[s] : s) * kStage; };
   auto load_a = [&](int64_t k0) {
-    const float4 *src = reinterpret_cast<const float4 *>(Ap + k0);
+    const float *src = Ap + (k0 / 4) * 256;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) ra[q] = src[q];
+    for (int q = 0; q < 4; ++q) ra[q] = *reinterpret_cast<const float4 *>(src + q * 256);
   };
   auto load_b = [&](int64_t k0, int ks) {
 #pragma unroll
@@ -224,18 +230,12 @@ gemm_split_kernel(int64_t R, int64_t K, int64_t Ncol, const float *__restrict__ 
     load_a(k0);
     load_b(k0, 0);
     load_b(k0, 1);
+  } else {
+    __syncthreads();  // s_inv is read by every wavefront in the epilogue
   }
   for (int s = 0; s < count; ++s) {
     __syncthreads();  // every wavefront is done reading the previous A slab
     {
-      float m = 0.f;
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        m = fmaxf(fmaxf(m, fmaxf(fabsf(ra[q].x), fabsf(ra[q].y))), fmaxf(fabsf(ra[q].z), fabsf(ra[q].w)));
-      m = fmaxf(m, __shfl_xor(m, 16, 64));
-      m = fmaxf(m, __shfl_xor(m, 32, 64));
-      const float sc = pow2_scale(m);
-      if (lk == 0) s_inv[ar] = 1.0f / sc;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const float v[8] = {ra[2 * h].x * sc,     ra[2 * h].y * sc,     ra[2 * h].z * sc,
@@ -286,11 +286,9 @@ gemm_split_kernel(int64_t R, int64_t K, int64_t Ncol, const float *__restrict__ 
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const double inv = (double)s_inv[r * 16 + 4 * lk + j];
+      for (int cb = 0; cb < CBW; ++cb)
 #pragma unroll
-        for (int cb = 0; cb < CBW; ++cb) acc[r][cb][j] = fma((double)d[r][cb][j], inv, acc[r][cb][j]);
-      }
+        for (int j = 0; j < 4; ++j) acc[r][cb][j] += (double)d[r][cb][j];
   }
 #pragma unroll
   for (int cb = 0; cb < CBW; ++cb) {
@@ -301,19 +299,21 @@ gemm_split_kernel(int64_t R, int64_t K, int64_t Ncol, const float *__restrict__ 
     for (int r = 0; r < 4; ++r)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int64_t row = row0 + r * 16 + 4 * lk + j;
-        if (row < R) C[row * ldc + col] = acc[r][cb][j] * ci;
+        const int row = r * 16 + 4 * lk + j;
+        const int64_t p = p0 + row;
+        if (p < P) C[(p * 5 + comp) * ldc + col] = acc[r][cb][j] * ((double)s_inv[row] * ci);
       }
   }
 }
 
 // FP32 form: v_mfma_f32_16x16x4_f32, lane l holds A[l%16][l/16], B[l/16][l%16].
+//   A  the blocked FP32 operand (as for the split forms)
 //   Bf [Ncol][ldb] FP32 C (unscaled), K contiguous.
 template <int CBW>
 __global__ void __launch_bounds__(kThreads)
-gemm_f32_kernel(int64_t R, int64_t K, int64_t Ncol, const float *__restrict__ A, int64_t lda,
-                const float *__restrict__ Bf, int64_t ldb, const uint8_t *__restrict__ flags, int nkb,
-                int flush, double *__restrict__ C, int64_t ldc) {
+gemm_f32_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *__restrict__ A,
+                int64_t lda, const float *__restrict__ Bf, int64_t ldb, const uint8_t *__restrict__ flags,
+                int nkb, int flush, double *__restrict__ C, int64_t ldc) {
   constexpr int BN = 32 * CBW;
   constexpr int LDF = kKB + 4;  // LDS row length in floats
   __shared__ float sA[kBM * LDF];
@@ -321,11 +321,12 @@ gemm_f32_kernel(int64_t R, int64_t K, int64_t Ncol, const float *__restrict__ A,
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
   const int li = lane & 15, lk = lane >> 4;
-  const int64_t row0 = (int64_t)blockIdx.x * kBM;
+  const int64_t rt = blockIdx.x;  // row tile: component c, points 64*pt ..
+  const int64_t comp = rt / npt, p0 = (rt - comp * npt) * 64;
   const int64_t col0 = (int64_t)blockIdx.y * BN;
   const int64_t nblk = (K + kKB - 1) / kKB;
   const int r_ld = tid >> 2, seg = (tid & 3) * 8;
-  const int64_t grow = row0 + r_ld;
+  const float *Ap = A + rt * (lda / 4) * 256 + r_ld * 4;
 
   f4 d[2][CBW];
   double acc[2][CBW][4];
@@ -339,15 +340,12 @@ gemm_f32_kernel(int64_t R, int64_t K, int64_t Ncol, const float *__restrict__ A,
     }
   int pending = 0;
   for (int64_t kb = 0; kb < nblk; ++kb) {
-    if (flags != nullptr && flags[(int64_t)blockIdx.x * nkb + kb] == 0) continue;
+    if (flags != nullptr && flags[rt * nkb + kb] == 0) continue;
     const int64_t k0 = kb * kKB;
     {
-      float4 x0 = {0.f, 0.f, 0.f, 0.f}, x1 = x0;
-      if (grow < R) {
-        const float4 *src = reinterpret_cast<const float4 *>(A + grow * lda + k0 + seg);
-        x0 = src[0];
-        x1 = src[1];
-      }
+      const float *src = Ap + ((k0 + seg) / 4) * 256;
+      const float4 x0 = *reinterpret_cast<const float4 *>(src);
+      const float4 x1 = *reinterpret_cast<const float4 *>(src + 256);
       float *dst = &sA[r_ld * LDF + seg];
       dst[0] = x0.x; dst[1] = x0.y; dst[2] = x0.z; dst[3] = x0.w;
       dst[4] = x1.x; dst[5] = x1.y; dst[6] = x1.z; dst[7] = x1.w;
@@ -394,8 +392,8 @@ gemm_f32_kernel(int64_t R, int64_t K, int64_t Ncol, const float *__restrict__ A,
       if (col >= Ncol) continue;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int64_t row = row0 + wr * 32 + rb * 16 + 4 * lk + j;
-        if (row < R) C[row * ldc + col] = acc[rb][cb][j] + (double)d[rb][cb][j];
+        const int64_t p = p0 + wr * 32 + rb * 16 + 4 * lk + j;
+        if (p < P) C[(p * 5 + comp) * ldc + col] = acc[rb][cb][j] + (double)d[rb][cb][j];
       }
     }
 }
@@ -433,48 +431,49 @@ split_coef_kernel(int64_t M, int64_t K, const double *__restrict__ coef, int64_t
 }
 
 template <int CBW>
-hipError_t launch_f32(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const float *A, int64_t lda,
-                      const void *B, int64_t ldb, const uint8_t *flags, int nkb, double *C, int64_t ldc) {
+hipError_t launch_f32(hipStream_t s, int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *A,
+                      int64_t lda, const void *B, int64_t ldb, const uint8_t *flags, int nkb, double *C,
+                      int64_t ldc) {
   constexpr int BN = 32 * CBW;
-  const int64_t gx = (R + kBM - 1) / kBM, gy = (Ncol + BN - 1) / BN;
+  const int64_t gx = 5 * npt, gy = (Ncol + BN - 1) / BN;
   if (gx > 0x7fffffff || gy > 65535) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(gemm_f32_kernel<CBW>, dim3((unsigned)gx, (unsigned)gy), dim3(kThreads), 0, s, R, K,
-                     Ncol, A, lda, (const float *)B, ldb, flags, nkb, 2, C, ldc);
+  hipLaunchKernelGGL(gemm_f32_kernel<CBW>, dim3((unsigned)gx, (unsigned)gy), dim3(kThreads), 0, s, P, npt,
+                     K, Ncol, A, lda, (const float *)B, ldb, flags, nkb, 2, C, ldc);
   return hipGetLastError();
 }
 
 template <int SP, int CBW>
-hipError_t launch_split(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const float *A, int64_t lda,
-                        const void *B, int64_t ldb, const float *colinv, const uint8_t *flags, int nkb,
-                        double *C, int64_t ldc) {
-  const int64_t gx = (R + kRM - 1) / kRM, gy = (Ncol + 64 * CBW - 1) / (64 * CBW);
+hipError_t launch_split(hipStream_t s, int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *A,
+                        int64_t lda, const float *rowmax, const void *B, int64_t ldb, const float *colinv,
+                        const uint8_t *flags, int nkb, double *C, int64_t ldc) {
+  const int64_t gx = 5 * npt, gy = (Ncol + 64 * CBW - 1) / (64 * CBW);
   if (gx > 0x7fffffff || gy > 65535 || (lda % kStage) != 0 || (ldb % kStage) != 0)
     return hipErrorInvalidValue;
   if (flags != nullptr && (nkb > kMaxFlagBlocks || (int64_t)nkb * 32 < K)) return hipErrorInvalidValue;
   hipLaunchKernelGGL((gemm_split_kernel<SP, CBW>), dim3((unsigned)gx, (unsigned)gy), dim3(kThreads), 0, s,
-                     R, K, Ncol, A, lda, (const uint16_t *)B, ldb, colinv, flags, nkb, C, ldc);
+                     P, npt, K, Ncol, A, lda, rowmax, (const uint16_t *)B, ldb, colinv, flags, nkb, C, ldc);
   return hipGetLastError();
 }
 
 template <int SP>
-hipError_t launch_form(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const float *A, int64_t lda,
-                       const void *B, int64_t ldb, const float *colinv, const uint8_t *flags, int nkb,
-                       double *C, int64_t ldc) {
+hipError_t launch_form(hipStream_t s, int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *A,
+                       int64_t lda, const float *rowmax, const void *B, int64_t ldb, const float *colinv,
+                       const uint8_t *flags, int nkb, double *C, int64_t ldc) {
   if constexpr (SP == QMCKL_AMD_SPLIT_FP32) {
     const int64_t nb = (Ncol + 31) / 32;  // 32-column groups
-    if (nb <= 1) return launch_f32<1>(s, R, K, Ncol, A, lda, B, ldb, flags, nkb, C, ldc);
-    if (nb <= 2) return launch_f32<2>(s, R, K, Ncol, A, lda, B, ldb, flags, nkb, C, ldc);
-    if (nb <= 3) return launch_f32<3>(s, R, K, Ncol, A, lda, B, ldb, flags, nkb, C, ldc);
-    if (nb <= 4) return launch_f32<4>(s, R, K, Ncol, A, lda, B, ldb, flags, nkb, C, ldc);
-    return launch_f32<5>(s, R, K, Ncol, A, lda, B, ldb, flags, nkb, C, ldc);
+    if (nb <= 1) return launch_f32<1>(s, P, npt, K, Ncol, A, lda, B, ldb, flags, nkb, C, ldc);
+    if (nb <= 2) return launch_f32<2>(s, P, npt, K, Ncol, A, lda, B, ldb, flags, nkb, C, ldc);
+    if (nb <= 3) return launch_f32<3>(s, P, npt, K, Ncol, A, lda, B, ldb, flags, nkb, C, ldc);
+    if (nb <= 4) return launch_f32<4>(s, P, npt, K, Ncol, A, lda, B, ldb, flags, nkb, C, ldc);
+    return launch_f32<5>(s, P, npt, K, Ncol, A, lda, B, ldb, flags, nkb, C, ldc);
   } else {
     const int cbw = split_wg_cols(SP, Ncol) / 64;
-    if (cbw == 1) return launch_split<SP, 1>(s, R, K, Ncol, A, lda, B, ldb, colinv, flags, nkb, C, ldc);
-    if (cbw == 2) return launch_split<SP, 2>(s, R, K, Ncol, A, lda, B, ldb, colinv, flags, nkb, C, ldc);
-    if (cbw == 3) return launch_split<SP, 3>(s, R, K, Ncol, A, lda, B, ldb, colinv, flags, nkb, C, ldc);
+    if (cbw == 1) return launch_split<SP, 1>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, C, ldc);
+    if (cbw == 2) return launch_split<SP, 2>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, C, ldc);
+    if (cbw == 3) return launch_split<SP, 3>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, C, ldc);
     if constexpr (Terms<SP>::n == 2) {
-      if (cbw == 4) return launch_split<SP, 4>(s, R, K, Ncol, A, lda, B, ldb, colinv, flags, nkb, C, ldc);
-      if (cbw == 5) return launch_split<SP, 5>(s, R, K, Ncol, A, lda, B, ldb, colinv, flags, nkb, C, ldc);
+      if (cbw == 4) return launch_split<SP, 4>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, C, ldc);
+      if (cbw == 5) return launch_split<SP, 5>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, C, ldc);
     }
     return hipErrorInvalidValue;
   }
@@ -514,21 +513,22 @@ hipError_t launch_split_coef(hipStream_t s, int form, int64_t M, int64_t K, cons
   return hipGetLastError();
 }
 
-hipError_t launch_mo_gemm_mixed(hipStream_t s, int form, int64_t R, int64_t K, int64_t Ncol,
-                                const float *A, int64_t lda, const void *B, int64_t ldb,
-                                const float *colinv, const uint8_t *flags, int nkb, double *C,
+hipError_t launch_mo_gemm_mixed(hipStream_t s, int form, int64_t P, int64_t K, int64_t Ncol,
+                                const float *A, int64_t lda, const float *rowmax, const void *B,
+                                int64_t ldb, const float *colinv, const uint8_t *flags, int nkb, double *C,
                                 int64_t ldc) {
-  if (R <= 0 || Ncol <= 0) return hipSuccess;
+  if (P <= 0 || Ncol <= 0) return hipSuccess;
   if (K <= 0 || (lda % 64) != 0 || (ldb % 64) != 0 || lda < K || ldb < K) return hipErrorInvalidValue;
+  const int64_t npt = (P + 63) / 64;
   switch (form) {
     case QMCKL_AMD_SPLIT_FP16X2:
-      return launch_form<QMCKL_AMD_SPLIT_FP16X2>(s, R, K, Ncol, A, lda, B, ldb, colinv, flags, nkb, C, ldc);
+      return launch_form<QMCKL_AMD_SPLIT_FP16X2>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, C, ldc);
     case QMCKL_AMD_SPLIT_BF16X2:
-      return launch_form<QMCKL_AMD_SPLIT_BF16X2>(s, R, K, Ncol, A, lda, B, ldb, colinv, flags, nkb, C, ldc);
+      return launch_form<QMCKL_AMD_SPLIT_BF16X2>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, C, ldc);
     case QMCKL_AMD_SPLIT_BF16X3:
-      return launch_form<QMCKL_AMD_SPLIT_BF16X3>(s, R, K, Ncol, A, lda, B, ldb, colinv, flags, nkb, C, ldc);
+      return launch_form<QMCKL_AMD_SPLIT_BF16X3>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, C, ldc);
     case QMCKL_AMD_SPLIT_FP32:
-      return launch_form<QMCKL_AMD_SPLIT_FP32>(s, R, K, Ncol, A, lda, B, ldb, colinv, flags, nkb, C, ldc);
+      return launch_form<QMCKL_AMD_SPLIT_FP32>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, C, ldc);
     default:
       return hipErrorInvalidValue;
   }

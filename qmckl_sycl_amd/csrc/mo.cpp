@@ -39,33 +39,32 @@ const char *gemm_timer_name(qmckl_precision_device form) {
   }
 }
 
-// Mixed forms: the FP64 AO assembly stored as FP32 rows [5P][ld] with
-// nonzero flags, then the split MFMA contraction into dst.
+// Mixed forms: the FP64 AO assembly stored as FP32 in the blocked operand
+// layout with row maxima and nonzero flags, then the split MFMA contraction
+// into dst.
 qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Context *c, double *dst,
                                             const char *fn) {
   if (!c->ao.provided || c->point.num <= 0 || c->point.coord == nullptr || !c->nucleus.provided)
     return fail(context, QMCKL_NOT_PROVIDED_DEVICE, "qmckl_ao_basis", nullptr);
   const qmckl_precision_device form = c->mo.precision;
-  const int64_t P = c->point.num, A = c->ao.ao_num, M = c->mo.mo_num, R = 5 * P;
+  const int64_t P = c->point.num, A = c->ao.ao_num, M = c->mo.mo_num;
   const int64_t ld = round_up(A, 64);
   const int nkb = (int)(ld / 32);
-  const int64_t ntile = (R + 63) / 64;
+  const int64_t rows = 5 * round_up(P, 64);  // (component, point) rows, points padded to 64
   AoBasis &ao = c->ao;
   MoBasis &mo = c->mo;
-  // operands
-  const bool fresh = ao.ao_f32 == nullptr || ao.ao_f32_cap < (int64_t)(sizeof(float) * R * ld);
-  if (!grow_bytes(c, reinterpret_cast<void **>(&ao.ao_f32), &ao.ao_f32_cap, (int64_t)sizeof(float) * R * ld))
+  if (!grow_bytes(c, reinterpret_cast<void **>(&ao.ao_f32), &ao.ao_f32_cap, (int64_t)sizeof(float) * rows * ld) ||
+      !grow_bytes(c, reinterpret_cast<void **>(&ao.ao_f32_rowmax), &ao.ao_f32_rowmax_cap,
+                  (int64_t)sizeof(float) * rows))
     return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
   uint8_t *flags = nullptr;
   if (nkb <= kMaxFlagBlocks) {
-    if (!grow_bytes(c, reinterpret_cast<void **>(&ao.ao_f32_flags), &ao.ao_f32_flags_cap, ntile * nkb))
+    if (!grow_bytes(c, reinterpret_cast<void **>(&ao.ao_f32_flags), &ao.ao_f32_flags_cap, rows / 64 * nkb))
       return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
     flags = ao.ao_f32_flags;
   }
   hipError_t e = hipSuccess;
-  if (fresh)  // the padding columns [A, ld) are never written: zero them once
-    e = hipMemsetAsync(ao.ao_f32, 0, ao.ao_f32_cap, c->stream);
-  if (e == hipSuccess && (mo.coef_split == nullptr || mo.coef_split_form != form)) {
+  if (mo.coef_split == nullptr || mo.coef_split_form != form) {
     dfree(c, mo.coef_split);
     mo.coef_split = static_cast<uint16_t *>(
         dmalloc(c, sizeof(uint16_t) * split_terms(form) * split_cols_padded(form, M) * ld));
@@ -76,14 +75,14 @@ qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Contex
   }
   if (e == hipSuccess) {
     KernelTimer t(c, "ao_vgl_f32");
-    e = launch_ao_vgl_f32(c->stream, ao.dev, P, c->point.coord, c->nucleus.coord, ao.ao_factor,
-                          ao.ao_f32, ld, flags, flags ? nkb : 0);
+    e = launch_ao_rows_f32(c->stream, ao.dev, P, c->point.coord, c->nucleus.coord, ao.ao_factor, ld,
+                           ao.ao_f32, ao.ao_f32_rowmax, flags, flags ? nkb : 0);
     t.stop();
   }
   if (e == hipSuccess) {
     KernelTimer t(c, gemm_timer_name(form));
-    e = launch_mo_gemm_mixed(c->stream, form, R, A, M, ao.ao_f32, ld, mo.coef_split, ld, mo.coef_colinv,
-                             flags, flags ? nkb : 0, dst, M);
+    e = launch_mo_gemm_mixed(c->stream, form, P, A, M, ao.ao_f32, ld, ao.ao_f32_rowmax, mo.coef_split, ld,
+                             mo.coef_colinv, flags, flags ? nkb : 0, dst, M);
     t.stop();
   }
   if (e == hipSuccess) e = sync(c);

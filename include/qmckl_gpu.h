@@ -192,6 +192,22 @@ qmckl_exit_code_device qmckl_get_electron_en_distance_device(qmckl_context_devic
                                                              double *en_distance,
                                                              int64_t size_max);
 
+/* ---- electron-electron distances ------------------------------------------ */
+
+/* qmckl_gpu.hpp:1043-1044 (src/qmckl_electron.cpp:192-240) */
+qmckl_exit_code_device qmckl_provide_ee_distance_device(qmckl_context_device context);
+/* include/qmckl_electron.hpp:34-35 (src/qmckl_electron.cpp:333-371).
+ * coord [3][walk_num*elec_num] (point w*elec_num + i), ee_distance
+ * [walk_num][elec_num][elec_num] with ee_distance[w][j][i] = |r_wi - r_wj|. */
+qmckl_exit_code_device qmckl_compute_ee_distance_device(qmckl_context_device context,
+                                                        int64_t elec_num, int64_t walk_num,
+                                                        const double *coord, double *ee_distance);
+/* New getter (the reference has none): ee_distance [walk][elec][elec],
+ * size_max >= elec_num^2 * walk_num. */
+qmckl_exit_code_device qmckl_get_electron_ee_distance_device(qmckl_context_device context,
+                                                             double *ee_distance,
+                                                             int64_t size_max);
+
 /* ---- AO basis (include/qmckl_gpu.hpp:586-752) ---------------------------- */
 
 bool qmckl_ao_basis_provided_device(qmckl_context_device context);
@@ -301,6 +317,12 @@ qmckl_exit_code_device qmckl_set_mo_basis_coefficient_device(qmckl_context_devic
 qmckl_exit_code_device qmckl_finalize_mo_basis_device(qmckl_context_device context);
 qmckl_exit_code_device qmckl_provide_mo_basis_mo_vgl_device(qmckl_context_device context);
 qmckl_exit_code_device qmckl_provide_mo_basis_mo_value_device(qmckl_context_device context);
+/* src/qmckl_mo.cpp:31-106 (defined there, declared in no header; its
+ * result type is bool there although it returns exit codes).  Keeps the MOs
+ * i with keep[i] != 0, in order; keep is a HOST array of size_max >= mo_num
+ * entries.  Selecting no MO is rejected (QMCKL_INVALID_ARG_2_DEVICE). */
+qmckl_exit_code_device qmckl_mo_basis_select_mo_device(qmckl_context_device context, int32_t *keep,
+                                                       int64_t size_max);
 /* mo_vgl [P][5][M], size_max >= 5*M*P */
 qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_device(qmckl_context_device context,
                                                         double *mo_vgl, int64_t size_max);

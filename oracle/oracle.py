@@ -33,6 +33,7 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         basis = [_I64, _I64, _I32, _I64, _I64, _D, _D, _D, _D]
         L.qo_en_distance.argtypes = [_i64, _i64, _D, _D, _D]
+        L.qo_ee_distance.argtypes = [_i64, _i64, _D, _D]
         L.qo_shell_vgl.argtypes = [_i64, _i64, _i64, _i64, _D, _D] + basis + [_D]
         L.qo_ao_vgl.argtypes = [_i64, _i64, _i64, _i64, _i64, _D, _D] + basis + [_D, _D, _D]
         L.qo_ao_value.argtypes = [_i64, _i64, _i64, _i64, _i64, _D, _D] + basis + [_D, _D]
@@ -90,6 +91,16 @@ def en_distance(sys_, coord_T=None):
     out = np.empty((P, sys_.nucl_num))
     nc = _c(sys_.nucl_coord)
     _check(lib().qo_en_distance(P, sys_.nucl_num, _p(coord), _p(nc), _p(out)), "en_distance")
+    return out
+
+
+def ee_distance(sys_, coord_T=None):
+    """[walk][elec][elec] electron-electron distances of each walker."""
+    coord = _c(sys_.coord_T() if coord_T is None else coord_T)
+    E = sys_.elec_num
+    W = coord.shape[1] // E
+    out = np.empty((W, E, E))
+    _check(lib().qo_ee_distance(E, W, _p(coord), _p(out)), "ee_distance")
     return out
 
 

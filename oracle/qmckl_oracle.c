@@ -82,6 +82,27 @@ int qo_en_distance(int64_t P, int64_t N, const double *coord,
   return 0;
 }
 
+/* Electron-electron distances per walker (src/qmckl_electron.cpp:333-371:
+ * qmckl_distance('T','T') of each walker's electrons against themselves):
+ * ee_distance[w][j][i] = |r_(w,i) - r_(w,j)|, coord [3][W*E], point w*E + i. */
+int qo_ee_distance(int64_t E, int64_t W, const double *coord, double *ee_distance) {
+  if (E <= 0 || W <= 0) return 1;
+  const int64_t P = E * W;
+#pragma omp parallel for schedule(static)
+  for (int64_t w = 0; w < W; ++w) {
+    for (int64_t j = 0; j < E; ++j) {
+      for (int64_t i = 0; i < E; ++i) {
+        const int64_t a = w * E + i, b = w * E + j;
+        double x = coord[a] - coord[b];
+        double y = coord[P + a] - coord[P + b];
+        double z = coord[2 * P + a] - coord[2 * P + b];
+        ee_distance[(w * E + j) * E + i] = sqrt(r2_of(x, y, z));
+      }
+    }
+  }
+  return 0;
+}
+
 int qo_finalize_ao(int64_t N, int64_t S, int64_t Np,
                    const int64_t *nucleus_index, const int64_t *nucleus_shell_num,
                    const int32_t *shell_ang_mom, const int64_t *shell_prim_num,

@@ -35,6 +35,9 @@ int qo_get_num_threads(void);
 int qo_en_distance(int64_t P, int64_t N, const double *coord,
                    const double *nucl_coord, double *en_distance);
 
+/* ee_distance [W][E][E], coord [3][W*E] (point w*E + i). */
+int qo_ee_distance(int64_t E, int64_t W, const double *coord, double *ee_distance);
+
 /* Derived basis data computed at finalize time.  coef_norm[Np],
  * nucleus_max_ang_mom[N], nucleus_range[N]. */
 int qo_finalize_ao(int64_t N, int64_t S, int64_t Np,

@@ -94,6 +94,9 @@ struct Electron {
   double *en_distance = nullptr;  // [P][N]
   int64_t en_distance_cap = 0;
   uint64_t en_distance_date = 0;
+  double *ee_distance = nullptr;  // [walk][E][E]
+  int64_t ee_distance_cap = 0;
+  uint64_t ee_distance_date = 0;
   int32_t uninitialized = (1 << 1) - 1;
   bool provided = false;
 };
@@ -242,6 +245,8 @@ hipError_t launch_soa_from_aos(hipStream_t s, int64_t n, const double *aos, doub
 hipError_t launch_aos_from_soa(hipStream_t s, int64_t n, const double *soa, double *aos);
 hipError_t launch_transpose(hipStream_t s, int64_t rows, int64_t cols, const double *in,
                             double *out);
+// ee_distance [W][E][E] of the W walkers' E electrons, coord [3][W*E]
+hipError_t launch_ee_distance(hipStream_t s, int64_t E, int64_t W, const double *coord, double *out);
 hipError_t launch_en_distance(hipStream_t s, int64_t P, int64_t N, const double *coord,
                               const double *nucl, double *out);
 hipError_t launch_shell_vgl(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,

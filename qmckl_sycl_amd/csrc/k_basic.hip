@@ -1,4 +1,5 @@
-// Layout kernels and the electron-nucleus distance kernel.
+// Layout kernels and the electron-nucleus / electron-electron distance
+// kernels.
 //
 // en_distance[p][n] = |r_p - R_n| (src/qmckl_electron.cpp:302-331 calls
 // qmckl_distance_device('T','T') whose kernel is at 536-549; here the plain
@@ -97,6 +98,43 @@ unsigned grid_for(int64_t work) {
   return (unsigned)g;
 }
 
+// ee_distance[w][j][i] = |r_(w,i) - r_(w,j)| (src/qmckl_electron.cpp:333-371,
+// qmckl_distance('T','T') of each walker's electrons against themselves).
+// Workgroup = one walker's 64 x 64 tile of (i, j): the 64 j coordinates are
+// staged in LDS, thread (i, j0) computes j = j0, j0+4, ..., and every store
+// instruction of a wavefront writes 64 consecutive i of one row.  HBM-bound:
+// 8 B written per pair.
+constexpr int kEeTile = 64;
+__global__ void __launch_bounds__(kThreads) ee_distance_kernel(int64_t E, int64_t W, int64_t tiles,
+                                                               const double *__restrict__ coord,
+                                                               double *__restrict__ out) {
+  __shared__ double sj[3][kEeTile];
+  const int64_t P = E * W;
+  const int64_t b = blockIdx.x;
+  const int64_t w = b / (tiles * tiles);
+  const int64_t rem = b - w * tiles * tiles;
+  const int64_t i0 = (rem % tiles) * kEeTile, j0 = (rem / tiles) * kEeTile;
+  const int tid = threadIdx.x;
+  if (tid < kEeTile) {
+    const int64_t j = j0 + tid;
+    const int64_t pj = w * E + (j < E ? j : 0);
+    sj[0][tid] = coord[pj];
+    sj[1][tid] = coord[P + pj];
+    sj[2][tid] = coord[2 * P + pj];
+  }
+  __syncthreads();
+  const int64_t i = i0 + (tid & 63);
+  if (i >= E) return;
+  const int64_t pi = w * E + i;
+  const double xi = coord[pi], yi = coord[P + pi], zi = coord[2 * P + pi];
+  for (int jj = tid >> 6; jj < kEeTile; jj += kThreads / 64) {
+    const int64_t j = j0 + jj;
+    if (j >= E) break;
+    const double x = xi - sj[0][jj], y = yi - sj[1][jj], z = zi - sj[2][jj];
+    out[(w * E + j) * E + i] = sqrt(fma(x, x, fma(y, y, z * z)));
+  }
+}
+
 }  // namespace
 
 hipError_t launch_soa_from_aos(hipStream_t s, int64_t n, const double *aos, double *soa) {
@@ -133,6 +171,16 @@ hipError_t launch_en_distance(hipStream_t s, int64_t P, int64_t N, const double 
   const size_t lds = N <= kMaxNuclLds ? sizeof(double) * 3 * (size_t)N : 0;
   hipLaunchKernelGGL(en_distance_kernel, dim3(grid_for(P * N)), dim3(kThreads), lds, s, P, N, coord,
                      nucl, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_ee_distance(hipStream_t s, int64_t E, int64_t W, const double *coord, double *out) {
+  if (E <= 0 || W <= 0) return hipSuccess;
+  const int64_t tiles = (E + kEeTile - 1) / kEeTile;
+  const int64_t blocks = W * tiles * tiles;
+  if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(ee_distance_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, E, W, tiles, coord,
+                     out);
   return hipGetLastError();
 }
 

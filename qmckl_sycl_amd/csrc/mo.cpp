@@ -362,6 +362,49 @@ qmckl_exit_code_device qmckl_get_mo_basis_precision_device(qmckl_context_device 
   return QMCKL_SUCCESS_DEVICE;
 }
 
+// MO subset: keeps the MOs i with keep[i] != 0 (host array), in order
+// (src/qmckl_mo.cpp:31-106), then re-finalizes the MO basis.
+qmckl_exit_code_device qmckl_mo_basis_select_mo_device(qmckl_context_device context, int32_t *keep,
+                                                       int64_t size_max) {
+  Context *c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+  const char *fn = "qmckl_mo_basis_select_mo_device";
+  if (!c->mo.provided) return fail(context, QMCKL_NOT_PROVIDED_DEVICE, fn, nullptr);
+  if (keep == nullptr) return fail(context, QMCKL_INVALID_ARG_2_DEVICE, fn, "NULL pointer");
+  const int64_t M = c->mo.mo_num, A = c->ao.ao_num;
+  if (size_max < M) return fail(context, QMCKL_INVALID_ARG_3_DEVICE, fn, "Array too small: expected mo_num.");
+  int64_t kept = 0;
+  for (int64_t i = 0; i < M; ++i) kept += keep[i] != 0;
+  if (kept == 0) return fail(context, QMCKL_INVALID_ARG_2_DEVICE, fn, "no MO selected");
+  if (!has_device(c)) return fail(context, QMCKL_FAILURE_DEVICE, fn, "no device bound");
+  double *nc = static_cast<double *>(dmalloc(c, sizeof(double) * A * kept));
+  if (!nc) return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
+  // copy runs of consecutive kept rows of coefficient [M][A]
+  hipError_t e = hipSuccess;
+  int64_t dst = 0;
+  for (int64_t i = 0; i < M && e == hipSuccess;) {
+    if (keep[i] == 0) {
+      ++i;
+      continue;
+    }
+    int64_t j = i;
+    while (j < M && keep[j] != 0) ++j;
+    e = hipMemcpyAsync(nc + dst * A, c->mo.coefficient + i * A, sizeof(double) * A * (j - i),
+                       hipMemcpyDeviceToDevice, c->stream);
+    dst += j - i;
+    i = j;
+  }
+  if (e == hipSuccess) e = sync(c);
+  if (e != hipSuccess) {
+    dfree(c, nc);
+    return hip_check(context, e, fn);
+  }
+  dfree(c, c->mo.coefficient);
+  c->mo.coefficient = nc;
+  c->mo.mo_num = kept;
+  return qmckl_finalize_mo_basis_device(context);
+}
+
 // Batching form: each context's chain is enqueued on its own stream with the
 // chain's synchronisations deferred, then every stream is waited for once.
 qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_batch_device(int64_t count,

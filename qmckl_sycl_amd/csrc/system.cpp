@@ -340,4 +340,66 @@ qmckl_exit_code_device qmckl_get_electron_en_distance_device(qmckl_context_devic
   return hip_check(context, e, fn);
 }
 
+// ---- electron-electron distances ----
+
+qmckl_exit_code_device qmckl_compute_ee_distance_device(qmckl_context_device context, int64_t elec_num,
+                                                        int64_t walk_num, const double *coord,
+                                                        double *ee_distance) {
+  Context *c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+  const char *fn = "qmckl_compute_ee_distance_device";
+  if (elec_num <= 0) return QMCKL_INVALID_ARG_2_DEVICE;
+  if (walk_num <= 0) return QMCKL_INVALID_ARG_3_DEVICE;
+  if (!coord) return fail(context, QMCKL_INVALID_ARG_4_DEVICE, fn, "NULL pointer");
+  if (!ee_distance) return fail(context, QMCKL_INVALID_ARG_5_DEVICE, fn, "NULL pointer");
+  if (!has_device(c)) return fail(context, QMCKL_FAILURE_DEVICE, fn, "no device bound");
+  hipError_t e;
+  {
+    KernelTimer t(c, "ee_distance");
+    e = launch_ee_distance(c->stream, elec_num, walk_num, coord, ee_distance);
+    t.stop();
+  }
+  if (e == hipSuccess) e = sync(c);
+  return hip_check(context, e, fn);
+}
+
+qmckl_exit_code_device qmckl_provide_ee_distance_device(qmckl_context_device context) {
+  Context *c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+  const char *fn = "qmckl_provide_ee_distance_device";
+  if (!c->electron.provided) return fail(context, QMCKL_NOT_PROVIDED_DEVICE, fn, nullptr);
+  const int64_t E = c->electron.num, W = c->electron.walker.num;
+  if (W <= 0 || c->point.coord == nullptr || c->point.num != E * W)
+    return fail(context, QMCKL_NOT_PROVIDED_DEVICE, fn, "no electron coordinates set");
+  if (c->point.date > c->electron.ee_distance_date || c->electron.ee_distance == nullptr) {
+    const int64_t need = E * E * W;
+    if (need > c->electron.ee_distance_cap) {
+      if (!drealloc(c, reinterpret_cast<void **>(&c->electron.ee_distance), sizeof(double) * need))
+        return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
+      c->electron.ee_distance_cap = need;
+    }
+    qmckl_exit_code_device rc =
+        qmckl_compute_ee_distance_device(context, E, W, c->point.coord, c->electron.ee_distance);
+    if (rc != QMCKL_SUCCESS_DEVICE) return rc;
+    c->electron.ee_distance_date = c->date;
+  }
+  return QMCKL_SUCCESS_DEVICE;
+}
+
+qmckl_exit_code_device qmckl_get_electron_ee_distance_device(qmckl_context_device context,
+                                                             double *ee_distance, int64_t size_max) {
+  Context *c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+  const char *fn = "qmckl_get_electron_ee_distance_device";
+  if (ee_distance == nullptr) return fail(context, QMCKL_INVALID_ARG_2_DEVICE, fn, "NULL pointer");
+  qmckl_exit_code_device rc = qmckl_provide_ee_distance_device(context);
+  if (rc != QMCKL_SUCCESS_DEVICE) return rc;
+  const int64_t sze = c->electron.num * c->electron.num * c->electron.walker.num;
+  if (size_max < sze) return fail(context, QMCKL_INVALID_ARG_3_DEVICE, fn, "input array too small");
+  hipError_t e = hipMemcpyAsync(ee_distance, c->electron.ee_distance, sizeof(double) * sze,
+                                hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess) e = sync(c);
+  return hip_check(context, e, fn);
+}
+
 }  // extern "C"

@@ -287,3 +287,16 @@ def test_destroyed_context_is_invalid():
     n = ctypes.c_int64(0)
     assert L.qmckl_get_nucleus_num_device(h, ctypes.byref(n)) == INVALID_CONTEXT
     assert L.qmckl_context_destroy_device(h) == INVALID_CONTEXT
+
+
+def test_select_mo_and_ee_distance_validation(ctx):
+    L = lib()
+    keep = (ctypes.c_int32 * 4)(1, 0, 1, 1)
+    assert L.qmckl_mo_basis_select_mo_device(ctx.handle, keep, 4) == NOT_PROVIDED
+    assert L.qmckl_mo_basis_select_mo_device(0, keep, 4) == INVALID_CONTEXT
+    assert L.qmckl_provide_ee_distance_device(ctx.handle) == NOT_PROVIDED
+    assert L.qmckl_get_electron_ee_distance_device(ctx.handle, None, 10) == 2
+    assert L.qmckl_compute_ee_distance_device(ctx.handle, 0, 1, 1234, 1234) == 2
+    assert L.qmckl_compute_ee_distance_device(ctx.handle, 3, 0, 1234, 1234) == 3
+    assert L.qmckl_compute_ee_distance_device(ctx.handle, 3, 1, None, 1234) == 4
+    assert L.qmckl_compute_ee_distance_device(ctx.handle, 3, 1, 1234, None) == 5

@@ -82,6 +82,24 @@ def test_en_distance(oracle_lib, chbrclf):
         assert abs(d[p, n] - exact) <= 2 * np.spacing(exact)
 
 
+def test_ee_distance(oracle_lib, chbrclf):
+    """ee_distance per walker: numpy brute force, exact rational r^2 with a
+    correctly rounded sqrt, symmetry and a zero diagonal."""
+    d = oracle_lib.ee_distance(chbrclf)
+    W, E = chbrclf.walk_num, chbrclf.elec_num
+    assert d.shape == (W, E, E)
+    r = chbrclf.elec_coord  # [walk][elec][3]
+    ref = np.linalg.norm(r[:, None, :, :] - r[:, :, None, :], axis=3)  # [w][j][i]
+    nz = ref > 0
+    assert np.max(np.abs(d[nz] - ref[nz]) / ref[nz]) < 4e-16
+    assert np.array_equal(d, d.transpose(0, 2, 1))
+    assert np.all(np.diagonal(d, axis1=1, axis2=2) == 0.0)
+    for w, j, i in [(0, 0, 1), (1, 26, 40), (1, 67, 3)]:
+        s2 = sum((Fraction(float(r[w, i, k])) - Fraction(float(r[w, j, k]))) ** 2 for k in range(3))
+        exact = math.sqrt(float(s2))
+        assert abs(d[w, j, i] - exact) <= 2 * np.spacing(exact)
+
+
 def _one_shell_system(alpha, coef, l, points, expo_extra=()):
     """1 nucleus at the origin with one contracted shell of angular momentum l."""
     alpha = np.atleast_1d(np.asarray(alpha, dtype=np.float64))

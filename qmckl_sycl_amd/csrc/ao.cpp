@@ -87,8 +87,10 @@ void free_tables(Context *c, DeviceBasis &b) {
   dfree(c, b.nucl_max_l);
   dfree(c, b.nucl_prim_index);
   dfree(c, b.chunks);
+  dfree(c, b.stage_shell);
   b.chunks = nullptr;
   b.nchunks = 0;
+  b.stage_shell = nullptr;
   b.shells = nullptr;
   b.ao_shell = b.ao_abc = b.nucl_max_l = nullptr;
   b.coef_norm = b.nucl_range = nullptr;
@@ -188,8 +190,15 @@ std::string build_basis_tables(Context *c, DeviceBasis &b, int64_t N, int64_t S,
   chunks.push_back((int32_t)shells.size());
   b.nchunks = (int)chunks.size() - 1;
   b.chunks = h2d_new(c, chunks);
+  // first shell holding an AO of each 64-AO stage (S past the last AO), for
+  // kernels that split the AOs of a point between workgroups
+  const int64_t nst = (A + 63) / 64;
+  std::vector<int32_t> stage_shell((size_t)nst + 1);
+  for (int64_t st = 0; st <= nst; ++st)
+    stage_shell[(size_t)st] = 64 * st < A ? ao_shell[(size_t)(64 * st)] : (int32_t)shells.size();
+  b.stage_shell = h2d_new(c, stage_shell);
   if (!b.shells || !b.ao_shell || !b.ao_abc || !b.coef_norm || !b.nucl_range || !b.nucl_max_l ||
-      !b.nucl_prim_index || !b.chunks)
+      !b.nucl_prim_index || !b.chunks || !b.stage_shell)
     return "device allocation failed";
   if (sync(c) != hipSuccess) return "copy to device failed";
   return "";

@@ -64,6 +64,9 @@ struct DeviceBasis {
   // a chunk holds at most kChunkShells shells and kChunkAos AOs
   int32_t *chunks = nullptr;
   int nchunks = 0;
+  // [ceil(A/64)+1] position in `shells` of the shell holding AO 64*st (S
+  // past the last AO)
+  int32_t *stage_shell = nullptr;
   int max_shell_prims = 0;         // host copy: largest prim count of a shell
   int max_l = 0;                   // host copy
 };
@@ -256,11 +259,13 @@ hipError_t launch_ao_vgl(hipStream_t s, const DeviceBasis &b, int64_t P, const d
                          int ncomp, uint8_t *flags = nullptr, int nkb = 0);
 // AO operand of the mixed contraction, FP32 in the blocked layout
 // buf[c][64-point tile][AO/4][point in tile][4] (AOs zero-padded to ld, a
-// multiple of 64), row maxima rowmax[c][point] and, if flags is not null,
-// nonzero flags [5 * tiles][nkb] with nkb = ld / 32.
+// multiple of 64), partial row maxima rowmax[group][c][point] over each of
+// `groups` AO groups (ao_rows_groups) and, if flags is not null, nonzero
+// flags [5 * tiles][nkb] with nkb = ld / 32.
+int ao_rows_groups(int64_t P, int64_t ld);
 hipError_t launch_ao_rows_f32(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
-                              const double *nucl, const double *ao_factor, int64_t ld, float *buf,
-                              float *rowmax, uint8_t *flags, int nkb);
+                              const double *nucl, const double *ao_factor, int64_t ld, int groups,
+                              float *buf, float *rowmax, uint8_t *flags, int nkb);
 hipError_t launch_ao_from_shell_table(hipStream_t s, const DeviceBasis &b, int64_t P,
                                       const double *coord, const double *nucl,
                                       const double *ao_factor, const double *shell_vgl,
@@ -303,9 +308,9 @@ hipError_t launch_split_coef(hipStream_t s, int form, int64_t M, int64_t K, cons
 // Mixed MO contraction of the blocked AO operand (launch_ao_rows_f32) of P
 // points with the prepared coefficients into mo_vgl C [P][5][Ncol] (ldc).
 hipError_t launch_mo_gemm_mixed(hipStream_t s, int form, int64_t P, int64_t K, int64_t Ncol,
-                                const float *A, int64_t lda, const float *rowmax, const void *B,
-                                int64_t ldb, const float *colinv, const uint8_t *flags, int nkb,
-                                double *C, int64_t ldc);
+                                const float *A, int64_t lda, const float *rowmax, int groups,
+                                const void *B, int64_t ldb, const float *colinv, const uint8_t *flags,
+                                int nkb, double *C, int64_t ldc);
 hipError_t launch_strided_copy(hipStream_t s, int64_t rows, int64_t cols, const double *in,
                                int64_t ld_in, double *out, int64_t ld_out);
 

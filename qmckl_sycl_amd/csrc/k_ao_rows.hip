@@ -15,17 +15,22 @@
 // pieces, 64 of which (one per point of the tile) form a contiguous
 // kilobyte for each group of 4 AOs.
 // AOs past ao_num up to ld (a multiple of 64) are written as zeros.
-// Also written: rowmax[c][point] = max |entry| of the row (each lane owns
-// its five rows, so no reduction is needed), and nonzero flags per
-// (row tile, 32-AO block) gathered with wavefront ballots.
+// Also written: rowmax[group][c][point] = max |entry| of the row over the
+// group's AOs (each lane owns its five rows, so no reduction is needed; the
+// contraction takes the max over groups), and nonzero flags per (row tile,
+// 32-AO block) gathered with wavefront ballots.
 //
-// Work decomposition: a wavefront owns 64 points and walks all shells in
-// AO order; the shell loop is wavefront-uniform (shell data, nucleus and
+// Work decomposition: a wavefront owns 64 points and one group of 64-AO
+// stages (enough groups per point tile for about four wavefronts per SIMD;
+// a shell that straddles a group boundary is evaluated by both groups, each
+// writing its own AOs) and walks those shells in AO order; the shell loop is
+// wavefront-uniform (shell data, nucleus and
 // primitives come through scalar loads; a primitive screened for all 64
 // points costs no exp), and for l <= 3 the shell's Cartesian components
 // are unrolled at compile time, so each AO costs a few FP64 multiplies on
-// powers computed once per (point, shell).  No LDS, no
-// barriers: the four wavefronts of a workgroup are independent.
+// powers computed once per (point, shell).  No barriers: the four
+// wavefronts of a workgroup are independent, each with its own LDS ring for
+// assembling 16-byte pieces.
 #include <hip/hip_runtime.h>
 
 #include "internal.hpp"
@@ -37,10 +42,11 @@ namespace {
 constexpr int kThreads = 256;
 
 // Running state of one lane: the AO counter of the wavefront, row maxima
-// and nonzero bits of the current 32-AO block.  Each value is stored as it
-// is produced: the four 4-byte stores of a 16-byte piece come from four
-// consecutive AOs of the same lane, so L2 assembles whole lines before they
-// reach HBM.
+// and nonzero bits of the current 32-AO block.  Values are staged in the
+// wavefront's LDS ring [component][AO mod 4][lane] (the slot is a runtime
+// index, which registers cannot take without spilling); after every fourth
+// AO each lane reads its four values back and stores them as one 16-byte
+// piece, so each store instruction writes one contiguous kilobyte.
 struct RowWriter {
   float mx[5];
   uint32_t nz;       // bit c: some AO of the current 32-block is nonzero in row c
@@ -49,16 +55,31 @@ struct RowWriter {
   int64_t cstride;   // floats between components
   uint8_t *flags;    // flags + pt * nkb, component stride fstride
   int64_t fstride;
+  float *ring;       // this wavefront's [5][4][64] LDS ring
   int lane;
 
+  int k_lo, k_hi;    // AOs this wavefront writes (multiples of 64)
+
   __device__ __attribute__((always_inline)) void push(const double (&v)[5]) {
-    float *o = base + (int64_t)(k >> 2) * 256 + (k & 3);
+    if (k < k_lo || k >= k_hi) {  // part of a shell that another wavefront writes
+      ++k;
+      return;
+    }
+    const int slot = k & 3;
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
       const float w = (float)v[c];
-      o[c * cstride] = w;
+      ring[(c * 4 + slot) * 64 + lane] = w;
       mx[c] = fmaxf(mx[c], fabsf(w));
       nz |= (w != 0.0f ? 1u : 0u) << c;
+    }
+    if (slot == 3) {
+      float *o = base + (int64_t)(k >> 2) * 256;
+#pragma unroll
+      for (int c = 0; c < 5; ++c)
+        *reinterpret_cast<float4 *>(o + c * cstride) =
+            make_float4(ring[(c * 4 + 0) * 64 + lane], ring[(c * 4 + 1) * 64 + lane],
+                        ring[(c * 4 + 2) * 64 + lane], ring[(c * 4 + 3) * 64 + lane]);
     }
     if ((k & 31) == 31) {
       if (flags != nullptr) {
@@ -169,12 +190,16 @@ __device__ __attribute__((always_inline)) inline void shell_aos_any(RowWriter &w
 __global__ void __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads), amdgpu_waves_per_eu(4)))
 ao_rows_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord,
                const double *__restrict__ nucl, const ShellDesc *__restrict__ shells,
-               const double *__restrict__ ao_factor, const double *__restrict__ expo,
-               const double *__restrict__ coef, const double *__restrict__ nucl_range, int64_t A,
-               int64_t ld, int64_t npt, float *__restrict__ buf, float *__restrict__ rowmax,
-               uint8_t *__restrict__ flags, int nkb) {
+               const int32_t *__restrict__ stage_shell, const double *__restrict__ ao_factor,
+               const double *__restrict__ expo, const double *__restrict__ coef,
+               const double *__restrict__ nucl_range, int64_t A, int64_t ld, int64_t npt, int groups,
+               float *__restrict__ buf, float *__restrict__ rowmax, uint8_t *__restrict__ flags, int nkb) {
+  __shared__ float s_ring[kThreads / 64][5 * 4 * 64];
   const int lane = threadIdx.x & 63;
-  const int64_t pt = (int64_t)blockIdx.x * (kThreads / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t wi = (int64_t)blockIdx.x * (kThreads / 64) + wave;
+  const int64_t pt = wi / groups;
+  const int g = (int)(wi - pt * groups);
   if (pt >= npt) return;  // wavefront-uniform
   const int64_t p = pt * 64 + lane;
   const bool inside = p < P;
@@ -182,19 +207,27 @@ ao_rows_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord,
   const double py = inside ? coord[P + p] : 0.0;
   const double pz = inside ? coord[2 * P + p] : 0.0;
 
+  // this wavefront's AOs: 64-AO stages [g*nst/groups, (g+1)*nst/groups)
+  const int64_t nst = ld / 64, nst_a = (A + 63) / 64;
+  const int64_t st_lo = g * nst / groups, st_hi = (g + 1) * nst / groups;
   RowWriter w;
 #pragma unroll
   for (int c = 0; c < 5; ++c) w.mx[c] = 0.0f;
   w.nz = 0;
-  w.k = 0;
+  w.k_lo = (int)(64 * st_lo);
+  w.k_hi = (int)(64 * st_hi);
   w.cstride = npt * (ld / 4) * 256;
   w.base = buf + pt * (ld / 4) * 256 + lane * 4;
   w.flags = flags != nullptr ? flags + pt * nkb : nullptr;
   w.fstride = npt * nkb;
   w.lane = lane;
+  w.ring = s_ring[wave];
 
-  for (int64_t s = 0; s < S; ++s) {
+  const int64_t s_first = st_lo < nst_a ? stage_shell[st_lo] : S;
+  w.k = s_first < S ? shells[s_first].ao_start : w.k_lo;
+  for (int64_t s = s_first; s < S; ++s) {
     const ShellDesc d = shells[s];
+    if (d.ao_start >= w.k_hi) break;
     const double x = px - nucl[d.nucl];
     const double y = py - nucl[N + d.nucl];
     const double z = pz - nucl[2 * N + d.nucl];
@@ -224,32 +257,47 @@ ao_rows_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord,
       default: shell_aos_any(w, ao_factor, x, y, z, R, d.l); break;
     }
   }
-  // zero AOs up to ld (the padding the contraction reads, and the last 32-block's flags)
+  // zero AOs from ao_num up to this wavefront's end (the padding the
+  // contraction reads, and the last 32-block's flags)
+  if (w.k < w.k_lo) w.k = w.k_lo;
   const double zero[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  while (w.k < ld) w.push(zero);
-  if (inside) {
+  while (w.k < w.k_hi) w.push(zero);
+  // partial row maxima of this AO group, [groups][5][npt*64]
+  float *rm = rowmax + (int64_t)g * 5 * npt * 64 + pt * 64 + lane;
 #pragma unroll
-    for (int c = 0; c < 5; ++c) rowmax[(c * npt + pt) * 64 + lane] = w.mx[c];
-  } else {
-#pragma unroll
-    for (int c = 0; c < 5; ++c) rowmax[(c * npt + pt) * 64 + lane] = 0.0f;
-  }
+  for (int c = 0; c < 5; ++c) rm[c * npt * 64] = inside ? w.mx[c] : 0.0f;
 }
 
 }  // namespace
 
+int ao_rows_groups(int64_t P, int64_t ld) {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    cus = 256;
+  const int64_t npt = (P + 63) / 64, nst = ld / 64;
+  const int64_t want = 16 * (int64_t)cus;  // about four wavefronts per SIMD
+  int64_t g = (want + npt - 1) / npt;
+  if (g > nst) g = nst;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
 hipError_t launch_ao_rows_f32(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
-                              const double *nucl, const double *ao_factor, int64_t ld, float *buf,
-                              float *rowmax, uint8_t *flags, int nkb) {
+                              const double *nucl, const double *ao_factor, int64_t ld, int groups,
+                              float *buf, float *rowmax, uint8_t *flags, int nkb) {
   if (P <= 0) return hipSuccess;
-  if (b.max_l > kMaxL || ld % 64 != 0 || ld < b.ao_num) return hipErrorInvalidValue;
+  if (b.max_l > kMaxL || ld % 64 != 0 || ld < b.ao_num || groups < 1 || groups > ld / 64 ||
+      b.stage_shell == nullptr)
+    return hipErrorInvalidValue;
   if (flags != nullptr && (int64_t)nkb * 32 != ld) return hipErrorInvalidValue;
   const int64_t npt = (P + 63) / 64;
-  const int64_t blocks = (npt + kThreads / 64 - 1) / (kThreads / 64);
-  if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+  const int64_t waves = npt * groups;
+  const int64_t blocks = (waves + kThreads / 64 - 1) / (kThreads / 64);
+  if (blocks > 0x7fffffff || ld > 0x7fffffff) return hipErrorInvalidValue;
   hipLaunchKernelGGL(ao_rows_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, P, (int)b.nucl_num,
-                     b.shell_num, coord, nucl, b.shells, ao_factor, b.expo, b.coef_norm, b.nucl_range,
-                     b.ao_num, ld, npt, buf, rowmax, flags, nkb);
+                     b.shell_num, coord, nucl, b.shells, b.stage_shell, ao_factor, b.expo, b.coef_norm,
+                     b.nucl_range, b.ao_num, ld, npt, groups, buf, rowmax, flags, nkb);
   return hipGetLastError();
 }
 

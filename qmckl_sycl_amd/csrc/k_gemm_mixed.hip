@@ -152,7 +152,7 @@ __device__ inline void split8(const float (&v)[8], uint4 (&out)[Terms<SP>::n]) {
 template <int SP, int CBW>
 __global__ void __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads), amdgpu_waves_per_eu(1, 1)))
 gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *__restrict__ A,
-                  int64_t lda, const float *__restrict__ rowmax, const uint16_t *__restrict__ Bs,
+                  int64_t lda, const float *__restrict__ rowmax, int groups, const uint16_t *__restrict__ Bs,
                   int64_t ldb, const float *__restrict__ colinv, const uint8_t *__restrict__ flags, int nkb,
                   double *__restrict__ C, int64_t ldc) {
   constexpr int T = Terms<SP>::n;
@@ -194,7 +194,9 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
   // scale (from its max |entry| over all K).
   const int ar = lane, aq = wave * 16;
   const float *Ap = A + rt * (lda / 4) * 256 + (aq / 4) * 256 + ar * 4;
-  const float sc = pow2_scale(rowmax[rt * 64 + ar]);
+  float rmax = 0.0f;  // the row's max |entry|: max over the AO groups' partial maxima
+  for (int g = 0; g < groups; ++g) rmax = fmaxf(rmax, rowmax[(int64_t)g * 5 * npt * 64 + rt * 64 + ar]);
+  const float sc = pow2_scale(rmax);
   if (wave == 0) s_inv[ar] = 1.0f / sc;
   // C operand: lane holds B[8*lk..8*lk+7][li] of each 16-column tile
   const uint16_t *Bp = Bs + (col0 + li) * T * ldb + 8 * lk;
@@ -444,21 +446,22 @@ hipError_t launch_f32(hipStream_t s, int64_t P, int64_t npt, int64_t K, int64_t 
 
 template <int SP, int CBW>
 hipError_t launch_split(hipStream_t s, int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *A,
-                        int64_t lda, const float *rowmax, const void *B, int64_t ldb, const float *colinv,
-                        const uint8_t *flags, int nkb, double *C, int64_t ldc) {
+                        int64_t lda, const float *rowmax, int groups, const void *B, int64_t ldb,
+                        const float *colinv, const uint8_t *flags, int nkb, double *C, int64_t ldc) {
   const int64_t gx = 5 * npt, gy = (Ncol + 64 * CBW - 1) / (64 * CBW);
   if (gx > 0x7fffffff || gy > 65535 || (lda % kStage) != 0 || (ldb % kStage) != 0)
     return hipErrorInvalidValue;
   if (flags != nullptr && (nkb > kMaxFlagBlocks || (int64_t)nkb * 32 < K)) return hipErrorInvalidValue;
   hipLaunchKernelGGL((gemm_split_kernel<SP, CBW>), dim3((unsigned)gx, (unsigned)gy), dim3(kThreads), 0, s,
-                     P, npt, K, Ncol, A, lda, rowmax, (const uint16_t *)B, ldb, colinv, flags, nkb, C, ldc);
+                     P, npt, K, Ncol, A, lda, rowmax, groups, (const uint16_t *)B, ldb, colinv, flags, nkb,
+                     C, ldc);
   return hipGetLastError();
 }
 
 template <int SP>
 hipError_t launch_form(hipStream_t s, int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *A,
-                       int64_t lda, const float *rowmax, const void *B, int64_t ldb, const float *colinv,
-                       const uint8_t *flags, int nkb, double *C, int64_t ldc) {
+                       int64_t lda, const float *rowmax, int groups, const void *B, int64_t ldb,
+                       const float *colinv, const uint8_t *flags, int nkb, double *C, int64_t ldc) {
   if constexpr (SP == QMCKL_AMD_SPLIT_FP32) {
     const int64_t nb = (Ncol + 31) / 32;  // 32-column groups
     if (nb <= 1) return launch_f32<1>(s, P, npt, K, Ncol, A, lda, B, ldb, flags, nkb, C, ldc);
@@ -468,12 +471,12 @@ hipError_t launch_form(hipStream_t s, int64_t P, int64_t npt, int64_t K, int64_t
     return launch_f32<5>(s, P, npt, K, Ncol, A, lda, B, ldb, flags, nkb, C, ldc);
   } else {
     const int cbw = split_wg_cols(SP, Ncol) / 64;
-    if (cbw == 1) return launch_split<SP, 1>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, C, ldc);
-    if (cbw == 2) return launch_split<SP, 2>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, C, ldc);
-    if (cbw == 3) return launch_split<SP, 3>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, C, ldc);
+    if (cbw == 1) return launch_split<SP, 1>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, C, ldc);
+    if (cbw == 2) return launch_split<SP, 2>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, C, ldc);
+    if (cbw == 3) return launch_split<SP, 3>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, C, ldc);
     if constexpr (Terms<SP>::n == 2) {
-      if (cbw == 4) return launch_split<SP, 4>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, C, ldc);
-      if (cbw == 5) return launch_split<SP, 5>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, C, ldc);
+      if (cbw == 4) return launch_split<SP, 4>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, C, ldc);
+      if (cbw == 5) return launch_split<SP, 5>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, C, ldc);
     }
     return hipErrorInvalidValue;
   }
@@ -514,21 +517,22 @@ hipError_t launch_split_coef(hipStream_t s, int form, int64_t M, int64_t K, cons
 }
 
 hipError_t launch_mo_gemm_mixed(hipStream_t s, int form, int64_t P, int64_t K, int64_t Ncol,
-                                const float *A, int64_t lda, const float *rowmax, const void *B,
-                                int64_t ldb, const float *colinv, const uint8_t *flags, int nkb, double *C,
-                                int64_t ldc) {
+                                const float *A, int64_t lda, const float *rowmax, int groups,
+                                const void *B, int64_t ldb, const float *colinv, const uint8_t *flags,
+                                int nkb, double *C, int64_t ldc) {
   if (P <= 0 || Ncol <= 0) return hipSuccess;
-  if (K <= 0 || (lda % 64) != 0 || (ldb % 64) != 0 || lda < K || ldb < K) return hipErrorInvalidValue;
+  if (K <= 0 || (lda % 64) != 0 || (ldb % 64) != 0 || lda < K || ldb < K || groups < 1)
+    return hipErrorInvalidValue;
   const int64_t npt = (P + 63) / 64;
   switch (form) {
     case QMCKL_AMD_SPLIT_FP16X2:
-      return launch_form<QMCKL_AMD_SPLIT_FP16X2>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, C, ldc);
+      return launch_form<QMCKL_AMD_SPLIT_FP16X2>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, C, ldc);
     case QMCKL_AMD_SPLIT_BF16X2:
-      return launch_form<QMCKL_AMD_SPLIT_BF16X2>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, C, ldc);
+      return launch_form<QMCKL_AMD_SPLIT_BF16X2>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, C, ldc);
     case QMCKL_AMD_SPLIT_BF16X3:
-      return launch_form<QMCKL_AMD_SPLIT_BF16X3>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, C, ldc);
+      return launch_form<QMCKL_AMD_SPLIT_BF16X3>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, C, ldc);
     case QMCKL_AMD_SPLIT_FP32:
-      return launch_form<QMCKL_AMD_SPLIT_FP32>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, C, ldc);
+      return launch_form<QMCKL_AMD_SPLIT_FP32>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, C, ldc);
     default:
       return hipErrorInvalidValue;
   }

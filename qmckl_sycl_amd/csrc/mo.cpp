@@ -53,9 +53,10 @@ qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Contex
   const int64_t rows = 5 * round_up(P, 64);  // (component, point) rows, points padded to 64
   AoBasis &ao = c->ao;
   MoBasis &mo = c->mo;
+  const int groups = ao_rows_groups(P, ld);
   if (!grow_bytes(c, reinterpret_cast<void **>(&ao.ao_f32), &ao.ao_f32_cap, (int64_t)sizeof(float) * rows * ld) ||
       !grow_bytes(c, reinterpret_cast<void **>(&ao.ao_f32_rowmax), &ao.ao_f32_rowmax_cap,
-                  (int64_t)sizeof(float) * rows))
+                  (int64_t)sizeof(float) * rows * groups))
     return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
   uint8_t *flags = nullptr;
   if (nkb <= kMaxFlagBlocks) {
@@ -76,13 +77,13 @@ qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Contex
   if (e == hipSuccess) {
     KernelTimer t(c, "ao_vgl_f32");
     e = launch_ao_rows_f32(c->stream, ao.dev, P, c->point.coord, c->nucleus.coord, ao.ao_factor, ld,
-                           ao.ao_f32, ao.ao_f32_rowmax, flags, flags ? nkb : 0);
+                           groups, ao.ao_f32, ao.ao_f32_rowmax, flags, flags ? nkb : 0);
     t.stop();
   }
   if (e == hipSuccess) {
     KernelTimer t(c, gemm_timer_name(form));
-    e = launch_mo_gemm_mixed(c->stream, form, P, A, M, ao.ao_f32, ld, ao.ao_f32_rowmax, mo.coef_split, ld,
-                             mo.coef_colinv, flags, flags ? nkb : 0, dst, M);
+    e = launch_mo_gemm_mixed(c->stream, form, P, A, M, ao.ao_f32, ld, ao.ao_f32_rowmax, groups,
+                             mo.coef_split, ld, mo.coef_colinv, flags, flags ? nkb : 0, dst, M);
     t.stop();
   }
   if (e == hipSuccess) e = sync(c);

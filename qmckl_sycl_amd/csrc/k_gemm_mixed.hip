@@ -88,33 +88,40 @@ __device__ inline f4 mfma(const uint4 &a, const uint4 &b, f4 c) {
 }
 
 // Split GEMM (FP16X2, BF16X2, BF16X3).
-//   A      [R][lda] FP32 AO rows, lda a multiple of 64, entries past K zero
-//   Bs     [Npad][T][ldb] split C terms (pre-scaled by their column scale),
-//          K contiguous, ldb a multiple of 64; Npad = Ncol rounded up to the
-//          workgroup width, rows past Ncol zero
+//   A      the FP32 AO operand in the blocked layout of launch_ao_rows_f32:
+//          row tile rt = (component c, 64 points), [rt][lda/4][64][4]
+//   rowmax [groups][5 * tiles * 64] partial row maxima (max over groups)
+//   Bs     split C terms (pre-scaled by their column scale) in the MFMA
+//          fragment order of split_coef_kernel (frag = 16), MOs padded to
+//          the workgroup width with zeros
 //   colinv [Ncol] inverse column scales
-//   flags  [ceil(R/64)][nkb] nonzero flags per (64-row tile, 32-wide K
-//          block), or null
+//   flags  [5 * tiles][nkb] nonzero flags per (row tile, 32-wide K block),
+//          or null
 // Workgroup tile 64 rows x 64*CBW columns (all MOs for mo_num <= 320, so
 // each FP32 AO row is read from HBM once); wavefront w owns columns
 // [16*CBW*w, 16*CBW*(w+1)) for all 64 rows: 4 x CBW MFMA tiles, one
-// wavefront per SIMD.  K advances in stages of 64; stages whose two flags
-// are zero are dropped from the workgroup's stage list up front.  Per stage:
+// wavefront per SIMD.  Each row is scaled by the power of two that brings
+// its largest |entry| into [2^13, 2^14).  K advances in stages of 64;
+// stages whose two flags are zero are dropped from the workgroup's stage
+// list up front.  Per stage:
 //   - the FP32 A slab (64 x 64, loaded into registers while the previous
-//     stage computed) is scaled per row by the power of two that brings the
-//     row's largest |entry| in the stage into [2^13, 2^14) and split into
-//     LDS, shared by the four wavefronts;
-//   - each wavefront reads its C terms straight from global memory (L2) into
-//     registers in the MFMA operand layout (no LDS round trip: no other
-//     wavefront uses them); the next stage's half-stage of C is requested as
-//     soon as the current one has been consumed;
+//     stage computed) is scaled, split and written to LDS, shared by the
+//     four wavefronts;
+//   - each wavefront reads its C terms straight from L2 into registers
+//     (no other wavefront uses them); in fragment order every such load is
+//     one contiguous kilobyte; the next stage's half-stage of C is requested
+//     as soon as the current one has been consumed;
 //   - 2 x 4 x CBW x (3 or 6) MFMAs accumulate in FP32 over the 64 K, and the
-//     stage's sums are unscaled (exactly) into FP64 accumulators.
-// Measured on MI355X at 409,600 x 4,160 x 320: the loop is bound by the
-// C operand traffic through each CU's L1 (64 rows share each C byte) and by
-// the FP64 accumulators' register footprint, which keeps the tile at 64
-// rows and one wavefront per SIMD; v_mfma_f32_32x32x16 tiles and a
-// software-pipelined single-barrier loop were both measured slower here.
+//     stage's sums are added into FP64 accumulators (unscaled, exactly, at
+//     the store).
+// The row (c, point) is written to mo_vgl[point][c][:].
+// Measured on MI355X at 409,600 x 4,160 x 320: storing C in fragment order
+// instead of [MO][term][K] took the kernel from 5.6 to 4.2 ms (a load of 16
+// columns' 64-byte runs touched 16 cache lines); 64 x 96 tiles of
+// v_mfma_f32_32x32x16 (MOs padded to 384) came to 4.5 ms, and a
+// software-pipelined single-barrier loop and persistent workgroups were
+// measured slower.  The FP64 accumulators (160 registers per wavefront for
+// 64 x 80) keep the tile at 64 rows and one wavefront per SIMD.
 constexpr int kRM = 64;           // rows per workgroup
 constexpr int kStage = 64;        // K per stage
 constexpr int kLDS = kStage + 8;  // LDS row length in halves (bank spread)
@@ -198,9 +205,13 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
   for (int g = 0; g < groups; ++g) rmax = fmaxf(rmax, rowmax[(int64_t)g * 5 * npt * 64 + rt * 64 + ar]);
   const float sc = pow2_scale(rmax);
   if (wave == 0) s_inv[ar] = 1.0f / sc;
-  // C operand: lane holds B[8*lk..8*lk+7][li] of each 16-column tile
-  const uint16_t *Bp = Bs + (col0 + li) * T * ldb + 8 * lk;
-  const int64_t cb_stride = 16 * T * ldb;
+  // C operand in fragment order (split_coef_kernel): the 16-byte piece of
+  // lane l for column tile ct, 32-K chunk kc, term t is at
+  // ((ct * nkc + kc) * T + t) * 512 + 8 * l halves, so each load of a
+  // wavefront reads one contiguous kilobyte
+  const int64_t nkc = ldb / 32;
+  const uint16_t *Bp = Bs + (col0 / 16) * nkc * T * 512 + 8 * lane;
+  const int64_t cb_stride = nkc * T * 512;
 
   float4 ra[4];
   uint4 bq[2][CBW][T];
@@ -215,7 +226,7 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
     for (int cb = 0; cb < CBW; ++cb)
 #pragma unroll
       for (int t = 0; t < T; ++t)
-        bq[ks][cb][t] = *reinterpret_cast<const uint4 *>(Bp + cb * cb_stride + t * ldb + k0 + ks * 32);
+        bq[ks][cb][t] = *reinterpret_cast<const uint4 *>(Bp + cb * cb_stride + ((k0 / 32 + ks) * T + t) * 512);
   };
 
   f4 d[4][CBW];
@@ -401,11 +412,16 @@ gemm_f32_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *__
 }
 
 // One workgroup per MO i: column scale from max_k |C[i][k]| and the split
-// terms of C[i][k] * scale at Bs[(i*T + t)*ldb + k], zero past K and for the
-// padding MOs i >= M (FP32 form: the scaled-by-one values at Bs[i*ldb + k]).
+// terms of C[i][k] * scale, zero past K and for the padding MOs i >= M.
+// The terms are stored in the MFMA fragment order of the contraction that
+// will read them (frag = 16: 16x16x32 tiles; 32 would suit 32x32x16): element
+// (i, k) of term t goes to lane l = i % frag + frag * ((k % (512/frag)) / 8),
+// slot k % 8, of fragment ((i / frag) * (ldb / (512/frag)) + k / (512/frag))
+// * T + t, 512 halves per fragment.  FP32 form: scaled-by-one values at
+// Bs[i*ldb + k] (row layout).
 template <int SP>
 __global__ void __launch_bounds__(kThreads)
-split_coef_kernel(int64_t M, int64_t K, const double *__restrict__ coef, int64_t ldb,
+split_coef_kernel(int64_t M, int64_t K, const double *__restrict__ coef, int64_t ldb, int frag,
                   uint16_t *__restrict__ Bs, float *__restrict__ colinv) {
   __shared__ float red[kThreads];
   const int64_t i = blockIdx.x;
@@ -427,7 +443,11 @@ split_coef_kernel(int64_t M, int64_t K, const double *__restrict__ coef, int64_t
     if constexpr (SP == QMCKL_AMD_SPLIT_FP32) {
       reinterpret_cast<float *>(Bs)[i * ldb + k] = x;
     } else {
-      split_store<SP>(x, Bs + i * Terms<SP>::n * ldb + k, (int)ldb);
+      constexpr int T = Terms<SP>::n;
+      const int kw = 512 / frag;  // K per fragment
+      const int64_t lane = i % frag + frag * ((k % kw) / 8);
+      const int64_t fragment = ((i / frag) * (ldb / kw) + k / kw) * T;
+      split_store<SP>(x, Bs + fragment * 512 + lane * 8 + k % 8, 512);
     }
   }
 }
@@ -497,19 +517,21 @@ int split_terms(int form) {
 hipError_t launch_split_coef(hipStream_t s, int form, int64_t M, int64_t K, const double *coef,
                              int64_t ldb, uint16_t *B, float *colinv) {
   if (M <= 0) return hipSuccess;
+  if (ldb % 64 != 0) return hipErrorInvalidValue;
   const unsigned g = (unsigned)split_cols_padded(form, M);
+  const int frag = 16;  // fragment order of gemm_split_kernel's 16x16x32 tiles
   switch (form) {
     case QMCKL_AMD_SPLIT_FP16X2:
-      hipLaunchKernelGGL(split_coef_kernel<QMCKL_AMD_SPLIT_FP16X2>, dim3(g), dim3(kThreads), 0, s, M, K, coef, ldb, B, colinv);
+      hipLaunchKernelGGL(split_coef_kernel<QMCKL_AMD_SPLIT_FP16X2>, dim3(g), dim3(kThreads), 0, s, M, K, coef, ldb, frag, B, colinv);
       break;
     case QMCKL_AMD_SPLIT_BF16X2:
-      hipLaunchKernelGGL(split_coef_kernel<QMCKL_AMD_SPLIT_BF16X2>, dim3(g), dim3(kThreads), 0, s, M, K, coef, ldb, B, colinv);
+      hipLaunchKernelGGL(split_coef_kernel<QMCKL_AMD_SPLIT_BF16X2>, dim3(g), dim3(kThreads), 0, s, M, K, coef, ldb, frag, B, colinv);
       break;
     case QMCKL_AMD_SPLIT_BF16X3:
-      hipLaunchKernelGGL(split_coef_kernel<QMCKL_AMD_SPLIT_BF16X3>, dim3(g), dim3(kThreads), 0, s, M, K, coef, ldb, B, colinv);
+      hipLaunchKernelGGL(split_coef_kernel<QMCKL_AMD_SPLIT_BF16X3>, dim3(g), dim3(kThreads), 0, s, M, K, coef, ldb, frag, B, colinv);
       break;
     case QMCKL_AMD_SPLIT_FP32:
-      hipLaunchKernelGGL(split_coef_kernel<QMCKL_AMD_SPLIT_FP32>, dim3(g), dim3(kThreads), 0, s, M, K, coef, ldb, B, colinv);
+      hipLaunchKernelGGL(split_coef_kernel<QMCKL_AMD_SPLIT_FP32>, dim3(g), dim3(kThreads), 0, s, M, K, coef, ldb, frag, B, colinv);
       break;
     default: return hipErrorInvalidValue;
   }

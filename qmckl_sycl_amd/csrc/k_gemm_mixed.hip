@@ -273,27 +273,19 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
 #pragma unroll
         for (int r = 0; r < 4; ++r)
           a[t][r] = *reinterpret_cast<const uint4 *>(&sA[t][(r * 16 + li) * kLDS + ks * 32 + 8 * lk]);
+      // product by product over all 4 x CBW tiles: 4*CBW independent
+      // accumulators between dependent MFMAs (one wavefront per SIMD issues
+      // 16x16x32 at its full rate only with many chains in flight)
+      constexpr int NP = T == 3 ? 6 : 3;
+      constexpr int pa[6] = {0, 0, 1, 0, 2, 1}, pb[6] = {0, 1, 0, 2, 0, 1};  // (A term, C term)
 #pragma unroll
-      for (int cb = 0; cb < CBW; ++cb) {
-        const uint4 *b = bq[ks][cb];
-        f4 x[4];
+      for (int pr = 0; pr < NP; ++pr)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) x[r] = mfma<SP>(a[0][r], b[0], ks == 0 ? f4{0.f, 0.f, 0.f, 0.f} : d[r][cb]);
+        for (int cb = 0; cb < CBW; ++cb)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) x[r] = mfma<SP>(a[0][r], b[1], x[r]);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) x[r] = mfma<SP>(a[1][r], b[0], x[r]);
-        if constexpr (T == 3) {
-#pragma unroll
-          for (int r = 0; r < 4; ++r) x[r] = mfma<SP>(a[0][r], b[2], x[r]);
-#pragma unroll
-          for (int r = 0; r < 4; ++r) x[r] = mfma<SP>(a[2][r], b[0], x[r]);
-#pragma unroll
-          for (int r = 0; r < 4; ++r) x[r] = mfma<SP>(a[1][r], b[1], x[r]);
-        }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) d[r][cb] = x[r];
-      }
+          for (int r = 0; r < 4; ++r)
+            d[r][cb] = mfma<SP>(a[pa[pr]][r], bq[ks][cb][pb[pr]],
+                                pr == 0 && ks == 0 ? f4{0.f, 0.f, 0.f, 0.f} : d[r][cb]);
       if (more) load_b(k1, ks);  // this half-stage's C registers are free again
     }
 #pragma unroll

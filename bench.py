@@ -183,6 +183,44 @@ def measured_traffic(s, kernel):
     return None if k is None else k.get("bytes_per_launch")
 
 
+def bench_adjoint(steps, warmup):
+    """Differentiation form at C5 (water64, 32 walkers, weights w ~ N(0,1)
+    shaped like mo_vgl): the forward FP64 mo_vgl then the adjoint (dL/dR,
+    dL/dalpha, dL/dC of L = sum w mo_vgl), timed per step."""
+    import torch
+    from qmckl_sycl_amd import systems
+    from qmckl_sycl_amd.context import Context, DeviceArray, lib
+    L = lib()
+    s, w = systems.water64_diff(walk_num=32)
+    ctx = Context(0)
+    try:
+        ctx.set_system(s)
+        wd = DeviceArray.from_numpy(ctx, w)
+        dR = DeviceArray(ctx, (s.nucl_num, 3))
+        dA = DeviceArray(ctx, (s.prim_num,))
+        dC = DeviceArray(ctx, (s.mo_num, s.ao_num))
+        mo = DeviceArray(ctx, (s.point_num, 5, s.mo_num))
+        h = ctx.handle
+
+        def step():
+            ctx.check(L.qmckl_get_mo_basis_mo_vgl_inplace_device(h, mo.ptr, mo.size))
+            ctx.check(L.qmckl_get_mo_basis_mo_vgl_adjoint_device(h, wd.ptr, wd.size, dR.ptr, dA.ptr, dC.ptr))
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        sec = (time.perf_counter() - t0) / steps
+        res = {"workload": f"water64 at 32 walkers ({s.point_num} points), FP64 forward + adjoint",
+               "ms_per_step": sec * 1e3, "value": s.point_num / sec, "unit": "electron-points/s"}
+        log("adjoint", json.dumps(res))
+        return res
+    finally:
+        ctx.destroy()
+
+
 def roofline(kernels, s):
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
     kind, work = kernel_work(dom, s)
@@ -388,6 +426,7 @@ def main():
         out["speedup_vs_plain_fp64"] = head["value"] / results["plain"]["value"]
     if not args.no_batch:
         out["batch_48_ragged"] = bench_batch(["plain", args.form], max(2, args.steps // 3), 1)
+        out["adjoint_c5"] = bench_adjoint(max(2, args.steps // 3), 1)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(s, args.cpu_seconds)
     print(json.dumps(out), flush=True)

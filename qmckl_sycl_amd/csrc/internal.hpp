@@ -141,6 +141,8 @@ struct MoBasis {
   int64_t mo_num = 0;
   double *coefficient = nullptr;    // [M][A]
   double *coefficient_t = nullptr;  // [A][M]
+  double *coef_frag = nullptr;      // coefficient_t in FP64 MFMA fragment order
+  int64_t coef_frag_nk4 = 0;
   double *mo_vgl = nullptr;         // [P][5][M]
   int64_t mo_vgl_cap = 0;
   uint64_t mo_vgl_date = 0;
@@ -273,6 +275,15 @@ hipError_t launch_ao_from_shell_table(hipStream_t s, const DeviceBasis &b, int64
 hipError_t launch_mo_gemm_f64(hipStream_t s, int64_t rows, int64_t K, int64_t Ncol,
                               const double *A, int64_t lda, const double *B, int64_t ldb,
                               double *C, int64_t ldc, const uint8_t *flags = nullptr, int nkb = 0);
+// Coefficients in the MFMA fragment order of the FP64 forward contraction:
+// Bf holds (Ncol rounded up to 64 * f64_frag_cbw(Ncol)) x (K rounded up to
+// 16) doubles; *nk4 receives the K steps of 4 per column block.
+int f64_frag_cbw(int64_t Ncol);
+hipError_t launch_frag_b_f64(hipStream_t s, int64_t K, int64_t Ncol, const double *B, int64_t ldb,
+                             double *Bf, int64_t *nk4);
+hipError_t launch_mo_gemm_f64_frag(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const double *A,
+                                   int64_t lda, const double *Bf, int64_t nk4, double *C, int64_t ldc,
+                                   const uint8_t *flags = nullptr, int nkb = 0);
 // C[R][Ncol] = A^T B with A stored [K][lda] (rows of A^T contiguous along
 // K-major storage), split over K into `splits` partials (partial holds
 // splits*R*Ncol doubles) reduced in a fixed order.

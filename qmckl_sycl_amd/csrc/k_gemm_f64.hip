@@ -135,6 +135,145 @@ hipError_t dispatch(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const dou
   return launch_cbw<5, TRANSA>(s, R, K, Ncol, A, lda, B, ldb, C, ldc, flags, nkb, splits);
 }
 
+// The forward MO contraction with the coefficients in MFMA fragment order.
+// For v_mfma_f64_16x16x4_f64 lane l holds B[4*ks + l/16][16*cb + l%16] of
+// column block cb at K step ks; the prepared copy stores that fragment at
+//   Bf[(cb * nk4 + ks) * 64 + l],   nk4 = K padded to 16, over 4
+// (zeros past K and past the last MO), so a wavefront loads each fragment
+// as 512 contiguous bytes straight from L2 into registers.  Only the AO
+// slab goes through LDS (all four wavefronts use its rows).  A workgroup
+// owns 64 rows x 64*CBW columns (wavefront w: column blocks w, w+4, ...);
+// K advances in stages of 16 with the next stage's AO slab and coefficient
+// fragments requested while the current stage computes; stages whose 32-AO
+// block is screened on all 64 rows are dropped from the stage list up front
+// (adding exact zeros changes nothing, so results equal the dense kernel's
+// bit for bit).
+constexpr int kMaxStages16 = 2 * kMaxFlagBlocks;
+
+__global__ void __launch_bounds__(kThreads)
+frag_b_f64_kernel(int64_t K, int64_t Ncol, const double *__restrict__ B, int64_t ldb, int64_t nk4,
+                  int64_t total, double *__restrict__ Bf) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t frag = i >> 6;
+    const int l = (int)(i & 63);
+    const int64_t cb = frag / nk4, ks = frag - cb * nk4;
+    const int64_t k = 4 * ks + l / 16, col = 16 * cb + l % 16;
+    Bf[i] = (k < K && col < Ncol) ? B[k * ldb + col] : 0.0;
+  }
+}
+
+template <int CBW>
+__global__ void __launch_bounds__(kThreads, 2)
+gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restrict__ A, int64_t lda,
+                     const double *__restrict__ Bf, int64_t nk4, double *__restrict__ C, int64_t ldc,
+                     const uint8_t *__restrict__ flags, int nkb) {
+  __shared__ double As[kBK][kBM + 1];
+  __shared__ int s_list[kMaxStages16];
+  __shared__ int s_count;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int64_t row0 = (int64_t)blockIdx.x * kBM;
+  const int64_t cb0 = (int64_t)blockIdx.y * 4 * CBW + wave;  // this wavefront's first column block
+  const int nst = (int)((K + kBK - 1) / kBK);
+
+  int count = nst;
+  if (flags != nullptr) {
+    if (wave == 0) {
+      const uint8_t *f = flags + (int64_t)blockIdx.x * nkb;
+      int base = 0;
+      for (int s0 = 0; s0 < nst; s0 += 64) {
+        const int st = s0 + lane;
+        const bool live = st < nst && f[st >> 1] != 0;
+        const uint64_t m = __ballot(live);
+        if (live) s_list[base + __popcll(m & ((1ull << lane) - 1ull))] = st;
+        base += __popcll(m);
+      }
+      if (lane == 0) s_count = base;
+    }
+    __syncthreads();
+    count = s_count;
+  }
+
+  // A slot: row ar, 4 consecutive K at aq (rows past R read row R-1, K past
+  // the end reads zero)
+  const int ar = tid >> 2, aq = (tid & 3) * 4;
+  const double *Ap = A + (row0 + ar < R ? row0 + ar : R - 1) * lda + aq;
+  const double *Bp = Bf + cb0 * nk4 * 64 + lane;
+  const int64_t cb_stride = 4 * nk4 * 64;  // the wavefront's next column block
+
+  double ra[4];
+  double b[4][CBW];
+  auto stage_k0 = [&](int s) { return (int64_t)(flags != nullptr ? s_list[s] : s) * kBK; };
+  auto load_a = [&](int64_t k0) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) ra[j] = k0 + aq + j < K ? Ap[k0 + j] : 0.0;
+  };
+  auto load_b = [&](int64_t k0, int ks) {
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb) b[ks][cb] = Bp[cb * cb_stride + (k0 / 4 + ks) * 64];
+  };
+
+  d4 acc[4][CBW];
+#pragma unroll
+  for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb) acc[rb][cb] = d4{0.0, 0.0, 0.0, 0.0};
+
+  if (count > 0) {
+    const int64_t k0 = stage_k0(0);
+    load_a(k0);
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) load_b(k0, ks);
+  }
+  for (int s = 0; s < count; ++s) {
+    __syncthreads();  // every wavefront is done reading the previous AO slab
+#pragma unroll
+    for (int j = 0; j < 4; ++j) As[aq + j][ar] = ra[j];
+    __syncthreads();
+    const bool more = s + 1 < count;
+    const int64_t k1 = more ? stage_k0(s + 1) : 0;
+    if (more) load_a(k1);
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      double a[4];
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb) a[rb] = As[4 * ks + lk][rb * 16 + li];
+#pragma unroll
+      for (int cb = 0; cb < CBW; ++cb)
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+          acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], b[ks][cb], acc[rb][cb], 0, 0, 0);
+      if (more) load_b(k1, ks);  // this K step's fragments are free again
+    }
+  }
+#pragma unroll
+  for (int cb = 0; cb < CBW; ++cb) {
+    const int64_t col = (cb0 + 4 * cb) * 16 + li;
+    if (col >= Ncol) continue;
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t row = row0 + rb * 16 + lk + 4 * j;
+        if (row < R) C[row * ldc + col] = acc[rb][cb][j];
+      }
+  }
+}
+
+template <int CBW>
+hipError_t launch_frag(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const double *A, int64_t lda,
+                       const double *Bf, int64_t nk4, double *C, int64_t ldc, const uint8_t *flags,
+                       int nkb) {
+  const int64_t gx = (R + kBM - 1) / kBM, gy = (Ncol + 64 * CBW - 1) / (64 * CBW);
+  if (gx > 0x7fffffff || gy > 65535) return hipErrorInvalidValue;
+  if (flags != nullptr && ((int64_t)nkb * 32 < K || nkb > kMaxFlagBlocks)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((gemm_f64_frag_kernel<CBW>), dim3((unsigned)gx, (unsigned)gy), dim3(kThreads), 0, s, R,
+                     K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb);
+  return hipGetLastError();
+}
+
 // out[i] = sum_z part[z][i], z ascending (fixed order: deterministic)
 __global__ void __launch_bounds__(kThreads)
 sum_splits_kernel(int64_t n, int splits, const double *__restrict__ part, double *__restrict__ out) {
@@ -172,6 +311,38 @@ hipError_t launch_gemm_tn_f64(hipStream_t s, int64_t R, int64_t K, int64_t Ncol,
   if (g > 65536) g = 65536;
   hipLaunchKernelGGL(sum_splits_kernel, dim3((unsigned)g), dim3(kThreads), 0, s, n, splits, partial, C);
   return hipGetLastError();
+}
+
+int f64_frag_cbw(int64_t Ncol) {
+  const int64_t nb = (Ncol + 15) / 16;
+  return nb <= 4 ? 1 : nb <= 8 ? 2 : nb <= 12 ? 3 : nb <= 16 ? 4 : 5;
+}
+
+hipError_t launch_frag_b_f64(hipStream_t s, int64_t K, int64_t Ncol, const double *B, int64_t ldb,
+                             double *Bf, int64_t *nk4) {
+  *nk4 = (K + kBK - 1) / kBK * (kBK / 4);
+  const int64_t npad = round_up(Ncol, 64 * f64_frag_cbw(Ncol));
+  const int64_t total = (npad / 16) * (*nk4) * 64;
+  if (total <= 0) return hipSuccess;
+  int64_t g = (total + kThreads - 1) / kThreads;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(frag_b_f64_kernel, dim3((unsigned)g), dim3(kThreads), 0, s, K, Ncol, B, ldb, *nk4, total,
+                     Bf);
+  return hipGetLastError();
+}
+
+hipError_t launch_mo_gemm_f64_frag(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const double *A,
+                                   int64_t lda, const double *Bf, int64_t nk4, double *C, int64_t ldc,
+                                   const uint8_t *flags, int nkb) {
+  if (R <= 0 || Ncol <= 0) return hipSuccess;
+  if (K <= 0 || nk4 * 4 < K) return hipErrorInvalidValue;
+  switch (f64_frag_cbw(Ncol)) {
+    case 1: return launch_frag<1>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb);
+    case 2: return launch_frag<2>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb);
+    case 3: return launch_frag<3>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb);
+    case 4: return launch_frag<4>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb);
+    default: return launch_frag<5>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb);
+  }
 }
 
 }  // namespace qmckl_amd

@@ -5,7 +5,8 @@
 // set_coefficient 526-583) and src/qmckl_mo_sycl.cpp (contraction 11-56,
 // finalize 136-190: coefficient_t[A][M] = coefficient[M][A]).  The
 // contraction mo_vgl[(p,c)][i] = sum_k ao_vgl[(p,c)][k] coefficient_t[k][i]
-// is one GEMM with 5P rows, run by the FP64 MFMA kernel.
+// is one GEMM with 5P rows, run by the FP64 MFMA kernel on a copy of the
+// coefficients in MFMA fragment order made at finalize.
 #include <algorithm>
 
 #include "internal.hpp"
@@ -103,8 +104,8 @@ qmckl_exit_code_device compute_mo_vgl(qmckl_context_device context, Context *c, 
   hipError_t e;
   {
     KernelTimer t(c, "mo_gemm_f64");
-    e = launch_mo_gemm_f64(c->stream, 5 * P, A, M, c->ao.ao_vgl, A, c->mo.coefficient_t, M, dst, M,
-                           flags, flags ? nkb : 0);
+    e = launch_mo_gemm_f64_frag(c->stream, 5 * P, A, M, c->ao.ao_vgl, A, c->mo.coef_frag,
+                                c->mo.coef_frag_nk4, dst, M, flags, flags ? nkb : 0);
     t.stop();
   }
   if (e == hipSuccess) e = sync(c);
@@ -123,7 +124,8 @@ qmckl_exit_code_device compute_mo_value(qmckl_context_device context, Context *c
     if (rc != QMCKL_SUCCESS_DEVICE)
       return fail(context, QMCKL_NOT_PROVIDED_DEVICE, "qmckl_ao_basis_ao_value_device", nullptr);
     KernelTimer t(c, "mo_value_gemm_f64");
-    e = launch_mo_gemm_f64(c->stream, P, A, M, c->ao.ao_value, A, c->mo.coefficient_t, M, dst, M);
+    e = launch_mo_gemm_f64_frag(c->stream, P, A, M, c->ao.ao_value, A, c->mo.coef_frag,
+                                c->mo.coef_frag_nk4, dst, M);
     t.stop();
   }
   if (e == hipSuccess) e = sync(c);
@@ -157,7 +159,13 @@ qmckl_exit_code_device qmckl_finalize_mo_basis_device(qmckl_context_device conte
   const int64_t A = c->ao.ao_num, M = c->mo.mo_num;
   if (!drealloc(c, reinterpret_cast<void **>(&c->mo.coefficient_t), sizeof(double) * A * M))
     return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
+  // the same coefficients in the fragment order of the FP64 contraction
+  const int64_t frag_len = round_up(M, 64 * f64_frag_cbw(M)) * round_up(A, 16);
+  if (!drealloc(c, reinterpret_cast<void **>(&c->mo.coef_frag), sizeof(double) * frag_len))
+    return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
   hipError_t e = launch_transpose(c->stream, M, A, c->mo.coefficient, c->mo.coefficient_t);
+  if (e == hipSuccess)
+    e = launch_frag_b_f64(c->stream, A, M, c->mo.coefficient_t, M, c->mo.coef_frag, &c->mo.coef_frag_nk4);
   if (e == hipSuccess) e = sync(c);
   if (e != hipSuccess) return hip_check(context, e, fn);
   c->mo.mo_vgl_date = c->mo.mo_value_date = 0;

@@ -95,7 +95,7 @@ def run_form(ctx, s, form, steps, warmup):
     t1 = time.perf_counter()
     ctx.check(L.qmckl_set_kernel_timing_device(h, False))
     kernels = {}
-    for name in ("en_distance", "ao_vgl", "ao_vgl_f32", GEMM_KERNEL[form]):
+    for name in ("en_distance", "point_order", "ao_vgl", "ao_vgl_f32", GEMM_KERNEL[form]):
         n = ctypes.c_int64(0)
         ms = ctypes.c_double(0.0)
         L.qmckl_get_kernel_timing_device(h, name.encode(), ctypes.byref(n), ctypes.byref(ms))

@@ -140,6 +140,8 @@ std::string build_basis_tables(Context *c, DeviceBasis &b, int64_t N, int64_t S,
       d.ao_start = (int32_t)ao_shell.size();
       d.l = l;
       d.shell = (int32_t)s;
+      d.a_min = np > 0 ? exponent[p0] : 0.0;
+      for (int64_t j = p0; j < p0 + np; ++j) d.a_min = std::min(d.a_min, exponent[j]);
       const int32_t pos = (int32_t)shells.size();
       shells.push_back(d);
       for (int a = l; a >= 0; --a)

@@ -37,6 +37,7 @@ struct ShellDesc {
   int32_t ao_start;    // first AO
   int32_t l;           // angular momentum
   int32_t shell;       // index in the user's shell arrays
+  double a_min;        // smallest exponent: the shell is zero where a_min * r^2 > kCutoff
 };
 
 // Output of the AO kernel: values at out[(p*ncomp + c)*ld + k] and optional
@@ -133,6 +134,11 @@ struct AoBasis {
   int64_t ao_f32_rowmax_cap = 0;
   uint8_t *ao_f32_flags = nullptr;
   int64_t ao_f32_flags_cap = 0;
+  // spatial order of the points for the operand's tiles, and its workspace
+  int32_t *point_perm = nullptr;
+  int64_t point_perm_cap = 0;
+  void *order_ws = nullptr;
+  int64_t order_ws_cap = 0;
   int32_t uninitialized = (1 << 14) - 1;
   bool provided = false;
 };
@@ -251,6 +257,12 @@ hipError_t launch_aos_from_soa(hipStream_t s, int64_t n, const double *soa, doub
 hipError_t launch_transpose(hipStream_t s, int64_t rows, int64_t cols, const double *in,
                             double *out);
 // ee_distance [W][E][E] of the W walkers' E electrons, coord [3][W*E]
+// Spatial order of the points (cells of a grid over the nuclei's bounding
+// box, in Z order): perm[P] receives the point indices; workspace holds
+// point_order_workspace_bytes(P) bytes.
+int64_t point_order_workspace_bytes(int64_t P);
+hipError_t launch_point_order(hipStream_t s, int64_t P, const double *coord, int64_t N, const double *nucl,
+                              void *workspace, int32_t *perm);
 hipError_t launch_ee_distance(hipStream_t s, int64_t E, int64_t W, const double *coord, double *out);
 hipError_t launch_en_distance(hipStream_t s, int64_t P, int64_t N, const double *coord,
                               const double *nucl, double *out);
@@ -261,13 +273,16 @@ hipError_t launch_ao_vgl(hipStream_t s, const DeviceBasis &b, int64_t P, const d
                          int ncomp, uint8_t *flags = nullptr, int nkb = 0);
 // AO operand of the mixed contraction, FP32 in the blocked layout
 // buf[c][64-point tile][AO/4][point in tile][4] (AOs zero-padded to ld, a
-// multiple of 64), partial row maxima rowmax[group][c][point] over each of
-// `groups` AO groups (ao_rows_groups) and, if flags is not null, nonzero
-// flags [5 * tiles][nkb] with nkb = ld / 32.
+// multiple of 64), tile row i holding point perm[i] (null: point i),
+// partial row maxima rowmax[group][c][tile row] over each of `groups` AO
+// groups (ao_rows_groups) and, if flags is not null, nonzero flags
+// [5 * tiles][nkb] with nkb = ld / 32.  With flags, a 64-AO stage that the
+// screen zeroes for all 64 points of a tile is not written (its two flags
+// are zero, so the contraction never reads it).
 int ao_rows_groups(int64_t P, int64_t ld);
 hipError_t launch_ao_rows_f32(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
-                              const double *nucl, const double *ao_factor, int64_t ld, int groups,
-                              float *buf, float *rowmax, uint8_t *flags, int nkb);
+                              const int32_t *perm, const double *nucl, const double *ao_factor, int64_t ld,
+                              int groups, float *buf, float *rowmax, uint8_t *flags, int nkb);
 hipError_t launch_ao_from_shell_table(hipStream_t s, const DeviceBasis &b, int64_t P,
                                       const double *coord, const double *nucl,
                                       const double *ao_factor, const double *shell_vgl,
@@ -317,11 +332,12 @@ inline int64_t split_cols_padded(int form, int64_t M) {
 hipError_t launch_split_coef(hipStream_t s, int form, int64_t M, int64_t K, const double *coef,
                              int64_t ldb, uint16_t *B, float *colinv);
 // Mixed MO contraction of the blocked AO operand (launch_ao_rows_f32) of P
-// points with the prepared coefficients into mo_vgl C [P][5][Ncol] (ldc).
+// points with the prepared coefficients into mo_vgl C [P][5][Ncol] (ldc);
+// tile row i holds point perm[i] (null: point i).
 hipError_t launch_mo_gemm_mixed(hipStream_t s, int form, int64_t P, int64_t K, int64_t Ncol,
                                 const float *A, int64_t lda, const float *rowmax, int groups,
                                 const void *B, int64_t ldb, const float *colinv, const uint8_t *flags,
-                                int nkb, double *C, int64_t ldc);
+                                int nkb, const int32_t *perm, double *C, int64_t ldc);
 hipError_t launch_strided_copy(hipStream_t s, int64_t rows, int64_t cols, const double *in,
                                int64_t ld_in, double *out, int64_t ld_out);
 

@@ -15,15 +15,21 @@
 // pieces, 64 of which (one per point of the tile) form a contiguous
 // kilobyte for each group of 4 AOs.
 // AOs past ao_num up to ld (a multiple of 64) are written as zeros.
-// Also written: rowmax[group][c][point] = max |entry| of the row over the
+// Also written: rowmax[group][c][tile row] = max |entry| of the row over the
 // group's AOs (each lane owns its five rows, so no reduction is needed; the
 // contraction takes the max over groups), and nonzero flags per (row tile,
 // 32-AO block) gathered with wavefront ballots.
 //
+// The 64 points of tile t are perm[64t .. 64t+63] (the caller orders the
+// points so that a tile is spatially compact).  With flags, a 64-AO stage
+// whose shells are all screened for the tile's 64 points is neither
+// evaluated nor written; its flags are zero, so the contraction skips it.
+//
 // Work decomposition: a wavefront owns 64 points and one group of 64-AO
-// stages (enough groups per point tile for about four wavefronts per SIMD;
-// a shell that straddles a group boundary is evaluated by both groups, each
-// writing its own AOs) and walks those shells in AO order; the shell loop is
+// stages (enough groups per point tile for about four wavefronts per SIMD)
+// and walks each live stage's shells in AO order (a shell that straddles a
+// stage boundary is evaluated for each stage, each writing its own AOs);
+// the shell loop is
 // wavefront-uniform (shell data, nucleus and
 // primitives come through scalar loads; a primitive screened for all 64
 // points costs no exp), and for l <= 3 the shell's Cartesian components
@@ -58,10 +64,10 @@ struct RowWriter {
   float *ring;       // this wavefront's [5][4][64] LDS ring
   int lane;
 
-  int k_lo, k_hi;    // AOs this wavefront writes (multiples of 64)
+  int k_lo, k_hi;    // AOs of the current stage
 
   __device__ __attribute__((always_inline)) void push(const double (&v)[5]) {
-    if (k < k_lo || k >= k_hi) {  // part of a shell that another wavefront writes
+    if (k < k_lo || k >= k_hi) {  // part of a shell outside the current stage
       ++k;
       return;
     }
@@ -188,7 +194,7 @@ __device__ __attribute__((always_inline)) inline void shell_aos_any(RowWriter &w
 }
 
 __global__ void __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads), amdgpu_waves_per_eu(4)))
-ao_rows_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord,
+ao_rows_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord, const int32_t *__restrict__ perm,
                const double *__restrict__ nucl, const ShellDesc *__restrict__ shells,
                const int32_t *__restrict__ stage_shell, const double *__restrict__ ao_factor,
                const double *__restrict__ expo, const double *__restrict__ coef,
@@ -201,8 +207,9 @@ ao_rows_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord,
   const int64_t pt = wi / groups;
   const int g = (int)(wi - pt * groups);
   if (pt >= npt) return;  // wavefront-uniform
-  const int64_t p = pt * 64 + lane;
-  const bool inside = p < P;
+  const int64_t i = pt * 64 + lane;
+  const bool inside = i < P;
+  const int64_t p = inside ? (perm != nullptr ? (int64_t)perm[i] : i) : 0;
   const double px = inside ? coord[p] : 0.0;
   const double py = inside ? coord[P + p] : 0.0;
   const double pz = inside ? coord[2 * P + p] : 0.0;
@@ -214,8 +221,6 @@ ao_rows_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord,
 #pragma unroll
   for (int c = 0; c < 5; ++c) w.mx[c] = 0.0f;
   w.nz = 0;
-  w.k_lo = (int)(64 * st_lo);
-  w.k_hi = (int)(64 * st_hi);
   w.cstride = npt * (ld / 4) * 256;
   w.base = buf + pt * (ld / 4) * 256 + lane * 4;
   w.flags = flags != nullptr ? flags + pt * nkb : nullptr;
@@ -223,45 +228,72 @@ ao_rows_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord,
   w.lane = lane;
   w.ring = s_ring[wave];
 
-  const int64_t s_first = st_lo < nst_a ? stage_shell[st_lo] : S;
-  w.k = s_first < S ? shells[s_first].ao_start : w.k_lo;
-  for (int64_t s = s_first; s < S; ++s) {
-    const ShellDesc d = shells[s];
-    if (d.ao_start >= w.k_hi) break;
-    const double x = px - nucl[d.nucl];
-    const double y = py - nucl[N + d.nucl];
-    const double z = pz - nucl[2 * N + d.nucl];
-    const double r2 = fma(x, x, fma(y, y, z * z));
-    const bool live = inside && !(r2 > kCutoff * nucl_range[d.nucl]);
-    double R[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    const double *ex = expo + d.prim_start;
-    const double *cf = coef + d.prim_start;
-    for (int j = 0; j < d.prim_num; ++j) {
-      const double a = ex[j];
-      const double ar2 = a * r2;
-      if (live && !(ar2 > kCutoff)) {
-        const double v = cf[j] * exp(-ar2);
-        const double two_a = -2.0 * a * v;
-        R[0] += v;
-        R[1] += two_a * x;
-        R[2] += two_a * y;
-        R[3] += two_a * z;
-        R[4] += two_a * (3.0 - 2.0 * ar2);
+  // Stage by stage; a shell that straddles a stage boundary is evaluated
+  // for each stage it reaches (each writes its own AOs).
+  for (int64_t st = st_lo; st < st_hi; ++st) {
+    const int k_end = (int)(64 * (st + 1));
+    const int64_t s_first = st < nst_a ? stage_shell[st] : S;
+    if (flags != nullptr) {
+      // Is any AO of the stage nonzero for some point of the tile?  Same
+      // screen as the evaluation below: a shell is zero at r^2 when
+      // a * r^2 > cutoff for all of its exponents a (a_min * r^2 rounds no
+      // higher than a * r^2), or past the nucleus range.
+      bool any = false;
+      for (int64_t s = s_first; s < S; ++s) {
+        const ShellDesc d = shells[s];
+        if (d.ao_start >= k_end) break;
+        const double x = px - nucl[d.nucl];
+        const double y = py - nucl[N + d.nucl];
+        const double z = pz - nucl[2 * N + d.nucl];
+        const double r2 = fma(x, x, fma(y, y, z * z));
+        any |= inside && !(r2 > kCutoff * nucl_range[d.nucl]) && !(d.a_min * r2 > kCutoff);
+      }
+      if (__ballot(any) == 0) {  // all zero: not written, flagged empty
+        if (lane < 10) flags[pt * nkb + (lane >> 1) * w.fstride + 2 * st + (lane & 1)] = 0;
+        continue;
       }
     }
-    switch (d.l) {
-      case 0: shell_aos<0>(w, ao_factor, x, y, z, R); break;
-      case 1: shell_aos<1>(w, ao_factor, x, y, z, R); break;
-      case 2: shell_aos<2>(w, ao_factor, x, y, z, R); break;
-      case 3: shell_aos<3>(w, ao_factor, x, y, z, R); break;
-      default: shell_aos_any(w, ao_factor, x, y, z, R, d.l); break;
+    w.k_lo = (int)(64 * st);
+    w.k_hi = k_end;
+    w.k = s_first < S ? shells[s_first].ao_start : w.k_lo;
+    for (int64_t s = s_first; s < S; ++s) {
+      const ShellDesc d = shells[s];
+      if (d.ao_start >= k_end) break;
+      const double x = px - nucl[d.nucl];
+      const double y = py - nucl[N + d.nucl];
+      const double z = pz - nucl[2 * N + d.nucl];
+      const double r2 = fma(x, x, fma(y, y, z * z));
+      const bool live = inside && !(r2 > kCutoff * nucl_range[d.nucl]);
+      double R[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      const double *ex = expo + d.prim_start;
+      const double *cf = coef + d.prim_start;
+      for (int j = 0; j < d.prim_num; ++j) {
+        const double a = ex[j];
+        const double ar2 = a * r2;
+        if (live && !(ar2 > kCutoff)) {
+          const double v = cf[j] * exp(-ar2);
+          const double two_a = -2.0 * a * v;
+          R[0] += v;
+          R[1] += two_a * x;
+          R[2] += two_a * y;
+          R[3] += two_a * z;
+          R[4] += two_a * (3.0 - 2.0 * ar2);
+        }
+      }
+      switch (d.l) {
+        case 0: shell_aos<0>(w, ao_factor, x, y, z, R); break;
+        case 1: shell_aos<1>(w, ao_factor, x, y, z, R); break;
+        case 2: shell_aos<2>(w, ao_factor, x, y, z, R); break;
+        case 3: shell_aos<3>(w, ao_factor, x, y, z, R); break;
+        default: shell_aos_any(w, ao_factor, x, y, z, R, d.l); break;
+      }
     }
+    // zero AOs from ao_num up to the stage's end (the padding the
+    // contraction reads, and the last 32-block's flags)
+    if (w.k < w.k_lo) w.k = w.k_lo;
+    const double zero[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    while (w.k < w.k_hi) w.push(zero);
   }
-  // zero AOs from ao_num up to this wavefront's end (the padding the
-  // contraction reads, and the last 32-block's flags)
-  if (w.k < w.k_lo) w.k = w.k_lo;
-  const double zero[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  while (w.k < w.k_hi) w.push(zero);
   // partial row maxima of this AO group, [groups][5][npt*64]
   float *rm = rowmax + (int64_t)g * 5 * npt * 64 + pt * 64 + lane;
 #pragma unroll
@@ -284,8 +316,8 @@ int ao_rows_groups(int64_t P, int64_t ld) {
 }
 
 hipError_t launch_ao_rows_f32(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
-                              const double *nucl, const double *ao_factor, int64_t ld, int groups,
-                              float *buf, float *rowmax, uint8_t *flags, int nkb) {
+                              const int32_t *perm, const double *nucl, const double *ao_factor, int64_t ld,
+                              int groups, float *buf, float *rowmax, uint8_t *flags, int nkb) {
   if (P <= 0) return hipSuccess;
   if (b.max_l > kMaxL || ld % 64 != 0 || ld < b.ao_num || groups < 1 || groups > ld / 64 ||
       b.stage_shell == nullptr)
@@ -296,7 +328,7 @@ hipError_t launch_ao_rows_f32(hipStream_t s, const DeviceBasis &b, int64_t P, co
   const int64_t blocks = (waves + kThreads / 64 - 1) / (kThreads / 64);
   if (blocks > 0x7fffffff || ld > 0x7fffffff) return hipErrorInvalidValue;
   hipLaunchKernelGGL(ao_rows_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, P, (int)b.nucl_num,
-                     b.shell_num, coord, nucl, b.shells, b.stage_shell, ao_factor, b.expo, b.coef_norm,
+                     b.shell_num, coord, perm, nucl, b.shells, b.stage_shell, ao_factor, b.expo, b.coef_norm,
                      b.nucl_range, b.ao_num, ld, npt, groups, buf, rowmax, flags, nkb);
   return hipGetLastError();
 }

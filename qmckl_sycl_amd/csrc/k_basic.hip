@@ -135,6 +135,113 @@ __global__ void __launch_bounds__(kThreads) ee_distance_kernel(int64_t E, int64_
   }
 }
 
+// Spatial order of the points for the tiled AO/MO kernels: points are
+// binned into a 16 x 16 x 16 grid of cells over the bounding box of the
+// nuclei (points outside fall into the border cells), the cells are taken
+// in Z (Morton) order, and perm[i] is the i-th point in that order.  64
+// consecutive entries of perm then lie close together, so whole 64-AO
+// stages of a tile are screened out far more often than for 64 points
+// spread over a walker.  Within a cell the order is that of atomic slot
+// allocation; it does not matter, since every point's values are computed
+// independently of the other points of its tile.
+// Workspace: bbox[6] doubles (64 bytes), counts[kCells] int32, cell[P] int32.
+constexpr int kCellBits = 4;
+constexpr int kCells = 1 << (3 * kCellBits);
+constexpr int kScanThreads = 1024;
+
+__global__ void __launch_bounds__(kThreads) order_setup_kernel(int64_t N, const double *__restrict__ nucl,
+                                                               double *__restrict__ bbox,
+                                                               int32_t *__restrict__ counts) {
+  __shared__ double red[6][kThreads];
+  const int t = threadIdx.x;
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t n = t; n < N; n += kThreads)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = fmin(lo[d], nucl[d * N + n]);
+      hi[d] = fmax(hi[d], nucl[d * N + n]);
+    }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    red[d][t] = lo[d];
+    red[3 + d][t] = hi[d];
+  }
+  __syncthreads();
+  for (int h = kThreads / 2; h > 0; h >>= 1) {
+    if (t < h)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        red[d][t] = fmin(red[d][t], red[d][t + h]);
+        red[3 + d][t] = fmax(red[3 + d][t], red[3 + d][t + h]);
+      }
+    __syncthreads();
+  }
+  if (t < 6) bbox[t] = red[t][0];
+  for (int i = t; i < kCells; i += kThreads) counts[i] = 0;
+}
+
+__device__ inline int spread_bits(int v) {  // bit b of v -> bit 3b
+  int r = 0;
+#pragma unroll
+  for (int b = 0; b < kCellBits; ++b) r |= ((v >> b) & 1) << (3 * b);
+  return r;
+}
+
+__global__ void __launch_bounds__(kThreads) order_cells_kernel(int64_t P, const double *__restrict__ coord,
+                                                               const double *__restrict__ bbox,
+                                                               int32_t *__restrict__ counts,
+                                                               int32_t *__restrict__ cell) {
+  const int64_t p = blockIdx.x * (int64_t)kThreads + threadIdx.x;
+  if (p >= P) return;
+  int code = 0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double lo = bbox[d], w = bbox[3 + d] - lo;
+    const double g = (w > 0.0 ? (coord[d * P + p] - lo) / w : 0.5) * (1 << kCellBits);
+    // out-of-box values go to the border cells, NaN to cell 0
+    const int q = g >= (double)((1 << kCellBits) - 1) ? (1 << kCellBits) - 1 : (g > 0.0 ? (int)g : 0);
+    code |= spread_bits(q) << d;
+  }
+  cell[p] = code;
+  atomicAdd(&counts[code], 1);
+}
+
+// counts -> exclusive offsets, one workgroup of kScanThreads threads, each
+// owning kCells / kScanThreads consecutive cells
+__global__ void __launch_bounds__(kScanThreads) order_scan_kernel(int32_t *__restrict__ counts) {
+  constexpr int per = kCells / kScanThreads;
+  __shared__ int32_t part[kScanThreads];
+  const int t = threadIdx.x;
+  int32_t v[per], sum = 0;
+#pragma unroll
+  for (int i = 0; i < per; ++i) {
+    v[i] = counts[t * per + i];
+    sum += v[i];
+  }
+  part[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {  // inclusive scan of the partial sums
+    const int32_t add = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += add;
+    __syncthreads();
+  }
+  int32_t run = part[t] - sum;
+#pragma unroll
+  for (int i = 0; i < per; ++i) {
+    counts[t * per + i] = run;
+    run += v[i];
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) order_scatter_kernel(int64_t P, const int32_t *__restrict__ cell,
+                                                                 int32_t *__restrict__ offsets,
+                                                                 int32_t *__restrict__ perm) {
+  const int64_t p = blockIdx.x * (int64_t)kThreads + threadIdx.x;
+  if (p >= P) return;
+  perm[atomicAdd(&offsets[cell[p]], 1)] = (int32_t)p;
+}
+
 }  // namespace
 
 hipError_t launch_soa_from_aos(hipStream_t s, int64_t n, const double *aos, double *soa) {
@@ -181,6 +288,23 @@ hipError_t launch_ee_distance(hipStream_t s, int64_t E, int64_t W, const double 
   if (blocks > 0x7fffffff) return hipErrorInvalidValue;
   hipLaunchKernelGGL(ee_distance_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, E, W, tiles, coord,
                      out);
+  return hipGetLastError();
+}
+
+int64_t point_order_workspace_bytes(int64_t P) { return 64 + 4 * (int64_t)kCells + 4 * P; }
+
+hipError_t launch_point_order(hipStream_t s, int64_t P, const double *coord, int64_t N, const double *nucl,
+                              void *workspace, int32_t *perm) {
+  if (P <= 0) return hipSuccess;
+  if (P > 0x7fffffff || N <= 0) return hipErrorInvalidValue;
+  double *bbox = static_cast<double *>(workspace);
+  int32_t *counts = reinterpret_cast<int32_t *>(static_cast<char *>(workspace) + 64);
+  int32_t *cell = counts + kCells;
+  const unsigned blocks = (unsigned)((P + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL(order_setup_kernel, dim3(1), dim3(kThreads), 0, s, N, nucl, bbox, counts);
+  hipLaunchKernelGGL(order_cells_kernel, dim3(blocks), dim3(kThreads), 0, s, P, coord, bbox, counts, cell);
+  hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(kScanThreads), 0, s, counts);
+  hipLaunchKernelGGL(order_scatter_kernel, dim3(blocks), dim3(kThreads), 0, s, P, cell, counts, perm);
   return hipGetLastError();
 }
 

@@ -97,31 +97,37 @@ __device__ inline f4 mfma(const uint4 &a, const uint4 &b, f4 c) {
 //   colinv [Ncol] inverse column scales
 //   flags  [5 * tiles][nkb] nonzero flags per (row tile, 32-wide K block),
 //          or null
+//   perm   the point of each tile row (row (c, i) holds point perm[i]),
+//          or null for the identity
 // Workgroup tile 64 rows x 64*CBW columns (all MOs for mo_num <= 320, so
-// each FP32 AO row is read from HBM once); wavefront w owns columns
-// [16*CBW*w, 16*CBW*(w+1)) for all 64 rows: 4 x CBW MFMA tiles, one
-// wavefront per SIMD.  Each row is scaled by the power of two that brings
+// each FP32 AO row is read from HBM once), eight wavefronts, two per SIMD:
+// wavefront w owns rows [32*(w/4), +32) and columns [16*CBW*(w%4), +16*CBW),
+// 2 x CBW MFMA tiles.  Each row is scaled by the power of two that brings
 // its largest |entry| into [2^13, 2^14).  K advances in stages of 64;
 // stages whose two flags are zero are dropped from the workgroup's stage
 // list up front.  Per stage:
 //   - the FP32 A slab (64 x 64, loaded into registers while the previous
 //     stage computed) is scaled, split and written to LDS, shared by the
-//     four wavefronts;
-//   - each wavefront reads its C terms straight from L2 into registers
-//     (no other wavefront uses them); in fragment order every such load is
-//     one contiguous kilobyte; the next stage's half-stage of C is requested
-//     as soon as the current one has been consumed;
-//   - 2 x 4 x CBW x (3 or 6) MFMAs accumulate in FP32 over the 64 K, and the
+//     eight wavefronts;
+//   - each wavefront reads its C terms straight from L2 into registers (the
+//     two wavefronts of a column quarter read the same bytes); in fragment
+//     order every such load is one contiguous kilobyte; the next stage's
+//     half-stage of C is requested as soon as the current one has been
+//     consumed;
+//   - 2 x 2 x CBW x (3 or 6) MFMAs accumulate in FP32 over the 64 K, and the
 //     stage's sums are added into FP64 accumulators (unscaled, exactly, at
 //     the store).
-// The row (c, point) is written to mo_vgl[point][c][:].
-// Measured on MI355X at 409,600 x 4,160 x 320: storing C in fragment order
-// instead of [MO][term][K] took the kernel from 5.6 to 4.2 ms (a load of 16
-// columns' 64-byte runs touched 16 cache lines); 64 x 96 tiles of
-// v_mfma_f32_32x32x16 (MOs padded to 384) came to 4.5 ms, and a
-// software-pipelined single-barrier loop and persistent workgroups were
-// measured slower.  The FP64 accumulators (160 registers per wavefront for
-// 64 x 80) keep the tile at 64 rows and one wavefront per SIMD.
+// The row (c, i) is written to mo_vgl[perm[i]][c][:].
+// Measured on MI355X at 409,600 x 4,160 x 320 (water64, 128 walkers):
+// storing C in fragment order instead of [MO][term][K] took the kernel from
+// 5.6 to 4.2 ms (a load of 16 columns' 64-byte runs touched 16 cache
+// lines); splitting each wavefront's 64 rows between two wavefronts per
+// SIMD (the FP64 accumulators then fit 256 registers) from 4.3 to 3.7 ms.
+// Measured slower: 64 x 96 tiles of v_mfma_f32_32x32x16 (MOs padded to
+// 384), software-pipelined single-barrier loops (with and without
+// interleaving hints), persistent workgroups, four-wavefront workgroups of
+// 64 x 160 with the column halves on alternate XCDs, and FP32 sums over
+// four stages before the FP64 addition (no faster; twice the error).
 constexpr int kRM = 64;           // rows per workgroup
 constexpr int kStage = 64;        // K per stage
 constexpr int kLDS = kStage + 8;  // LDS row length in halves (bank spread)
@@ -157,12 +163,15 @@ __device__ inline void split8(const float (&v)[8], uint4 (&out)[Terms<SP>::n]) {
 }
 
 template <int SP, int CBW>
-__global__ void __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads), amdgpu_waves_per_eu(1, 1)))
+__global__ void __attribute__((amdgpu_flat_work_group_size(512, 512), amdgpu_waves_per_eu(2, 2)))
 gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *__restrict__ A,
                   int64_t lda, const float *__restrict__ rowmax, int groups, const uint16_t *__restrict__ Bs,
                   int64_t ldb, const float *__restrict__ colinv, const uint8_t *__restrict__ flags, int nkb,
-                  double *__restrict__ C, int64_t ldc) {
+                  const int32_t *__restrict__ perm, double *__restrict__ C, int64_t ldc) {
   constexpr int T = Terms<SP>::n;
+  constexpr int NW = 8;            // wavefronts: 2 row halves x 4 column quarters
+  constexpr int RB = 2;            // 16-row blocks per wavefront
+  constexpr int KQ = kStage / NW;  // K per thread in the A slab
   __shared__ uint16_t sA[T][kRM * kLDS];
   __shared__ float s_inv[kRM];
   __shared__ int s_list[kMaxStages];
@@ -170,9 +179,10 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  const int64_t rt = blockIdx.x;                  // row tile: component c, points 64*pt ..
+  const int cq = wave & 3, rh = wave >> 2;  // column quarter, row half
+  const int64_t rt = blockIdx.x;            // row tile: component c, tile rows 64*pt ..
   const int64_t comp = rt / npt, p0 = (rt - comp * npt) * 64;
-  const int64_t col0 = (int64_t)blockIdx.y * 64 * CBW + wave * 16 * CBW;  // this wavefront's first column
+  const int64_t col0 = (int64_t)blockIdx.y * 64 * CBW + cq * 16 * CBW;  // this wavefront's first column
   const int nst = (int)((K + kStage - 1) / kStage);
 
   // stage list
@@ -194,12 +204,12 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
     count = s_count;
   }
 
-  // A slot: row ar = lane of the tile, 16 consecutive K at aq = 16*wave,
-  // read as four 16-byte pieces of the blocked operand (consecutive lanes,
+  // A slot: row ar = lane of the tile, KQ consecutive K at aq = KQ*wave,
+  // read as 16-byte pieces of the blocked operand (consecutive lanes,
   // consecutive pieces); 16 consecutive lanes take 16 consecutive rows, so
   // the LDS stores below do not conflict.  Each row has one power-of-two
   // scale (from its max |entry| over all K).
-  const int ar = lane, aq = wave * 16;
+  const int ar = lane, aq = wave * KQ;
   const float *Ap = A + rt * (lda / 4) * 256 + (aq / 4) * 256 + ar * 4;
   float rmax = 0.0f;  // the row's max |entry|: max over the AO groups' partial maxima
   for (int g = 0; g < groups; ++g) rmax = fmaxf(rmax, rowmax[(int64_t)g * 5 * npt * 64 + rt * 64 + ar]);
@@ -213,13 +223,13 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
   const uint16_t *Bp = Bs + (col0 / 16) * nkc * T * 512 + 8 * lane;
   const int64_t cb_stride = nkc * T * 512;
 
-  float4 ra[4];
+  float4 ra[KQ / 4];
   uint4 bq[2][CBW][T];
   auto stage_k0 = [&](int s) { return (int64_t)(flags != nullptr ? s_list[s] : s) * kStage; };
   auto load_a = [&](int64_t k0) {
     const float *src = Ap + (k0 / 4) * 256;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) ra[q] = *reinterpret_cast<const float4 *>(src + q * 256);
+    for (int q = 0; q < KQ / 4; ++q) ra[q] = *reinterpret_cast<const float4 *>(src + q * 256);
   };
   auto load_b = [&](int64_t k0, int ks) {
 #pragma unroll
@@ -229,10 +239,10 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
         bq[ks][cb][t] = *reinterpret_cast<const uint4 *>(Bp + cb * cb_stride + ((k0 / 32 + ks) * T + t) * 512);
   };
 
-  f4 d[4][CBW];
-  double acc[4][CBW][4];
+  f4 d[RB][CBW];
+  double acc[RB][CBW][4];
 #pragma unroll
-  for (int r = 0; r < 4; ++r)
+  for (int r = 0; r < RB; ++r)
 #pragma unroll
     for (int cb = 0; cb < CBW; ++cb)
 #pragma unroll
@@ -248,17 +258,15 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
   }
   for (int s = 0; s < count; ++s) {
     __syncthreads();  // every wavefront is done reading the previous A slab
-    {
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const float v[8] = {ra[2 * h].x * sc,     ra[2 * h].y * sc,     ra[2 * h].z * sc,
-                            ra[2 * h].w * sc,     ra[2 * h + 1].x * sc, ra[2 * h + 1].y * sc,
-                            ra[2 * h + 1].z * sc, ra[2 * h + 1].w * sc};
-        uint4 o[T];
-        split8<SP>(v, o);
+    for (int h = 0; h < KQ / 8; ++h) {
+      const float v[8] = {ra[2 * h].x * sc,     ra[2 * h].y * sc,     ra[2 * h].z * sc,
+                          ra[2 * h].w * sc,     ra[2 * h + 1].x * sc, ra[2 * h + 1].y * sc,
+                          ra[2 * h + 1].z * sc, ra[2 * h + 1].w * sc};
+      uint4 o[T];
+      split8<SP>(v, o);
 #pragma unroll
-        for (int t = 0; t < T; ++t) *reinterpret_cast<uint4 *>(&sA[t][ar * kLDS + aq + 8 * h]) = o[t];
-      }
+      for (int t = 0; t < T; ++t) *reinterpret_cast<uint4 *>(&sA[t][ar * kLDS + aq + 8 * h]) = o[t];
     }
     __syncthreads();
     const bool more = s + 1 < count;
@@ -267,15 +275,14 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
 
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks) {
-      uint4 a[T][4];
+      uint4 a[T][RB];
 #pragma unroll
       for (int t = 0; t < T; ++t)
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-          a[t][r] = *reinterpret_cast<const uint4 *>(&sA[t][(r * 16 + li) * kLDS + ks * 32 + 8 * lk]);
-      // product by product over all 4 x CBW tiles: 4*CBW independent
-      // accumulators between dependent MFMAs (one wavefront per SIMD issues
-      // 16x16x32 at its full rate only with many chains in flight)
+        for (int r = 0; r < RB; ++r)
+          a[t][r] = *reinterpret_cast<const uint4 *>(&sA[t][((rh * RB + r) * 16 + li) * kLDS + ks * 32 + 8 * lk]);
+      // product by product over all RB x CBW tiles: RB*CBW independent
+      // accumulators between dependent MFMAs
       constexpr int NP = T == 3 ? 6 : 3;
       constexpr int pa[6] = {0, 0, 1, 0, 2, 1}, pb[6] = {0, 1, 0, 2, 0, 1};  // (A term, C term)
 #pragma unroll
@@ -283,13 +290,13 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
 #pragma unroll
         for (int cb = 0; cb < CBW; ++cb)
 #pragma unroll
-          for (int r = 0; r < 4; ++r)
+          for (int r = 0; r < RB; ++r)
             d[r][cb] = mfma<SP>(a[pa[pr]][r], bq[ks][cb][pb[pr]],
                                 pr == 0 && ks == 0 ? f4{0.f, 0.f, 0.f, 0.f} : d[r][cb]);
       if (more) load_b(k1, ks);  // this half-stage's C registers are free again
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+    for (int r = 0; r < RB; ++r)
 #pragma unroll
       for (int cb = 0; cb < CBW; ++cb)
 #pragma unroll
@@ -301,12 +308,15 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
     if (col >= Ncol) continue;
     const double ci = (double)colinv[col];
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+    for (int r = 0; r < RB; ++r)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int row = r * 16 + 4 * lk + j;
-        const int64_t p = p0 + row;
-        if (p < P) C[(p * 5 + comp) * ldc + col] = acc[r][cb][j] * ((double)s_inv[row] * ci);
+        const int row = (rh * RB + r) * 16 + 4 * lk + j;
+        const int64_t i = p0 + row;
+        if (i < P) {
+          const int64_t p = perm != nullptr ? (int64_t)perm[i] : i;
+          C[(p * 5 + comp) * ldc + col] = acc[r][cb][j] * ((double)s_inv[row] * ci);
+        }
       }
   }
 }
@@ -318,7 +328,7 @@ template <int CBW>
 __global__ void __launch_bounds__(kThreads)
 gemm_f32_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *__restrict__ A,
                 int64_t lda, const float *__restrict__ Bf, int64_t ldb, const uint8_t *__restrict__ flags,
-                int nkb, int flush, double *__restrict__ C, int64_t ldc) {
+                int nkb, int flush, const int32_t *__restrict__ perm, double *__restrict__ C, int64_t ldc) {
   constexpr int BN = 32 * CBW;
   constexpr int LDF = kKB + 4;  // LDS row length in floats
   __shared__ float sA[kBM * LDF];
@@ -343,9 +353,25 @@ gemm_f32_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *__
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[rb][cb][j] = 0.0;
     }
-  int pending = 0;
+  // d holds the FP32 sums of the aligned group of `flush` blocks kb / flush
+  // (groups fixed by K, not by which blocks are skipped, so a row's result
+  // does not depend on the other rows of its tile)
+  int64_t group = -1;
   for (int64_t kb = 0; kb < nblk; ++kb) {
     if (flags != nullptr && flags[rt * nkb + kb] == 0) continue;
+    if (kb / flush != group) {
+      if (group >= 0) {
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+          for (int cb = 0; cb < CBW; ++cb) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[rb][cb][j] += (double)d[rb][cb][j];
+            d[rb][cb] = f4{0.f, 0.f, 0.f, 0.f};
+          }
+      }
+      group = kb / flush;
+    }
     const int64_t k0 = kb * kKB;
     {
       const float *src = Ap + ((k0 + seg) / 4) * 256;
@@ -377,17 +403,6 @@ gemm_f32_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *__
       }
     }
     __syncthreads();
-    if (++pending >= flush) {
-#pragma unroll
-      for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-        for (int cb = 0; cb < CBW; ++cb) {
-#pragma unroll
-          for (int j = 0; j < 4; ++j) acc[rb][cb][j] += (double)d[rb][cb][j];
-          d[rb][cb] = f4{0.f, 0.f, 0.f, 0.f};
-        }
-      pending = 0;
-    }
   }
 #pragma unroll
   for (int rb = 0; rb < 2; ++rb)
@@ -397,8 +412,11 @@ gemm_f32_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *__
       if (col >= Ncol) continue;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int64_t p = p0 + wr * 32 + rb * 16 + 4 * lk + j;
-        if (p < P) C[(p * 5 + comp) * ldc + col] = acc[rb][cb][j] + (double)d[rb][cb][j];
+        const int64_t i = p0 + wr * 32 + rb * 16 + 4 * lk + j;
+        if (i < P) {
+          const int64_t p = perm != nullptr ? (int64_t)perm[i] : i;
+          C[(p * 5 + comp) * ldc + col] = acc[rb][cb][j] + (double)d[rb][cb][j];
+        }
       }
     }
 }
@@ -446,49 +464,51 @@ split_coef_kernel(int64_t M, int64_t K, const double *__restrict__ coef, int64_t
 
 template <int CBW>
 hipError_t launch_f32(hipStream_t s, int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *A,
-                      int64_t lda, const void *B, int64_t ldb, const uint8_t *flags, int nkb, double *C,
-                      int64_t ldc) {
+                      int64_t lda, const void *B, int64_t ldb, const uint8_t *flags, int nkb,
+                      const int32_t *perm, double *C, int64_t ldc) {
   constexpr int BN = 32 * CBW;
   const int64_t gx = 5 * npt, gy = (Ncol + BN - 1) / BN;
   if (gx > 0x7fffffff || gy > 65535) return hipErrorInvalidValue;
   hipLaunchKernelGGL(gemm_f32_kernel<CBW>, dim3((unsigned)gx, (unsigned)gy), dim3(kThreads), 0, s, P, npt,
-                     K, Ncol, A, lda, (const float *)B, ldb, flags, nkb, 2, C, ldc);
+                     K, Ncol, A, lda, (const float *)B, ldb, flags, nkb, 2, perm, C, ldc);
   return hipGetLastError();
 }
 
 template <int SP, int CBW>
 hipError_t launch_split(hipStream_t s, int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *A,
                         int64_t lda, const float *rowmax, int groups, const void *B, int64_t ldb,
-                        const float *colinv, const uint8_t *flags, int nkb, double *C, int64_t ldc) {
+                        const float *colinv, const uint8_t *flags, int nkb, const int32_t *perm, double *C,
+                        int64_t ldc) {
   const int64_t gx = 5 * npt, gy = (Ncol + 64 * CBW - 1) / (64 * CBW);
   if (gx > 0x7fffffff || gy > 65535 || (lda % kStage) != 0 || (ldb % kStage) != 0)
     return hipErrorInvalidValue;
   if (flags != nullptr && (nkb > kMaxFlagBlocks || (int64_t)nkb * 32 < K)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((gemm_split_kernel<SP, CBW>), dim3((unsigned)gx, (unsigned)gy), dim3(kThreads), 0, s,
-                     P, npt, K, Ncol, A, lda, rowmax, groups, (const uint16_t *)B, ldb, colinv, flags, nkb,
-                     C, ldc);
+  hipLaunchKernelGGL((gemm_split_kernel<SP, CBW>), dim3((unsigned)gx, (unsigned)gy), dim3(512), 0, s, P, npt,
+                     K, Ncol, A, lda, rowmax, groups, (const uint16_t *)B, ldb, colinv, flags, nkb, perm, C,
+                     ldc);
   return hipGetLastError();
 }
 
 template <int SP>
 hipError_t launch_form(hipStream_t s, int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *A,
                        int64_t lda, const float *rowmax, int groups, const void *B, int64_t ldb,
-                       const float *colinv, const uint8_t *flags, int nkb, double *C, int64_t ldc) {
+                       const float *colinv, const uint8_t *flags, int nkb, const int32_t *perm, double *C,
+                       int64_t ldc) {
   if constexpr (SP == QMCKL_AMD_SPLIT_FP32) {
     const int64_t nb = (Ncol + 31) / 32;  // 32-column groups
-    if (nb <= 1) return launch_f32<1>(s, P, npt, K, Ncol, A, lda, B, ldb, flags, nkb, C, ldc);
-    if (nb <= 2) return launch_f32<2>(s, P, npt, K, Ncol, A, lda, B, ldb, flags, nkb, C, ldc);
-    if (nb <= 3) return launch_f32<3>(s, P, npt, K, Ncol, A, lda, B, ldb, flags, nkb, C, ldc);
-    if (nb <= 4) return launch_f32<4>(s, P, npt, K, Ncol, A, lda, B, ldb, flags, nkb, C, ldc);
-    return launch_f32<5>(s, P, npt, K, Ncol, A, lda, B, ldb, flags, nkb, C, ldc);
+    if (nb <= 1) return launch_f32<1>(s, P, npt, K, Ncol, A, lda, B, ldb, flags, nkb, perm, C, ldc);
+    if (nb <= 2) return launch_f32<2>(s, P, npt, K, Ncol, A, lda, B, ldb, flags, nkb, perm, C, ldc);
+    if (nb <= 3) return launch_f32<3>(s, P, npt, K, Ncol, A, lda, B, ldb, flags, nkb, perm, C, ldc);
+    if (nb <= 4) return launch_f32<4>(s, P, npt, K, Ncol, A, lda, B, ldb, flags, nkb, perm, C, ldc);
+    return launch_f32<5>(s, P, npt, K, Ncol, A, lda, B, ldb, flags, nkb, perm, C, ldc);
   } else {
     const int cbw = split_wg_cols(SP, Ncol) / 64;
-    if (cbw == 1) return launch_split<SP, 1>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, C, ldc);
-    if (cbw == 2) return launch_split<SP, 2>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, C, ldc);
-    if (cbw == 3) return launch_split<SP, 3>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, C, ldc);
+    if (cbw == 1) return launch_split<SP, 1>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, perm, C, ldc);
+    if (cbw == 2) return launch_split<SP, 2>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, perm, C, ldc);
+    if (cbw == 3) return launch_split<SP, 3>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, perm, C, ldc);
     if constexpr (Terms<SP>::n == 2) {
-      if (cbw == 4) return launch_split<SP, 4>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, C, ldc);
-      if (cbw == 5) return launch_split<SP, 5>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, C, ldc);
+      if (cbw == 4) return launch_split<SP, 4>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, perm, C, ldc);
+      if (cbw == 5) return launch_split<SP, 5>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, perm, C, ldc);
     }
     return hipErrorInvalidValue;
   }
@@ -533,20 +553,20 @@ hipError_t launch_split_coef(hipStream_t s, int form, int64_t M, int64_t K, cons
 hipError_t launch_mo_gemm_mixed(hipStream_t s, int form, int64_t P, int64_t K, int64_t Ncol,
                                 const float *A, int64_t lda, const float *rowmax, int groups,
                                 const void *B, int64_t ldb, const float *colinv, const uint8_t *flags,
-                                int nkb, double *C, int64_t ldc) {
+                                int nkb, const int32_t *perm, double *C, int64_t ldc) {
   if (P <= 0 || Ncol <= 0) return hipSuccess;
   if (K <= 0 || (lda % 64) != 0 || (ldb % 64) != 0 || lda < K || ldb < K || groups < 1)
     return hipErrorInvalidValue;
   const int64_t npt = (P + 63) / 64;
   switch (form) {
     case QMCKL_AMD_SPLIT_FP16X2:
-      return launch_form<QMCKL_AMD_SPLIT_FP16X2>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, C, ldc);
+      return launch_form<QMCKL_AMD_SPLIT_FP16X2>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, perm, C, ldc);
     case QMCKL_AMD_SPLIT_BF16X2:
-      return launch_form<QMCKL_AMD_SPLIT_BF16X2>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, C, ldc);
+      return launch_form<QMCKL_AMD_SPLIT_BF16X2>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, perm, C, ldc);
     case QMCKL_AMD_SPLIT_BF16X3:
-      return launch_form<QMCKL_AMD_SPLIT_BF16X3>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, C, ldc);
+      return launch_form<QMCKL_AMD_SPLIT_BF16X3>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, perm, C, ldc);
     case QMCKL_AMD_SPLIT_FP32:
-      return launch_form<QMCKL_AMD_SPLIT_FP32>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, C, ldc);
+      return launch_form<QMCKL_AMD_SPLIT_FP32>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, perm, C, ldc);
     default:
       return hipErrorInvalidValue;
   }

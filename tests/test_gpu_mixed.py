@@ -97,6 +97,24 @@ def test_water64_forms_sampled(oracle_lib):
         assert max(e.values()) <= BAR.get(form, 1e-12), (form, e)
 
 
+@pytest.mark.parametrize("form", list(FORMS))
+def test_point_order_invariance(form):
+    """Each point's MOs do not depend on which points share its tile (the
+    library orders points spatially into tiles and skips screened stages
+    per tile): the same points given in another order give bit-identical
+    rows."""
+    s = S.water64(walk_num=8)
+    P = s.point_num
+    perm = np.random.default_rng(7).permutation(P)
+    X = np.asarray(s.elec_coord).reshape(P, 3)
+    with Context(0) as c:
+        c.set_system(s)
+        a = run_form(c, form, P)
+        c.set_electron_coord(X[perm].reshape(np.shape(s.elec_coord)))
+        b = run_form(c, form, P)
+    assert np.array_equal(b, a[perm])
+
+
 def test_switching_forms_recomputes(chb):
     with Context(0) as c:
         c.set_system(chb)

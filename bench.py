@@ -163,7 +163,7 @@ def bench_batch(forms, steps, warmup, scale=1.0):
 
 # device kernel name (as the profiler lists it) of each timed kernel
 KERNEL_SYMBOL = {"en_distance": "en_distance_kernel", "ao_vgl": "ao_kernel<5, false, double>",
-                 "ao_vgl_f32": "ao_rows_kernel", "mo_gemm_f64": "gemm_f64_kernel<",
+                 "ao_vgl_f32": "ao_rows_kernel", "mo_gemm_f64": "gemm_f64_frag_kernel<",
                  "mo_gemm_fp32": "gemm_f32_kernel<", "mo_gemm_fp16x2": "gemm_split_kernel<2,",
                  "mo_gemm_bf16x2": "gemm_split_kernel<3,", "mo_gemm_bf16x3": "gemm_split_kernel<4,"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_{workload}.json")

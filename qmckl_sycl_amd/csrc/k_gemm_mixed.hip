@@ -99,16 +99,17 @@ __device__ inline f4 mfma(const uint4 &a, const uint4 &b, f4 c) {
 //          or null
 //   perm   the point of each tile row (row (c, i) holds point perm[i]),
 //          or null for the identity
-// Workgroup tile 64 rows x 64*CBW columns (all MOs for mo_num <= 320, so
-// each FP32 AO row is read from HBM once), eight wavefronts, two per SIMD:
-// wavefront w owns rows [32*(w/4), +32) and columns [16*CBW*(w%4), +16*CBW),
-// 2 x CBW MFMA tiles.  Each row is scaled by the power of two that brings
+// Workgroup tile WR rows (one half of a 64-row tile, or all of it) x
+// 64*CBW columns (all MOs for mo_num <= 320, so each FP32 AO row is read
+// from HBM once), WR/8 wavefronts at two per SIMD: wavefront w owns rows
+// [32*(w/4), +32) and columns [16*CBW*(w%4), +16*CBW), 2 x CBW MFMA tiles.
+// Each row is scaled by the power of two that brings
 // its largest |entry| into [2^13, 2^14).  K advances in stages of 64;
 // stages whose two flags are zero are dropped from the workgroup's stage
 // list up front.  Per stage:
-//   - the FP32 A slab (64 x 64, loaded into registers while the previous
+//   - the FP32 A slab (WR x 64, loaded into registers while the previous
 //     stage computed) is scaled, split and written to LDS, shared by the
-//     eight wavefronts;
+//     workgroup's wavefronts;
 //   - each wavefront reads its C terms straight from L2 into registers (the
 //     two wavefronts of a column quarter read the same bytes); in fragment
 //     order every such load is one contiguous kilobyte; the next stage's
@@ -122,7 +123,9 @@ __device__ inline f4 mfma(const uint4 &a, const uint4 &b, f4 c) {
 // storing C in fragment order instead of [MO][term][K] took the kernel from
 // 5.6 to 4.2 ms (a load of 16 columns' 64-byte runs touched 16 cache
 // lines); splitting each wavefront's 64 rows between two wavefronts per
-// SIMD (the FP64 accumulators then fit 256 registers) from 4.3 to 3.7 ms.
+// SIMD (the FP64 accumulators then fit 256 registers) from 4.3 to 3.7 ms;
+// WR = 32 (two independent 4-wavefront workgroups per CU) equals WR = 64
+// for FP16X2 and is 2.5% faster for BF16X3.
 // Measured slower: 64 x 96 tiles of v_mfma_f32_32x32x16 (MOs padded to
 // 384), software-pipelined single-barrier loops (with and without
 // interleaving hints), persistent workgroups, four-wavefront workgroups of
@@ -162,26 +165,27 @@ __device__ inline void split8(const float (&v)[8], uint4 (&out)[Terms<SP>::n]) {
   }
 }
 
-template <int SP, int CBW>
-__global__ void __attribute__((amdgpu_flat_work_group_size(512, 512), amdgpu_waves_per_eu(2, 2)))
+template <int SP, int CBW, int WR>
+__global__ void __attribute__((amdgpu_flat_work_group_size(WR * 8, WR * 8), amdgpu_waves_per_eu(2, 2)))
 gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *__restrict__ A,
                   int64_t lda, const float *__restrict__ rowmax, int groups, const uint16_t *__restrict__ Bs,
                   int64_t ldb, const float *__restrict__ colinv, const uint8_t *__restrict__ flags, int nkb,
                   const int32_t *__restrict__ perm, double *__restrict__ C, int64_t ldc) {
   constexpr int T = Terms<SP>::n;
-  constexpr int NW = 8;            // wavefronts: 2 row halves x 4 column quarters
+  constexpr int NW = WR / 8;       // wavefronts: WR/32 row halves x 4 column quarters
   constexpr int RB = 2;            // 16-row blocks per wavefront
-  constexpr int KQ = kStage / NW;  // K per thread in the A slab
-  __shared__ uint16_t sA[T][kRM * kLDS];
-  __shared__ float s_inv[kRM];
+  constexpr int KQ = 8;            // K per thread in the A slab
+  __shared__ uint16_t sA[T][WR * kLDS];
+  __shared__ float s_inv[WR];
   __shared__ int s_list[kMaxStages];
   __shared__ int s_count;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const int cq = wave & 3, rh = wave >> 2;  // column quarter, row half
-  const int64_t rt = blockIdx.x;            // row tile: component c, tile rows 64*pt ..
-  const int64_t comp = rt / npt, p0 = (rt - comp * npt) * 64;
+  const int64_t rt = blockIdx.x / (64 / WR);  // row tile: component c, tile rows 64*pt ..
+  const int r0 = (int)(blockIdx.x % (64 / WR)) * WR;  // this workgroup's first row of the tile
+  const int64_t comp = rt / npt, p0 = (rt - comp * npt) * 64 + r0;
   const int64_t col0 = (int64_t)blockIdx.y * 64 * CBW + cq * 16 * CBW;  // this wavefront's first column
   const int nst = (int)((K + kStage - 1) / kStage);
 
@@ -209,12 +213,12 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
   // consecutive pieces); 16 consecutive lanes take 16 consecutive rows, so
   // the LDS stores below do not conflict.  Each row has one power-of-two
   // scale (from its max |entry| over all K).
-  const int ar = lane, aq = wave * KQ;
-  const float *Ap = A + rt * (lda / 4) * 256 + (aq / 4) * 256 + ar * 4;
+  const int ar = tid % WR, aq = (tid / WR) * KQ;
+  const float *Ap = A + rt * (lda / 4) * 256 + (aq / 4) * 256 + (r0 + ar) * 4;
   float rmax = 0.0f;  // the row's max |entry|: max over the AO groups' partial maxima
-  for (int g = 0; g < groups; ++g) rmax = fmaxf(rmax, rowmax[(int64_t)g * 5 * npt * 64 + rt * 64 + ar]);
+  for (int g = 0; g < groups; ++g) rmax = fmaxf(rmax, rowmax[(int64_t)g * 5 * npt * 64 + rt * 64 + r0 + ar]);
   const float sc = pow2_scale(rmax);
-  if (wave == 0) s_inv[ar] = 1.0f / sc;
+  if (aq == 0) s_inv[ar] = 1.0f / sc;
   // C operand in fragment order (split_coef_kernel): the 16-byte piece of
   // lane l for column tile ct, 32-K chunk kc, term t is at
   // ((ct * nkc + kc) * T + t) * 512 + 8 * l halves, so each load of a
@@ -479,11 +483,12 @@ hipError_t launch_split(hipStream_t s, int64_t P, int64_t npt, int64_t K, int64_
                         int64_t lda, const float *rowmax, int groups, const void *B, int64_t ldb,
                         const float *colinv, const uint8_t *flags, int nkb, const int32_t *perm, double *C,
                         int64_t ldc) {
-  const int64_t gx = 5 * npt, gy = (Ncol + 64 * CBW - 1) / (64 * CBW);
+  constexpr int WR = 32;
+  const int64_t gx = 5 * npt * (64 / WR), gy = (Ncol + 64 * CBW - 1) / (64 * CBW);
   if (gx > 0x7fffffff || gy > 65535 || (lda % kStage) != 0 || (ldb % kStage) != 0)
     return hipErrorInvalidValue;
   if (flags != nullptr && (nkb > kMaxFlagBlocks || (int64_t)nkb * 32 < K)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((gemm_split_kernel<SP, CBW>), dim3((unsigned)gx, (unsigned)gy), dim3(512), 0, s, P, npt,
+  hipLaunchKernelGGL((gemm_split_kernel<SP, CBW, WR>), dim3((unsigned)gx, (unsigned)gy), dim3(WR * 8), 0, s, P, npt,
                      K, Ncol, A, lda, rowmax, groups, (const uint16_t *)B, ldb, colinv, flags, nkb, perm, C,
                      ldc);
   return hipGetLastError();

@@ -273,9 +273,9 @@ hipError_t launch_ao_vgl(hipStream_t s, const DeviceBasis &b, int64_t P, const d
                          int ncomp, uint8_t *flags = nullptr, int nkb = 0);
 // AO operand of the mixed contraction, FP32 in the blocked layout
 // buf[c][64-point tile][AO/4][point in tile][4] (AOs zero-padded to ld, a
-// multiple of 64), tile row i holding point perm[i] (null: point i),
-// partial row maxima rowmax[group][c][tile row] over each of `groups` AO
-// groups (ao_rows_groups) and, if flags is not null, nonzero flags
+// multiple of 64), tile row i holding point perm[i] (null: point i), row
+// maxima rowmax[c][tile row] (the AOs are split over `groups` wavefronts
+// per tile, ao_rows_groups) and, if flags is not null, nonzero flags
 // [5 * tiles][nkb] with nkb = ld / 32.  With flags, a 64-AO stage that the
 // screen zeroes for all 64 points of a tile is not written (its two flags
 // are zero, so the contraction never reads it).
@@ -335,9 +335,9 @@ hipError_t launch_split_coef(hipStream_t s, int form, int64_t M, int64_t K, cons
 // points with the prepared coefficients into mo_vgl C [P][5][Ncol] (ldc);
 // tile row i holds point perm[i] (null: point i).
 hipError_t launch_mo_gemm_mixed(hipStream_t s, int form, int64_t P, int64_t K, int64_t Ncol,
-                                const float *A, int64_t lda, const float *rowmax, int groups,
-                                const void *B, int64_t ldb, const float *colinv, const uint8_t *flags,
-                                int nkb, const int32_t *perm, double *C, int64_t ldc);
+                                const float *A, int64_t lda, const float *rowmax, const void *B, int64_t ldb,
+                                const float *colinv, const uint8_t *flags, int nkb, const int32_t *perm,
+                                double *C, int64_t ldc);
 hipError_t launch_strided_copy(hipStream_t s, int64_t rows, int64_t cols, const double *in,
                                int64_t ld_in, double *out, int64_t ld_out);
 

@@ -15,10 +15,10 @@
 // pieces, 64 of which (one per point of the tile) form a contiguous
 // kilobyte for each group of 4 AOs.
 // AOs past ao_num up to ld (a multiple of 64) are written as zeros.
-// Also written: rowmax[group][c][tile row] = max |entry| of the row over the
-// group's AOs (each lane owns its five rows, so no reduction is needed; the
-// contraction takes the max over groups), and nonzero flags per (row tile,
-// 32-AO block) gathered with wavefront ballots.
+// Also written: rowmax[c][tile row] = max |entry| of the row (each lane owns
+// its five rows, so a wavefront needs no reduction; the AO groups of a tile
+// combine theirs with atomic integer maxima of the bit patterns), and
+// nonzero flags per (row tile, 32-AO block) gathered with wavefront ballots.
 //
 // The 64 points of tile t are perm[64t .. 64t+63] (the caller orders the
 // points so that a tile is spatially compact).  With flags, a 64-AO stage
@@ -38,6 +38,8 @@
 // wavefronts of a workgroup are independent, each with its own LDS ring for
 // assembling 16-byte pieces.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "internal.hpp"
 
@@ -294,10 +296,18 @@ ao_rows_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord, co
     const double zero[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     while (w.k < w.k_hi) w.push(zero);
   }
-  // partial row maxima of this AO group, [groups][5][npt*64]
-  float *rm = rowmax + (int64_t)g * 5 * npt * 64 + pt * 64 + lane;
+  // row maxima [5][npt*64]: the max over the AO groups, as an integer max
+  // of the bit patterns (the order of non-negative floats); the array was
+  // zeroed by fill_zero_kernel
+  uint32_t *rm = reinterpret_cast<uint32_t *>(rowmax) + pt * 64 + lane;
 #pragma unroll
-  for (int c = 0; c < 5; ++c) rm[c * npt * 64] = inside ? w.mx[c] : 0.0f;
+  for (int c = 0; c < 5; ++c)
+    if (inside && w.mx[c] > 0.0f) atomicMax(rm + c * npt * 64, __float_as_uint(w.mx[c]));
+}
+
+__global__ void __launch_bounds__(kThreads) fill_zero_kernel(int64_t n, uint32_t *__restrict__ p) {
+  for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+    p[i] = 0u;
 }
 
 }  // namespace
@@ -308,7 +318,10 @@ int ao_rows_groups(int64_t P, int64_t ld) {
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
     cus = 256;
   const int64_t npt = (P + 63) / 64, nst = ld / 64;
-  const int64_t want = 16 * (int64_t)cus;  // about four wavefronts per SIMD
+#ifndef AO_GROUP_WAVES
+#define AO_GROUP_WAVES 64
+#endif
+  const int64_t want = AO_GROUP_WAVES * (int64_t)cus;  // about sixteen wavefronts per SIMD
   int64_t g = (want + npt - 1) / npt;
   if (g > nst) g = nst;
   if (g < 1) g = 1;
@@ -327,6 +340,9 @@ hipError_t launch_ao_rows_f32(hipStream_t s, const DeviceBasis &b, int64_t P, co
   const int64_t waves = npt * groups;
   const int64_t blocks = (waves + kThreads / 64 - 1) / (kThreads / 64);
   if (blocks > 0x7fffffff || ld > 0x7fffffff) return hipErrorInvalidValue;
+  const int64_t nmax = 5 * npt * 64;
+  hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)std::min<int64_t>((nmax + kThreads - 1) / kThreads, 4096)),
+                     dim3(kThreads), 0, s, nmax, reinterpret_cast<uint32_t *>(rowmax));
   hipLaunchKernelGGL(ao_rows_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, P, (int)b.nucl_num,
                      b.shell_num, coord, perm, nucl, b.shells, b.stage_shell, ao_factor, b.expo, b.coef_norm,
                      b.nucl_range, b.ao_num, ld, npt, groups, buf, rowmax, flags, nkb);

@@ -16,6 +16,10 @@
 // for mo_num <= 320 each AO row is read from HBM once.  With nonzero flags
 // from the AO kernel, K tiles whose AOs are all screened on the 64 rows are
 // skipped (adding exact zeros changes nothing).
+// That kernel serves the low-level launchers and the adjoint's GEMMs; the
+// mo_vgl / mo_value providers use gemm_f64_frag_kernel below, which takes
+// the coefficients prepared at MO finalize in MFMA fragment order (36.0 ->
+// 14.3 ms at 409,600 x 4,160 x 320 on MI355X).
 #include <hip/hip_runtime.h>
 
 #include "internal.hpp"

@@ -90,7 +90,7 @@ __device__ inline f4 mfma(const uint4 &a, const uint4 &b, f4 c) {
 // Split GEMM (FP16X2, BF16X2, BF16X3).
 //   A      the FP32 AO operand in the blocked layout of launch_ao_rows_f32:
 //          row tile rt = (component c, 64 points), [rt][lda/4][64][4]
-//   rowmax [groups][5 * tiles * 64] partial row maxima (max over groups)
+//   rowmax [5 * tiles * 64] row maxima
 //   Bs     split C terms (pre-scaled by their column scale) in the MFMA
 //          fragment order of split_coef_kernel (frag = 16), MOs padded to
 //          the workgroup width with zeros
@@ -168,7 +168,7 @@ __device__ inline void split8(const float (&v)[8], uint4 (&out)[Terms<SP>::n]) {
 template <int SP, int CBW, int WR>
 __global__ void __attribute__((amdgpu_flat_work_group_size(WR * 8, WR * 8), amdgpu_waves_per_eu(2, 2)))
 gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *__restrict__ A,
-                  int64_t lda, const float *__restrict__ rowmax, int groups, const uint16_t *__restrict__ Bs,
+                  int64_t lda, const float *__restrict__ rowmax, const uint16_t *__restrict__ Bs,
                   int64_t ldb, const float *__restrict__ colinv, const uint8_t *__restrict__ flags, int nkb,
                   const int32_t *__restrict__ perm, double *__restrict__ C, int64_t ldc) {
   constexpr int T = Terms<SP>::n;
@@ -215,9 +215,7 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
   // scale (from its max |entry| over all K).
   const int ar = tid % WR, aq = (tid / WR) * KQ;
   const float *Ap = A + rt * (lda / 4) * 256 + (aq / 4) * 256 + (r0 + ar) * 4;
-  float rmax = 0.0f;  // the row's max |entry|: max over the AO groups' partial maxima
-  for (int g = 0; g < groups; ++g) rmax = fmaxf(rmax, rowmax[(int64_t)g * 5 * npt * 64 + rt * 64 + r0 + ar]);
-  const float sc = pow2_scale(rmax);
+  const float sc = pow2_scale(rowmax[rt * 64 + r0 + ar]);  // from the row's max |entry|
   if (aq == 0) s_inv[ar] = 1.0f / sc;
   // C operand in fragment order (split_coef_kernel): the 16-byte piece of
   // lane l for column tile ct, 32-K chunk kc, term t is at
@@ -480,7 +478,7 @@ hipError_t launch_f32(hipStream_t s, int64_t P, int64_t npt, int64_t K, int64_t 
 
 template <int SP, int CBW>
 hipError_t launch_split(hipStream_t s, int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *A,
-                        int64_t lda, const float *rowmax, int groups, const void *B, int64_t ldb,
+                        int64_t lda, const float *rowmax, const void *B, int64_t ldb,
                         const float *colinv, const uint8_t *flags, int nkb, const int32_t *perm, double *C,
                         int64_t ldc) {
   constexpr int WR = 32;
@@ -489,14 +487,14 @@ hipError_t launch_split(hipStream_t s, int64_t P, int64_t npt, int64_t K, int64_
     return hipErrorInvalidValue;
   if (flags != nullptr && (nkb > kMaxFlagBlocks || (int64_t)nkb * 32 < K)) return hipErrorInvalidValue;
   hipLaunchKernelGGL((gemm_split_kernel<SP, CBW, WR>), dim3((unsigned)gx, (unsigned)gy), dim3(WR * 8), 0, s, P, npt,
-                     K, Ncol, A, lda, rowmax, groups, (const uint16_t *)B, ldb, colinv, flags, nkb, perm, C,
+                     K, Ncol, A, lda, rowmax, (const uint16_t *)B, ldb, colinv, flags, nkb, perm, C,
                      ldc);
   return hipGetLastError();
 }
 
 template <int SP>
 hipError_t launch_form(hipStream_t s, int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *A,
-                       int64_t lda, const float *rowmax, int groups, const void *B, int64_t ldb,
+                       int64_t lda, const float *rowmax, const void *B, int64_t ldb,
                        const float *colinv, const uint8_t *flags, int nkb, const int32_t *perm, double *C,
                        int64_t ldc) {
   if constexpr (SP == QMCKL_AMD_SPLIT_FP32) {
@@ -508,12 +506,12 @@ hipError_t launch_form(hipStream_t s, int64_t P, int64_t npt, int64_t K, int64_t
     return launch_f32<5>(s, P, npt, K, Ncol, A, lda, B, ldb, flags, nkb, perm, C, ldc);
   } else {
     const int cbw = split_wg_cols(SP, Ncol) / 64;
-    if (cbw == 1) return launch_split<SP, 1>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, perm, C, ldc);
-    if (cbw == 2) return launch_split<SP, 2>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, perm, C, ldc);
-    if (cbw == 3) return launch_split<SP, 3>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, perm, C, ldc);
+    if (cbw == 1) return launch_split<SP, 1>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc);
+    if (cbw == 2) return launch_split<SP, 2>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc);
+    if (cbw == 3) return launch_split<SP, 3>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc);
     if constexpr (Terms<SP>::n == 2) {
-      if (cbw == 4) return launch_split<SP, 4>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, perm, C, ldc);
-      if (cbw == 5) return launch_split<SP, 5>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, perm, C, ldc);
+      if (cbw == 4) return launch_split<SP, 4>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc);
+      if (cbw == 5) return launch_split<SP, 5>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc);
     }
     return hipErrorInvalidValue;
   }
@@ -556,22 +554,22 @@ hipError_t launch_split_coef(hipStream_t s, int form, int64_t M, int64_t K, cons
 }
 
 hipError_t launch_mo_gemm_mixed(hipStream_t s, int form, int64_t P, int64_t K, int64_t Ncol,
-                                const float *A, int64_t lda, const float *rowmax, int groups,
-                                const void *B, int64_t ldb, const float *colinv, const uint8_t *flags,
-                                int nkb, const int32_t *perm, double *C, int64_t ldc) {
+                                const float *A, int64_t lda, const float *rowmax, const void *B, int64_t ldb,
+                                const float *colinv, const uint8_t *flags, int nkb, const int32_t *perm,
+                                double *C, int64_t ldc) {
   if (P <= 0 || Ncol <= 0) return hipSuccess;
-  if (K <= 0 || (lda % 64) != 0 || (ldb % 64) != 0 || lda < K || ldb < K || groups < 1)
+  if (K <= 0 || (lda % 64) != 0 || (ldb % 64) != 0 || lda < K || ldb < K)
     return hipErrorInvalidValue;
   const int64_t npt = (P + 63) / 64;
   switch (form) {
     case QMCKL_AMD_SPLIT_FP16X2:
-      return launch_form<QMCKL_AMD_SPLIT_FP16X2>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, perm, C, ldc);
+      return launch_form<QMCKL_AMD_SPLIT_FP16X2>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc);
     case QMCKL_AMD_SPLIT_BF16X2:
-      return launch_form<QMCKL_AMD_SPLIT_BF16X2>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, perm, C, ldc);
+      return launch_form<QMCKL_AMD_SPLIT_BF16X2>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc);
     case QMCKL_AMD_SPLIT_BF16X3:
-      return launch_form<QMCKL_AMD_SPLIT_BF16X3>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, perm, C, ldc);
+      return launch_form<QMCKL_AMD_SPLIT_BF16X3>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc);
     case QMCKL_AMD_SPLIT_FP32:
-      return launch_form<QMCKL_AMD_SPLIT_FP32>(s, P, npt, K, Ncol, A, lda, rowmax, groups, B, ldb, colinv, flags, nkb, perm, C, ldc);
+      return launch_form<QMCKL_AMD_SPLIT_FP32>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc);
     default:
       return hipErrorInvalidValue;
   }

@@ -57,7 +57,7 @@ qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Contex
   const int groups = ao_rows_groups(P, ld);
   if (!grow_bytes(c, reinterpret_cast<void **>(&ao.ao_f32), &ao.ao_f32_cap, (int64_t)sizeof(float) * rows * ld) ||
       !grow_bytes(c, reinterpret_cast<void **>(&ao.ao_f32_rowmax), &ao.ao_f32_rowmax_cap,
-                  (int64_t)sizeof(float) * rows * groups))
+                  (int64_t)sizeof(float) * rows))
     return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
   uint8_t *flags = nullptr;
   if (nkb <= kMaxFlagBlocks) {
@@ -98,8 +98,8 @@ qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Contex
   }
   if (e == hipSuccess) {
     KernelTimer t(c, gemm_timer_name(form));
-    e = launch_mo_gemm_mixed(c->stream, form, P, A, M, ao.ao_f32, ld, ao.ao_f32_rowmax, groups,
-                             mo.coef_split, ld, mo.coef_colinv, flags, flags ? nkb : 0, perm, dst, M);
+    e = launch_mo_gemm_mixed(c->stream, form, P, A, M, ao.ao_f32, ld, ao.ao_f32_rowmax, mo.coef_split, ld,
+                             mo.coef_colinv, flags, flags ? nkb : 0, perm, dst, M);
     t.stop();
   }
   if (e == hipSuccess) e = sync(c);

@@ -126,11 +126,17 @@ __device__ inline f4 mfma(const uint4 &a, const uint4 &b, f4 c) {
 // SIMD (the FP64 accumulators then fit 256 registers) from 4.3 to 3.7 ms;
 // WR = 32 (two independent 4-wavefront workgroups per CU) equals WR = 64
 // for FP16X2 and is 2.5% faster for BF16X3.
-// Measured slower: 64 x 96 tiles of v_mfma_f32_32x32x16 (MOs padded to
-// 384), software-pipelined single-barrier loops (with and without
+// With the points ordered into compact tiles the kernel runs 3.2 ms, and
+// 1.9 ms with its global loads taken out (a timing variant): 32 cycles per
+// 16x16x32 MFMA per SIMD, so the loop is MFMA-bound plus what the two
+// wavefronts per SIMD cannot hide of the operand loads.
+// Measured slower: 64 x 96 tiles of v_mfma_f32_32x32x16 per wavefront at
+// one per SIMD (MOs padded to 384; 4.0 ms, 2.7 ms without loads: 75 cycles
+// per 32x32x16), software-pipelined single-barrier loops (with and without
 // interleaving hints), persistent workgroups, four-wavefront workgroups of
-// 64 x 160 with the column halves on alternate XCDs, and FP32 sums over
-// four stages before the FP64 addition (no faster; twice the error).
+// 64 x 160 with the column halves on alternate XCDs, FP32 sums over two or
+// four stages before the FP64 addition (1% faster, twice the error), and
+// skipping the MFMAs of zero 32-K halves inside a live stage.
 constexpr int kRM = 64;           // rows per workgroup
 constexpr int kStage = 64;        // K per stage
 constexpr int kLDS = kStage + 8;  // LDS row length in halves (bank spread)

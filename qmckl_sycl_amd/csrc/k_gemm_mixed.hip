@@ -132,7 +132,7 @@ __device__ inline f4 mfma(const uint4 &a, const uint4 &b, f4 c) {
 // wavefronts per SIMD cannot hide of the operand loads.
 // Measured slower: 64 x 96 tiles of v_mfma_f32_32x32x16 per wavefront at
 // one per SIMD (MOs padded to 384; 4.0 ms, 2.7 ms without loads: 75 cycles
-// per 32x32x16), software-pipelined single-barrier loops (with and without
+// per 32x32x16), 32 x 96 tiles of it at two per SIMD (3.9 ms), software-pipelined single-barrier loops (with and without
 // interleaving hints), persistent workgroups, four-wavefront workgroups of
 // 64 x 160 with the column halves on alternate XCDs, FP32 sums over two or
 // four stages before the FP64 addition (1% faster, twice the error), and

@@ -401,6 +401,29 @@ qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_batch_device(int64_t count,
                                                               double *const *mo_vgl,
                                                               const int64_t *size_max);
 
+/* ---- batching form as a HIP graph (new; no counterpart in the reference) --
+ * create: runs the batch above once (same arguments and errors), then
+ * records the same work of all count contexts (every kernel of every
+ * context's chain, each context on its own stream, forked from and joined
+ * into one capture stream) into a HIP graph and returns its handle in
+ * *graph (ARG_5 if graph is NULL).
+ * launch: replays the graph and returns when every output is complete.  It
+ * reads the contexts' current point coordinates, so a caller moves the
+ * points with the usual setters (same point counts) between launches.  A
+ * change that alters what the recorded kernels take as arguments (point or
+ * basis sizes, precision form, MO coefficients, a reallocated buffer, a
+ * destroyed context) makes launch fail (QMCKL_FAILURE_DEVICE, or
+ * INVALID_CONTEXT) instead of running stale work: create a new graph.
+ * An unknown handle -> ARG_1.
+ * destroy: releases the graph (ARG_1 for an unknown handle). */
+qmckl_exit_code_device qmckl_mo_basis_mo_vgl_batch_graph_create_device(int64_t count,
+                                                                       const qmckl_context_device *contexts,
+                                                                       double *const *mo_vgl,
+                                                                       const int64_t *size_max,
+                                                                       int64_t *graph);
+qmckl_exit_code_device qmckl_mo_basis_mo_vgl_batch_graph_launch_device(int64_t graph);
+qmckl_exit_code_device qmckl_mo_basis_mo_vgl_batch_graph_destroy_device(int64_t graph);
+
 /* ---- kernel timing (new; no counterpart in the reference) ----------------
  * When enabled, every kernel the context launches is bracketed by HIP events
  * on the context's stream and its elapsed time is accumulated under the

@@ -267,3 +267,32 @@ def batch_mo_vgl(contexts, outputs):
     if rc != 0:
         raise QmcklError(rc, "qmckl_get_mo_basis_mo_vgl_batch_device")
     return rc
+
+
+class BatchGraph:
+    """qmckl_mo_basis_mo_vgl_batch_graph_{create,launch,destroy}_device over
+    Context objects and their DeviceArray outputs: the batch recorded once
+    into a HIP graph, replayed by launch()."""
+
+    def __init__(self, contexts, outputs):
+        n = len(contexts)
+        if n != len(outputs):
+            raise ValueError("one output per context")
+        handles = (ctypes.c_int64 * n)(*[c.handle for c in contexts])
+        ptrs = (ctypes.c_void_p * n)(*[o.ptr for o in outputs])
+        sizes = (ctypes.c_int64 * n)(*[o.size for o in outputs])
+        g = ctypes.c_int64(0)
+        rc = lib().qmckl_mo_basis_mo_vgl_batch_graph_create_device(n, handles, ptrs, sizes, ctypes.byref(g))
+        if rc != 0:
+            raise QmcklError(rc, "qmckl_mo_basis_mo_vgl_batch_graph_create_device")
+        self.handle = g.value
+
+    def launch(self):
+        """Replays the graph; returns the exit code (0 on success)."""
+        return lib().qmckl_mo_basis_mo_vgl_batch_graph_launch_device(self.handle)
+
+    def destroy(self):
+        if self.handle:
+            lib().qmckl_mo_basis_mo_vgl_batch_graph_destroy_device(self.handle)
+            self.handle = 0
+

@@ -109,7 +109,7 @@ qmckl_exit_code_device hip_check(qmckl_context_device context, hipError_t e, con
 }
 
 KernelTimer::KernelTimer(Context *c, const char *name) : c_(c), name_(name) {
-  if (!c_->timing || !has_device(c_)) return;
+  if (!c_->timing || c_->capturing || !has_device(c_)) return;
   if (hipEventCreate(&start_) != hipSuccess || hipEventCreate(&stop_) != hipSuccess ||
       hipEventRecord(start_, c_->stream) != hipSuccess) {
     (void)hipGetLastError();

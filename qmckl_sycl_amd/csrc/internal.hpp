@@ -190,6 +190,8 @@ struct Context {
   // set while a batch enqueues this context's work: the compute chain's
   // stream synchronisations are skipped and the batch waits once at the end
   bool defer_sync = false;
+  // set while a batch graph is captured: kernel timers record nothing
+  bool capturing = false;
   struct TimerAcc {
     int64_t launches = 0;
     double total_ms = 0.0;

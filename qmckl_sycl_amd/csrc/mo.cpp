@@ -8,6 +8,9 @@
 // is one GEMM with 5P rows, run by the FP64 MFMA kernel on a copy of the
 // coefficients in MFMA fragment order made at finalize.
 #include <algorithm>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
 
 #include "internal.hpp"
 
@@ -478,3 +481,154 @@ qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_batch_device(int64_t count,
 }
 
 }  // extern "C"
+
+namespace {
+
+// A captured batch: the graph, the stream it is launched on, and for every
+// context what the captured kernels read as arguments (sizes, precision,
+// buffer and table addresses), so that a launch after a change that would
+// make the recorded arguments wrong is refused instead of run.
+struct BatchGraph {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<qmckl_context_device> contexts;
+  std::vector<std::vector<uint64_t>> signature;
+};
+
+std::mutex g_graph_mutex;
+std::unordered_map<int64_t, BatchGraph *> &graphs() {
+  static std::unordered_map<int64_t, BatchGraph *> m;
+  return m;
+}
+int64_t g_next_graph = 1;
+
+std::vector<uint64_t> batch_signature(const Context *c, const double *out) {
+  auto u = [](const void *p) { return (uint64_t)(uintptr_t)p; };
+  const AoBasis &ao = c->ao;
+  const MoBasis &mo = c->mo;
+  const DeviceBasis &b = ao.dev;
+  return {(uint64_t)c->point.num, (uint64_t)c->nucleus.num, (uint64_t)ao.ao_num, (uint64_t)mo.mo_num,
+          (uint64_t)mo.precision, (uint64_t)mo.coef_split_form, u(out), u(c->point.coord), u(c->nucleus.coord),
+          u(ao.ao_factor), u(ao.ao_vgl), u(ao.ao_vgl_flags), u(ao.ao_f32), u(ao.ao_f32_rowmax),
+          u(ao.ao_f32_flags), u(ao.point_perm), u(ao.order_ws), u(b.shells), u(b.chunks), u(b.stage_shell),
+          u(b.expo), u(b.coef_norm), u(b.nucl_range), u(b.ao_shell), u(b.ao_abc), u(mo.coef_frag),
+          u(mo.coef_split), u(mo.coef_colinv)};
+}
+
+}  // namespace
+
+extern "C" {
+
+qmckl_exit_code_device qmckl_mo_basis_mo_vgl_batch_graph_create_device(int64_t count,
+                                                                       const qmckl_context_device *contexts,
+                                                                       double *const *mo_vgl,
+                                                                       const int64_t *size_max,
+                                                                       int64_t *graph) {
+  const char *fn = "qmckl_mo_basis_mo_vgl_batch_graph_create_device";
+  if (graph == nullptr) return QMCKL_INVALID_ARG_5_DEVICE;
+  // one eager run: validates the arguments, allocates every buffer and
+  // prepares the coefficients, so that the capture below only records
+  // kernels
+  qmckl_exit_code_device rc = qmckl_get_mo_basis_mo_vgl_batch_device(count, contexts, mo_vgl, size_max);
+  if (rc != QMCKL_SUCCESS_DEVICE) return rc;
+  Context *c0 = ctx_of(contexts[0]);
+  auto *g = new BatchGraph;
+  g->device = c0->device_id;
+  g->contexts.assign(contexts, contexts + count);
+  hipError_t e = hipSetDevice(g->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> joins((size_t)count, nullptr);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
+  for (int64_t i = 0; i < count && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&joins[(size_t)i], hipEventDisableTiming);
+  bool capturing = false;
+  if (e == hipSuccess) {
+    e = hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal);
+    capturing = e == hipSuccess;
+  }
+  if (e == hipSuccess) e = hipEventRecord(fork, g->stream);
+  // every context's stream joins the capture, enqueues its chain, and is
+  // joined back
+  for (int64_t i = 0; i < count && e == hipSuccess && rc == QMCKL_SUCCESS_DEVICE; ++i) {
+    Context *c = ctx_of(contexts[i]);
+    e = hipStreamWaitEvent(c->stream, fork, 0);
+    if (e != hipSuccess) break;
+    c->defer_sync = true;
+    c->capturing = true;
+    rc = qmckl_context_touch_device(contexts[i]);
+    if (rc == QMCKL_SUCCESS_DEVICE) rc = compute_mo_vgl(contexts[i], c, mo_vgl[i], fn);
+    c->defer_sync = false;
+    c->capturing = false;
+    if (rc == QMCKL_SUCCESS_DEVICE) e = hipEventRecord(joins[(size_t)i], c->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(g->stream, joins[(size_t)i], 0);
+  }
+  if (capturing) {
+    const hipError_t e2 = hipStreamEndCapture(g->stream, &g->graph);
+    if (e == hipSuccess) e = e2;
+  }
+  if (e == hipSuccess && rc == QMCKL_SUCCESS_DEVICE) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (fork) (void)hipEventDestroy(fork);
+  for (hipEvent_t j : joins)
+    if (j) (void)hipEventDestroy(j);
+  if (e == hipSuccess && rc == QMCKL_SUCCESS_DEVICE) {
+    for (int64_t i = 0; i < count; ++i) g->signature.push_back(batch_signature(ctx_of(contexts[i]), mo_vgl[i]));
+    std::lock_guard<std::mutex> lock(g_graph_mutex);
+    *graph = g_next_graph++;
+    graphs()[*graph] = g;
+    return QMCKL_SUCCESS_DEVICE;
+  }
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->stream) (void)hipStreamDestroy(g->stream);
+  delete g;
+  (void)hipGetLastError();
+  if (rc != QMCKL_SUCCESS_DEVICE) return rc;
+  return fail(contexts[0], QMCKL_FAILURE_DEVICE, fn, hipGetErrorString(e));
+}
+
+qmckl_exit_code_device qmckl_mo_basis_mo_vgl_batch_graph_launch_device(int64_t graph) {
+  const char *fn = "qmckl_mo_basis_mo_vgl_batch_graph_launch_device";
+  BatchGraph *g = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_graph_mutex);
+    auto it = graphs().find(graph);
+    if (it == graphs().end()) return QMCKL_INVALID_ARG_1_DEVICE;
+    g = it->second;
+  }
+  for (size_t i = 0; i < g->contexts.size(); ++i) {
+    Context *c = ctx_of(g->contexts[i]);
+    if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+    std::vector<uint64_t> now = batch_signature(c, reinterpret_cast<const double *>(g->signature[i][6]));
+    if (now != g->signature[i])
+      return fail(g->contexts[i], QMCKL_FAILURE_DEVICE, fn,
+                  "the context's sizes, precision form, basis or buffers changed since the capture");
+  }
+  hipError_t e = hipSetDevice(g->device);
+  if (e == hipSuccess) e = hipGraphLaunch(g->exec, g->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+  if (e != hipSuccess) return fail(g->contexts[0], QMCKL_FAILURE_DEVICE, fn, hipGetErrorString(e));
+  return QMCKL_SUCCESS_DEVICE;
+}
+
+qmckl_exit_code_device qmckl_mo_basis_mo_vgl_batch_graph_destroy_device(int64_t graph) {
+  BatchGraph *g = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_graph_mutex);
+    auto it = graphs().find(graph);
+    if (it == graphs().end()) return QMCKL_INVALID_ARG_1_DEVICE;
+    g = it->second;
+    graphs().erase(it);
+  }
+  (void)hipSetDevice(g->device);
+  (void)hipStreamSynchronize(g->stream);
+  (void)hipGraphExecDestroy(g->exec);
+  (void)hipGraphDestroy(g->graph);
+  (void)hipStreamDestroy(g->stream);
+  delete g;
+  return QMCKL_SUCCESS_DEVICE;
+}
+
+}  // extern "C"
+

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from qmckl_sycl_amd import systems as S
-from qmckl_sycl_amd.context import Context, DeviceArray, QmcklError, batch_mo_vgl
+from qmckl_sycl_amd.context import BatchGraph, Context, DeviceArray, QmcklError, batch_mo_vgl
 
 from helpers import RTOL_VGL, assert_vgl_close, say
 
@@ -87,3 +87,65 @@ def test_batch_rejects_small_output(molecules):
     finally:
         for c in ctxs:
             c.destroy()
+
+
+def _move(ctxs, mols, rng):
+    """New electron positions for every molecule (same counts)."""
+    for c, m in zip(ctxs, mols):
+        e = np.asarray(m.elec_coord)
+        c.set_electron_coord(e + rng.normal(0.0, 0.1, size=e.shape))
+
+
+def test_batch_graph_replays_match_eager(molecules):
+    """The batch recorded into a HIP graph: every replay, with the points
+    moved and eager batches run in between, equals the eager batch on the
+    same points bitwise (FP64 and FP16X2 contexts mixed)."""
+    mols = molecules[:12]
+    ctxs, outs = setup(mols, forms=[0, 2])
+    ref_outs = [DeviceArray(c, (m.point_num, 5, m.mo_num)) for c, m in zip(ctxs, mols)]
+    g = None
+    try:
+        g = BatchGraph(ctxs, outs)
+        rng = np.random.default_rng(11)
+        for rep in range(3):
+            _move(ctxs, mols, rng)
+            batch_mo_vgl(ctxs, ref_outs)          # eager work between replays
+            ref = [o.numpy() for o in ref_outs]
+            assert g.launch() == 0
+            got = [o.numpy() for o in outs]
+            for i, m in enumerate(mols):
+                assert np.array_equal(got[i], ref[i]), (rep, m.name)
+            say(f"replay {rep}: {len(mols)} molecules equal to the eager batch")
+    finally:
+        if g is not None:
+            g.destroy()
+        for c in ctxs:
+            c.destroy()
+
+
+def test_batch_graph_refuses_stale_launch(molecules):
+    """After a change that alters the recorded kernels' arguments (here the
+    precision form) a launch fails instead of running stale work; unknown
+    handles are rejected."""
+    mols = molecules[:3]
+    ctxs, outs = setup(mols)
+    g = None
+    try:
+        g = BatchGraph(ctxs, outs)
+        assert g.launch() == 0
+        ctxs[1].call("qmckl_set_mo_basis_precision_device", 2)
+        assert g.launch() == 101          # QMCKL_FAILURE_DEVICE
+        ctxs[1].call("qmckl_set_mo_basis_precision_device", 0)
+        assert g.launch() == 0
+        assert lib_rc_launch(987654321) == 1   # unknown handle: ARG_1
+    finally:
+        if g is not None:
+            g.destroy()
+        for c in ctxs:
+            c.destroy()
+
+
+def lib_rc_launch(handle):
+    from qmckl_sycl_amd.context import lib
+    return lib().qmckl_mo_basis_mo_vgl_batch_graph_launch_device(handle)
+

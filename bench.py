@@ -114,11 +114,14 @@ def run_form(ctx, s, form, steps, warmup):
 def bench_batch(forms, steps, warmup, scale=1.0):
     """Batching form: the 48-molecule ragged batch (2-64 nuclei, 30-1,000 AOs,
     16-512 walkers) through qmckl_get_mo_basis_mo_vgl_batch_device, one
-    context per molecule, timed per form; next to it the same molecules one
-    inplace getter call after another (each call synchronises)."""
+    context per molecule, timed per form with the default tile choices;
+    next to it the same molecules one inplace getter call after another
+    (each call synchronises), the batch replayed as a HIP graph, and for the
+    mixed forms both again after qmckl_mo_basis_autotune_device on every
+    context (the default tiles are the fixed-tile baseline)."""
     import torch
     from qmckl_sycl_amd import systems
-    from qmckl_sycl_amd.context import Context, DeviceArray, batch_mo_vgl, lib
+    from qmckl_sycl_amd.context import BatchGraph, Context, DeviceArray, batch_mo_vgl, lib
     L = lib()
     mols = systems.ragged_batch(48, scale=scale)
     points = int(sum(m.point_num for m in mols))
@@ -154,6 +157,28 @@ def bench_batch(forms, steps, warmup, scale=1.0):
                     c.check(L.qmckl_get_mo_basis_mo_vgl_inplace_device(c.handle, o.ptr, o.size))
             sec = timed(one_by_one)
             res[form]["one_call_per_molecule"] = {"ms_per_step": sec * 1e3, "value": points / sec}
+
+            def graph_rate():
+                g = BatchGraph(ctxs, outs)
+                try:
+                    def launch():
+                        rc = g.launch()
+                        if rc != 0:
+                            raise RuntimeError(f"graph launch returned {rc}")
+                    return timed(launch)
+                finally:
+                    g.destroy()
+            sec = graph_rate()
+            res[form]["graph"] = {"ms_per_step": sec * 1e3, "value": points / sec}
+            if form != "plain":
+                for c in ctxs:
+                    c.check(L.qmckl_mo_basis_autotune_device(c.handle))
+                sec = timed(lambda: batch_mo_vgl(ctxs, outs))
+                res[form]["autotuned"] = {"ms_per_step": sec * 1e3, "value": points / sec}
+                sec = graph_rate()
+                res[form]["autotuned_graph"] = {"ms_per_step": sec * 1e3, "value": points / sec}
+                for c in ctxs:  # back to the fixed tiles for any later form
+                    c.check(L.qmckl_set_mo_basis_tile_config_device(c.handle, -1, 0))
             log("batch", form, json.dumps(res[form]))
     finally:
         for c in ctxs:

@@ -401,6 +401,24 @@ qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_batch_device(int64_t count,
                                                               double *const *mo_vgl,
                                                               const int64_t *size_max);
 
+/* ---- tile choices and autotuner (new; no counterpart in the reference) -----
+ * The mixed precision forms have two tile choices that change speed, not
+ * results: whether points are ordered spatially into 64-point tiles
+ * (point_order -1 automatic, 0 off, 1 on) and how many wavefronts per CU
+ * the AO operand kernel aims at (group_waves 0 automatic, or 1..1024).
+ * set: ARG_2 / ARG_3 for values out of range.  get: ARG_2 / ARG_3 for NULL.
+ * autotune: times the context's mixed chain on its current points for
+ * point_order 0/1 x group_waves 16/32/64/128 and keeps the fastest (a
+ * process-wide cache reuses the choice for contexts of the same device,
+ * point, AO, MO and nucleus counts and form); FP64 form: nothing to choose.
+ * Needs the basis and points (NOT_PROVIDED otherwise).  It overwrites the
+ * context's cached mo_vgl, which is recomputed on the next get. */
+qmckl_exit_code_device qmckl_set_mo_basis_tile_config_device(qmckl_context_device context, int32_t point_order,
+                                                             int32_t group_waves);
+qmckl_exit_code_device qmckl_get_mo_basis_tile_config_device(qmckl_context_device context, int32_t *point_order,
+                                                             int32_t *group_waves);
+qmckl_exit_code_device qmckl_mo_basis_autotune_device(qmckl_context_device context);
+
 /* ---- batching form as a HIP graph (new; no counterpart in the reference) --
  * create: runs the batch above once (same arguments and errors), then
  * records the same work of all count contexts (every kernel of every

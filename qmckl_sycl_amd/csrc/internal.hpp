@@ -161,6 +161,13 @@ struct MoBasis {
   uint16_t *coef_split = nullptr;
   float *coef_colinv = nullptr;
   qmckl_precision_device coef_split_form = QMCKL_PRECISION_FP64_DEVICE;
+  // tile choices of the mixed forms (results do not depend on them):
+  // spatial point order -1 auto / 0 off / 1 on; AO-operand wavefronts per
+  // CU 0 auto / else that many (the AOs of a tile are split into groups to
+  // reach it); tuned: set by the autotuner
+  int32_t tile_point_order = -1;
+  int32_t tile_group_waves = 0;
+  bool tile_tuned = false;
   // adjoint scratch: G = w C [5P][A], per-block partials, split-K partials
   double *adj_G = nullptr;
   int64_t adj_G_cap = 0;
@@ -281,7 +288,7 @@ hipError_t launch_ao_vgl(hipStream_t s, const DeviceBasis &b, int64_t P, const d
 // [5 * tiles][nkb] with nkb = ld / 32.  With flags, a 64-AO stage that the
 // screen zeroes for all 64 points of a tile is not written (its two flags
 // are zero, so the contraction never reads it).
-int ao_rows_groups(int64_t P, int64_t ld);
+int ao_rows_groups(int64_t P, int64_t ld, int group_waves = 0);
 hipError_t launch_ao_rows_f32(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
                               const int32_t *perm, const double *nucl, const double *ao_factor, int64_t ld,
                               int groups, float *buf, float *rowmax, uint8_t *flags, int nkb);

@@ -312,16 +312,16 @@ __global__ void __launch_bounds__(kThreads) fill_zero_kernel(int64_t n, uint32_t
 
 }  // namespace
 
-int ao_rows_groups(int64_t P, int64_t ld) {
+int ao_rows_groups(int64_t P, int64_t ld, int group_waves) {
   int dev = 0, cus = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
     cus = 256;
   const int64_t npt = (P + 63) / 64, nst = ld / 64;
-#ifndef AO_GROUP_WAVES
-#define AO_GROUP_WAVES 64
-#endif
-  const int64_t want = AO_GROUP_WAVES * (int64_t)cus;  // about sixteen wavefronts per SIMD
+  // default 64 wavefronts per CU (16 per SIMD: measured on water64 at 128
+  // walkers, 1.31 ms at 16, 1.23 at 32, 1.14 at 64, 1.08 at 128 but with a
+  // slower contraction after it)
+  const int64_t want = (group_waves > 0 ? group_waves : 64) * (int64_t)cus;
   int64_t g = (want + npt - 1) / npt;
   if (g > nst) g = nst;
   if (g < 1) g = 1;

@@ -57,7 +57,7 @@ qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Contex
   const int64_t rows = 5 * round_up(P, 64);  // (component, point) rows, points padded to 64
   AoBasis &ao = c->ao;
   MoBasis &mo = c->mo;
-  const int groups = ao_rows_groups(P, ld);
+  const int groups = ao_rows_groups(P, ld, mo.tile_group_waves);
   if (!grow_bytes(c, reinterpret_cast<void **>(&ao.ao_f32), &ao.ao_f32_cap, (int64_t)sizeof(float) * rows * ld) ||
       !grow_bytes(c, reinterpret_cast<void **>(&ao.ao_f32_rowmax), &ao.ao_f32_rowmax_cap,
                   (int64_t)sizeof(float) * rows))
@@ -71,7 +71,8 @@ qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Contex
   // Tiles of spatially close points screen out far more 64-AO stages per
   // tile; worth its four small launches for large bases and many points
   // (water64 at 128 walkers: 94% -> 74% of the stages live).
-  const bool order = flags != nullptr && ld >= 1024 && P >= 4096;
+  const bool order = flags != nullptr && (mo.tile_point_order >= 0 ? mo.tile_point_order == 1
+                                                                    : ld >= 1024 && P >= 4096);
   if (order && (!grow_bytes(c, reinterpret_cast<void **>(&ao.point_perm), &ao.point_perm_cap,
                             (int64_t)sizeof(int32_t) * P) ||
                 !grow_bytes(c, &ao.order_ws, &ao.order_ws_cap, point_order_workspace_bytes(P))))
@@ -478,6 +479,125 @@ qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_batch_device(int64_t count,
     if (first == QMCKL_SUCCESS_DEVICE) first = rc;
   }
   return first;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Autotuned tile choices per (device, points, AOs, MOs, nuclei, form), shared
+// by every context of the process with that shape.
+struct TuneKey {
+  int device;
+  int64_t P, A, M, N;
+  int form;
+  bool operator==(const TuneKey &o) const {
+    return device == o.device && P == o.P && A == o.A && M == o.M && N == o.N && form == o.form;
+  }
+};
+struct TuneKeyHash {
+  size_t operator()(const TuneKey &k) const {
+    size_t h = (size_t)k.device * 1000003u ^ (size_t)k.form;
+    for (int64_t v : {k.P, k.A, k.M, k.N}) h = h * 1000003u ^ (size_t)v;
+    return h;
+  }
+};
+std::mutex g_tune_mutex;
+std::unordered_map<TuneKey, std::pair<int32_t, int32_t>, TuneKeyHash> &tune_cache() {
+  static std::unordered_map<TuneKey, std::pair<int32_t, int32_t>, TuneKeyHash> m;
+  return m;
+}
+
+}  // namespace
+
+extern "C" {
+
+qmckl_exit_code_device qmckl_set_mo_basis_tile_config_device(qmckl_context_device context, int32_t point_order,
+                                                             int32_t group_waves) {
+  Context *c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+  const char *fn = "qmckl_set_mo_basis_tile_config_device";
+  if (point_order < -1 || point_order > 1) return fail(context, QMCKL_INVALID_ARG_2_DEVICE, fn, "point_order not in {-1, 0, 1}");
+  if (group_waves < 0 || group_waves > 1024) return fail(context, QMCKL_INVALID_ARG_3_DEVICE, fn, "group_waves not in [0, 1024]");
+  c->mo.tile_point_order = point_order;
+  c->mo.tile_group_waves = group_waves;
+  c->mo.tile_tuned = false;
+  return QMCKL_SUCCESS_DEVICE;
+}
+
+qmckl_exit_code_device qmckl_get_mo_basis_tile_config_device(qmckl_context_device context, int32_t *point_order,
+                                                             int32_t *group_waves) {
+  Context *c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+  const char *fn = "qmckl_get_mo_basis_tile_config_device";
+  if (point_order == nullptr) return fail(context, QMCKL_INVALID_ARG_2_DEVICE, fn, "NULL pointer");
+  if (group_waves == nullptr) return fail(context, QMCKL_INVALID_ARG_3_DEVICE, fn, "NULL pointer");
+  *point_order = c->mo.tile_point_order;
+  *group_waves = c->mo.tile_group_waves;
+  return QMCKL_SUCCESS_DEVICE;
+}
+
+qmckl_exit_code_device qmckl_mo_basis_autotune_device(qmckl_context_device context) {
+  Context *c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+  const char *fn = "qmckl_mo_basis_autotune_device";
+  qmckl_exit_code_device rc = ready(context, c, fn);
+  if (rc != QMCKL_SUCCESS_DEVICE) return rc;
+  MoBasis &mo = c->mo;
+  if (mo.precision == QMCKL_PRECISION_FP64_DEVICE) {  // nothing to choose for the FP64 form
+    mo.tile_tuned = true;
+    return QMCKL_SUCCESS_DEVICE;
+  }
+  if (c->point.num <= 0 || c->point.coord == nullptr) return fail(context, QMCKL_NOT_PROVIDED_DEVICE, fn, "no points set");
+  const TuneKey key{c->device_id, c->point.num, c->ao.ao_num, mo.mo_num, c->nucleus.num, (int)mo.precision};
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    auto it = tune_cache().find(key);
+    if (it != tune_cache().end()) {
+      mo.tile_point_order = it->second.first;
+      mo.tile_group_waves = it->second.second;
+      mo.tile_tuned = true;
+      return QMCKL_SUCCESS_DEVICE;
+    }
+  }
+  const int64_t need = 5 * c->point.num * mo.mo_num;
+  if (!grow(c, &mo.mo_vgl, &mo.mo_vgl_cap, need)) return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
+  mo.mo_vgl_date = 0;  // the timing runs write into the provider's buffer
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  const bool can_order = round_up(c->ao.ao_num, 64) / 32 <= kMaxFlagBlocks;
+  float best = -1.0f;
+  int32_t best_order = 0, best_waves = 64;
+  for (int32_t order = 0; order <= (can_order ? 1 : 0) && e == hipSuccess && rc == QMCKL_SUCCESS_DEVICE; ++order)
+    for (int32_t waves : {16, 32, 64, 128}) {
+      mo.tile_point_order = order;
+      mo.tile_group_waves = waves;
+      rc = compute_mo_vgl_mixed(context, c, mo.mo_vgl, fn);  // warm-up
+      if (rc != QMCKL_SUCCESS_DEVICE) break;
+      e = hipEventRecord(e0, c->stream);
+      for (int r = 0; r < 2 && rc == QMCKL_SUCCESS_DEVICE; ++r) rc = compute_mo_vgl_mixed(context, c, mo.mo_vgl, fn);
+      if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.0f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess || rc != QMCKL_SUCCESS_DEVICE) break;
+      if (best < 0.0f || ms < best) {
+        best = ms;
+        best_order = order;
+        best_waves = waves;
+      }
+    }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (rc != QMCKL_SUCCESS_DEVICE) return rc;
+  if (e != hipSuccess) return hip_check(context, e, fn);
+  mo.tile_point_order = best_order;
+  mo.tile_group_waves = best_waves;
+  mo.tile_tuned = true;
+  std::lock_guard<std::mutex> lock(g_tune_mutex);
+  tune_cache()[key] = {best_order, best_waves};
+  return QMCKL_SUCCESS_DEVICE;
 }
 
 }  // extern "C"

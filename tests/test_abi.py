@@ -300,3 +300,29 @@ def test_select_mo_and_ee_distance_validation(ctx):
     assert L.qmckl_compute_ee_distance_device(ctx.handle, 3, 0, 1234, 1234) == 3
     assert L.qmckl_compute_ee_distance_device(ctx.handle, 3, 1, None, 1234) == 4
     assert L.qmckl_compute_ee_distance_device(ctx.handle, 3, 1, 1234, None) == 5
+
+
+def test_tile_config_and_autotune_validation(ctx):
+    L = lib()
+    po, gw = ctypes.c_int32(9), ctypes.c_int32(9)
+    assert L.qmckl_get_mo_basis_tile_config_device(ctx.handle, ctypes.byref(po), ctypes.byref(gw)) == 0
+    assert (po.value, gw.value) == (-1, 0)  # automatic by default
+    assert L.qmckl_set_mo_basis_tile_config_device(ctx.handle, 2, 0) == 2
+    assert L.qmckl_set_mo_basis_tile_config_device(ctx.handle, 0, -1) == 3
+    assert L.qmckl_set_mo_basis_tile_config_device(ctx.handle, 0, 2000) == 3
+    assert L.qmckl_set_mo_basis_tile_config_device(ctx.handle, 1, 32) == 0
+    assert L.qmckl_get_mo_basis_tile_config_device(ctx.handle, ctypes.byref(po), ctypes.byref(gw)) == 0
+    assert (po.value, gw.value) == (1, 32)
+    assert L.qmckl_get_mo_basis_tile_config_device(ctx.handle, None, ctypes.byref(gw)) == 2
+    assert L.qmckl_get_mo_basis_tile_config_device(ctx.handle, ctypes.byref(po), None) == 3
+    assert L.qmckl_set_mo_basis_tile_config_device(0, 0, 0) == INVALID_CONTEXT
+    assert L.qmckl_mo_basis_autotune_device(ctx.handle) == NOT_PROVIDED
+    assert L.qmckl_mo_basis_autotune_device(0) == INVALID_CONTEXT
+    g = ctypes.c_int64(0)
+    assert L.qmckl_mo_basis_mo_vgl_batch_graph_launch_device(12345) == 1
+    assert L.qmckl_mo_basis_mo_vgl_batch_graph_destroy_device(12345) == 1
+    assert L.qmckl_mo_basis_mo_vgl_batch_graph_create_device(1, None, None, None, None) == 5
+    handles = (ctypes.c_int64 * 1)(ctx.handle)
+    ptrs = (ctypes.c_void_p * 1)(1234)
+    sizes = (ctypes.c_int64 * 1)(10)
+    assert L.qmckl_mo_basis_mo_vgl_batch_graph_create_device(0, handles, ptrs, sizes, ctypes.byref(g)) == 1

@@ -115,6 +115,30 @@ def test_point_order_invariance(form):
     assert np.array_equal(b, a[perm])
 
 
+@pytest.mark.parametrize("form", ["fp16x2", "bf16x3"])
+def test_tile_choices_and_autotune_keep_results(form):
+    """The tile choices (spatial point order on/off, AO-operand wavefronts
+    per CU) change speed only: every choice, and the autotuner's, gives the
+    same rows bitwise."""
+    s = S.water64(walk_num=8)
+    P = s.point_num
+    with Context(0) as c:
+        c.set_system(s)
+        ref = run_form(c, form, P)
+        for order in (0, 1):
+            for waves in (16, 128):
+                c.call("qmckl_set_mo_basis_tile_config_device", order, waves)
+                c.touch()
+                assert np.array_equal(c.mo_vgl(P), ref), (order, waves)
+        c.call("qmckl_set_mo_basis_tile_config_device", -1, 0)
+        c.call("qmckl_mo_basis_autotune_device")
+        po, gw = ctypes.c_int32(-5), ctypes.c_int32(-5)
+        c.call("qmckl_get_mo_basis_tile_config_device", ctypes.byref(po), ctypes.byref(gw))
+        say(f"autotuned {form}: point_order={po.value} group_waves={gw.value}")
+        assert po.value in (0, 1) and gw.value in (16, 32, 64, 128)
+        assert np.array_equal(c.mo_vgl(P), ref)
+
+
 def test_switching_forms_recomputes(chb):
     with Context(0) as c:
         c.set_system(chb)

@@ -240,6 +240,19 @@ def bench_adjoint(steps, warmup):
         sec = (time.perf_counter() - t0) / steps
         res = {"workload": f"water64 at 32 walkers ({s.point_num} points), FP64 forward + adjoint",
                "ms_per_step": sec * 1e3, "value": s.point_num / sec, "unit": "electron-points/s"}
+        # per-kernel breakdown from a separate run with the HIP-event timers on
+        ctx.check(L.qmckl_reset_kernel_timing_device(h))
+        ctx.check(L.qmckl_set_kernel_timing_device(h, True))
+        for _ in range(2):
+            step()
+        ctx.check(L.qmckl_set_kernel_timing_device(h, False))
+        kernels = {}
+        for name in ("ao_vgl", "mo_gemm_f64", "adjoint_g_gemm_f64", "ao_adjoint", "adjoint_dc_gemm_f64"):
+            n, ms = ctypes.c_int64(0), ctypes.c_double(0.0)
+            L.qmckl_get_kernel_timing_device(h, name.encode(), ctypes.byref(n), ctypes.byref(ms))
+            if n.value:
+                kernels[name] = {"launches": int(n.value), "avg_ms": ms.value / n.value}
+        res["kernels"] = kernels
         log("adjoint", json.dumps(res))
         return res
     finally:

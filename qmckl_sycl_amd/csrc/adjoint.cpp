@@ -1,8 +1,10 @@
 // Differentiation form: adjoint of a weighted sum of mo_vgl with respect to
 // nuclear coordinates, primitive exponents and MO coefficients.
-//   G      = w C                  (FP64 MFMA GEMM, [5P][A])
-//   dL/dC  = w^T AO               (FP64 MFMA GEMM on the transposed weights,
-//                                  split over points, fixed-order reduction)
+//   G      = w C                  (FP64 MFMA GEMM, [5P][A], C in fragment
+//                                  order)
+//   dL/dC  = (AO^T w)^T           (FP64 MFMA GEMM with AO read K-major and w
+//                                  in fragment order, split over points,
+//                                  fixed-order reduction)
 //   dL/dR, dL/dalpha              (AO adjoint kernel recomputing primitives,
 //                                  per-block partials, fixed-order reduction)
 // There is no counterpart in the reference; the forward arithmetic it
@@ -49,9 +51,18 @@ qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_adjoint_device(qmckl_context_de
         !grow_d(c, &mo.adj_partR, &mo.adj_partR_cap, nblk * S * 3) ||
         !grow_d(c, &mo.adj_partA, &mo.adj_partA_cap, nblk * Np))
       return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
+    if (!mo.adj_cfrag_ok) {  // the coefficients [M][A] as the B operand, fragment order
+      const int64_t len = round_up(A, 64 * f64_frag_cbw(A)) * round_up(M, 16);
+      if (!grow_d(c, &mo.adj_cfrag, &mo.adj_cfrag_cap, len))
+        return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
+      e = launch_frag_b_f64(c->stream, M, A, mo.coefficient, A, mo.adj_cfrag, &mo.adj_cfrag_nk4);
+      if (e != hipSuccess) return hip_check(context, e, fn);
+      mo.adj_cfrag_ok = true;
+    }
     {
       KernelTimer t(c, "adjoint_g_gemm_f64");
-      e = launch_mo_gemm_f64(c->stream, 5 * P, M, A, weight, M, mo.coefficient, A, mo.adj_G, A);
+      e = launch_mo_gemm_f64_frag(c->stream, 5 * P, M, A, weight, M, mo.adj_cfrag, mo.adj_cfrag_nk4, mo.adj_G,
+                                  A);
       t.stop();
     }
     // primitives that belong to no shell keep a zero partial
@@ -72,13 +83,19 @@ qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_adjoint_device(qmckl_context_de
   if (d_mo_coefficient != nullptr) {
     qmckl_exit_code_device rc = qmckl_provide_ao_basis_ao_vgl_device(context);
     if (rc != QMCKL_SUCCESS_DEVICE) return rc;
-    const int splits = gemm_tn_splits(5 * P);
-    if (splits > 1 && !grow_d(c, &mo.adj_partC, &mo.adj_partC_cap, (int64_t)splits * M * A))
+    // dL/dC^T [A][M] = AO^T w: AO [5P][A] read K-major, the weights as the
+    // B operand in fragment order, split over the 5P points and components
+    const int splits = gemm_tn_frag_splits(A, 5 * P);
+    const int64_t wlen = round_up(M, 64 * f64_frag_cbw(M)) * round_up(5 * P, 16);
+    if (!grow_d(c, &mo.adj_partC, &mo.adj_partC_cap, (int64_t)splits * M * A) ||
+        !grow_d(c, &mo.adj_wfrag, &mo.adj_wfrag_cap, wlen))
       return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
-    {
+    int64_t nk4w = 0;
+    e = launch_frag_b_f64(c->stream, 5 * P, M, weight, M, mo.adj_wfrag, &nk4w);
+    if (e == hipSuccess) {
       KernelTimer t(c, "adjoint_dc_gemm_f64");
-      e = launch_gemm_tn_f64(c->stream, M, 5 * P, A, weight, M, c->ao.ao_vgl, A, d_mo_coefficient, A,
-                             mo.adj_partC, splits);
+      e = launch_gemm_tn_f64_frag(c->stream, A, 5 * P, M, c->ao.ao_vgl, A, mo.adj_wfrag, nk4w, d_mo_coefficient,
+                                  mo.adj_partC, splits);
       t.stop();
     }
     if (e == hipSuccess) e = sync(c);

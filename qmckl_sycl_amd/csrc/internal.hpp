@@ -177,6 +177,13 @@ struct MoBasis {
   int64_t adj_partA_cap = 0;
   double *adj_partC = nullptr;
   int64_t adj_partC_cap = 0;
+  // the adjoint's B operands in FP64 fragment order: the coefficients [M][A]
+  // (built on first use after finalize) and the weights [5P][M] (per call)
+  double *adj_cfrag = nullptr;
+  int64_t adj_cfrag_cap = 0, adj_cfrag_nk4 = 0;
+  bool adj_cfrag_ok = false;
+  double *adj_wfrag = nullptr;
+  int64_t adj_wfrag_cap = 0;
   int32_t uninitialized = (1 << 2) - 1;
   bool provided = false;
 };
@@ -308,6 +315,13 @@ hipError_t launch_frag_b_f64(hipStream_t s, int64_t K, int64_t Ncol, const doubl
 hipError_t launch_mo_gemm_f64_frag(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const double *A,
                                    int64_t lda, const double *Bf, int64_t nk4, double *C, int64_t ldc,
                                    const uint8_t *flags = nullptr, int nkb = 0);
+// out_t[Ncol][R] = (A^T Bf)^T with A stored K-major [K][lda] (element (r, k)
+// at A[k*lda + r]) and B in the fragment order of launch_frag_b_f64, split
+// over K into `splits` partials [splits][R][Ncol] reduced in a fixed order.
+int gemm_tn_frag_splits(int64_t R, int64_t K);
+hipError_t launch_gemm_tn_f64_frag(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const double *A,
+                                   int64_t lda, const double *Bf, int64_t nk4, double *out_t,
+                                   double *partial, int splits);
 // C[R][Ncol] = A^T B with A stored [K][lda] (rows of A^T contiguous along
 // K-major storage), split over K into `splits` partials (partial holds
 // splits*R*Ncol doubles) reduced in a fixed order.

@@ -167,11 +167,14 @@ frag_b_f64_kernel(int64_t K, int64_t Ncol, const double *__restrict__ B, int64_t
   }
 }
 
-template <int CBW>
+// TRANSA: the A operand is stored K-major (A[k*lda + row]).  gridDim.z
+// splits K into chunks of kchunk (a multiple of 16); split z writes its
+// partial product to C + z*R*ldc.  flags are for the unsplit provider path.
+template <int CBW, bool TRANSA>
 __global__ void __launch_bounds__(kThreads, 2)
 gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restrict__ A, int64_t lda,
                      const double *__restrict__ Bf, int64_t nk4, double *__restrict__ C, int64_t ldc,
-                     const uint8_t *__restrict__ flags, int nkb) {
+                     const uint8_t *__restrict__ flags, int nkb, int64_t kchunk) {
   __shared__ double As[kBK][kBM + 1];
   __shared__ int s_list[kMaxStages16];
   __shared__ int s_count;
@@ -180,7 +183,10 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
   const int li = lane & 15, lk = lane >> 4;
   const int64_t row0 = (int64_t)blockIdx.x * kBM;
   const int64_t cb0 = (int64_t)blockIdx.y * 4 * CBW + wave;  // this wavefront's first column block
-  const int nst = (int)((K + kBK - 1) / kBK);
+  const int64_t kbeg = (int64_t)blockIdx.z * kchunk;
+  const int64_t kend = kbeg + kchunk < K ? kbeg + kchunk : K;
+  const int nst = kend > kbeg ? (int)((kend - kbeg + kBK - 1) / kBK) : 0;
+  C += (int64_t)blockIdx.z * R * ldc;
 
   int count = nst;
   if (flags != nullptr) {
@@ -200,19 +206,36 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
     count = s_count;
   }
 
-  // A slot: row ar, 4 consecutive K at aq (rows past R read row R-1, K past
-  // the end reads zero)
+  // A slot: row ar, 4 consecutive K at aq (K-major storage: 4 consecutive
+  // rows ar4.. at K aq4); rows past R read zero or row R-1, K past the
+  // chunk reads zero
   const int ar = tid >> 2, aq = (tid & 3) * 4;
-  const double *Ap = A + (row0 + ar < R ? row0 + ar : R - 1) * lda + aq;
+  const int aq4 = tid >> 4, ar4 = (tid & 15) * 4;
+  const double *Ap = TRANSA ? A + row0 + ar4 : A + (row0 + ar < R ? row0 + ar : R - 1) * lda + aq;
   const double *Bp = Bf + cb0 * nk4 * 64 + lane;
   const int64_t cb_stride = 4 * nk4 * 64;  // the wavefront's next column block
 
   double ra[4];
   double b[4][CBW];
-  auto stage_k0 = [&](int s) { return (int64_t)(flags != nullptr ? s_list[s] : s) * kBK; };
+  auto stage_k0 = [&](int s) { return kbeg + (int64_t)(flags != nullptr ? s_list[s] : s) * kBK; };
   auto load_a = [&](int64_t k0) {
+    if constexpr (TRANSA) {
+      const bool kin = k0 + aq4 < kend;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) ra[j] = k0 + aq + j < K ? Ap[k0 + j] : 0.0;
+      for (int j = 0; j < 4; ++j) ra[j] = kin && row0 + ar4 + j < R ? Ap[(k0 + aq4) * lda + j] : 0.0;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) ra[j] = k0 + aq + j < kend ? Ap[k0 + j] : 0.0;
+    }
+  };
+  auto store_a = [&]() {
+    if constexpr (TRANSA) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) As[aq4][ar4 + j] = ra[j];
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) As[aq + j][ar] = ra[j];
+    }
   };
   auto load_b = [&](int64_t k0, int ks) {
 #pragma unroll
@@ -233,8 +256,7 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
   }
   for (int s = 0; s < count; ++s) {
     __syncthreads();  // every wavefront is done reading the previous AO slab
-#pragma unroll
-    for (int j = 0; j < 4; ++j) As[aq + j][ar] = ra[j];
+    store_a();
     __syncthreads();
     const bool more = s + 1 < count;
     const int64_t k1 = more ? stage_k0(s + 1) : 0;
@@ -266,16 +288,45 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
   }
 }
 
-template <int CBW>
+template <int CBW, bool TRANSA>
 hipError_t launch_frag(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const double *A, int64_t lda,
                        const double *Bf, int64_t nk4, double *C, int64_t ldc, const uint8_t *flags,
-                       int nkb) {
+                       int nkb, int splits) {
   const int64_t gx = (R + kBM - 1) / kBM, gy = (Ncol + 64 * CBW - 1) / (64 * CBW);
-  if (gx > 0x7fffffff || gy > 65535) return hipErrorInvalidValue;
-  if (flags != nullptr && ((int64_t)nkb * 32 < K || nkb > kMaxFlagBlocks)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((gemm_f64_frag_kernel<CBW>), dim3((unsigned)gx, (unsigned)gy), dim3(kThreads), 0, s, R,
-                     K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb);
+  if (gx > 0x7fffffff || gy > 65535 || splits < 1 || splits > 65535) return hipErrorInvalidValue;
+  if (flags != nullptr && ((int64_t)nkb * 32 < K || nkb > kMaxFlagBlocks || splits != 1))
+    return hipErrorInvalidValue;
+  const int64_t kchunk = ((K + splits - 1) / splits + kBK - 1) / kBK * kBK;
+  hipLaunchKernelGGL((gemm_f64_frag_kernel<CBW, TRANSA>), dim3((unsigned)gx, (unsigned)gy, (unsigned)splits),
+                     dim3(kThreads), 0, s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, kchunk);
   return hipGetLastError();
+}
+
+template <bool TRANSA>
+hipError_t dispatch_frag(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const double *A, int64_t lda,
+                         const double *Bf, int64_t nk4, double *C, int64_t ldc, const uint8_t *flags, int nkb,
+                         int splits) {
+  switch (f64_frag_cbw(Ncol)) {
+    case 1: return launch_frag<1, TRANSA>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, splits);
+    case 2: return launch_frag<2, TRANSA>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, splits);
+    case 3: return launch_frag<3, TRANSA>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, splits);
+    case 4: return launch_frag<4, TRANSA>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, splits);
+    default: return launch_frag<5, TRANSA>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, splits);
+  }
+}
+
+// out[m][a] = sum_z part[z][a][m], z ascending (fixed order: deterministic)
+__global__ void __launch_bounds__(kThreads)
+sum_splits_t_kernel(int64_t rows, int64_t cols, int splits, const double *__restrict__ part,
+                    double *__restrict__ out) {
+  const int64_t n = rows * cols;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t m = i / rows, a = i - m * rows;  // out index i = m*rows + a
+    double t = 0.0;
+    for (int z = 0; z < splits; ++z) t += part[(int64_t)z * n + a * cols + m];
+    out[i] = t;
+  }
 }
 
 // out[i] = sum_z part[z][i], z ascending (fixed order: deterministic)
@@ -340,13 +391,33 @@ hipError_t launch_mo_gemm_f64_frag(hipStream_t s, int64_t R, int64_t K, int64_t 
                                    const uint8_t *flags, int nkb) {
   if (R <= 0 || Ncol <= 0) return hipSuccess;
   if (K <= 0 || nk4 * 4 < K) return hipErrorInvalidValue;
-  switch (f64_frag_cbw(Ncol)) {
-    case 1: return launch_frag<1>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb);
-    case 2: return launch_frag<2>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb);
-    case 3: return launch_frag<3>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb);
-    case 4: return launch_frag<4>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb);
-    default: return launch_frag<5>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb);
-  }
+  return dispatch_frag<false>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, 1);
+}
+
+int gemm_tn_frag_splits(int64_t R, int64_t K) {
+  // about 2048 workgroups, at least 64 K steps of 16 per split
+  const int64_t tiles = (R + kBM - 1) / kBM;
+  int64_t sp = (2048 + tiles - 1) / tiles;
+  const int64_t maxsp = (K + 1023) / 1024;
+  if (sp > maxsp) sp = maxsp;
+  if (sp < 1) sp = 1;
+  if (sp > 256) sp = 256;
+  return (int)sp;
+}
+
+hipError_t launch_gemm_tn_f64_frag(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const double *A,
+                                   int64_t lda, const double *Bf, int64_t nk4, double *out_t,
+                                   double *partial, int splits) {
+  if (R <= 0 || Ncol <= 0) return hipSuccess;
+  if (K <= 0 || nk4 * 4 < K || splits < 1) return hipErrorInvalidValue;
+  hipError_t e = dispatch_frag<true>(s, R, K, Ncol, A, lda, Bf, nk4, partial, Ncol, nullptr, 0, splits);
+  if (e != hipSuccess) return e;
+  const int64_t n = R * Ncol;
+  int64_t g = (n + kThreads - 1) / kThreads;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(sum_splits_t_kernel, dim3((unsigned)g), dim3(kThreads), 0, s, R, Ncol, splits, partial,
+                     out_t);
+  return hipGetLastError();
 }
 
 }  // namespace qmckl_amd

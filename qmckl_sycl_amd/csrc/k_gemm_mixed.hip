@@ -34,7 +34,6 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 constexpr int kThreads = 256;
 constexpr int kBM = 64;
 constexpr int kKB = 32;
-constexpr int kPad = 8;  // halves of padding per LDS row
 
 // power-of-two scale bringing max |x| into [2^13, 2^14); the exponent is
 // clamped so that the scale and its inverse stay normal floats
@@ -137,7 +136,6 @@ __device__ inline f4 mfma(const uint4 &a, const uint4 &b, f4 c) {
 // 64 x 160 with the column halves on alternate XCDs, FP32 sums over two or
 // four stages before the FP64 addition (1% faster, twice the error), and
 // skipping the MFMAs of zero 32-K halves inside a live stage.
-constexpr int kRM = 64;           // rows per workgroup
 constexpr int kStage = 64;        // K per stage
 constexpr int kLDS = kStage + 8;  // LDS row length in halves (bank spread)
 constexpr int kMaxStages = kMaxFlagBlocks / 2;
@@ -178,7 +176,6 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
                   int64_t ldb, const float *__restrict__ colinv, const uint8_t *__restrict__ flags, int nkb,
                   const int32_t *__restrict__ perm, double *__restrict__ C, int64_t ldc) {
   constexpr int T = Terms<SP>::n;
-  constexpr int NW = WR / 8;       // wavefronts: WR/32 row halves x 4 column quarters
   constexpr int RB = 2;            // 16-row blocks per wavefront
   constexpr int KQ = 8;            // K per thread in the A slab
   __shared__ uint16_t sA[T][WR * kLDS];

@@ -189,6 +189,7 @@ qmckl_exit_code_device qmckl_finalize_mo_basis_device(qmckl_context_device conte
   if (e != hipSuccess) return hip_check(context, e, fn);
   c->mo.mo_vgl_date = c->mo.mo_value_date = 0;
   c->mo.coef_split_form = QMCKL_PRECISION_FP64_DEVICE;  // rebuilt on next mixed use
+  c->mo.adj_cfrag_ok = false;                            // and on next adjoint
   return QMCKL_SUCCESS_DEVICE;
 }
 

@@ -18,9 +18,10 @@
 //   - phase 2: lane = AO of the chunk (its shell, exponents (a,b,c), factor
 //     and nucleus are loaded once), wavefront w walks points w, w+4, ...;
 //     every store of a wavefront is one contiguous run of a [p][c][*] row,
-//     through a pointer advanced by a constant stride.  Nonzero flags are
-//     gathered with wavefront ballots into a scalar bit mask per chunk and
-//     written to LDS once per chunk and wavefront.
+//     through a pointer advanced by a constant stride.  Each lane ORs its
+//     AO's nonzero entries into a bit per 64-row tile; one wavefront ballot
+//     per tile and chunk turns them into the chunk's nonzero flags, written
+//     to LDS once per chunk and wavefront.
 // Two barriers per chunk: phase 1 -> phase 2, and phase 2 -> the next
 // chunk's phase 1 (which overwrites the radial tile).
 // r^2 is fma(x,x,fma(y,y,z*z)), as in the CPU oracle, so the screens decide
@@ -172,7 +173,7 @@ ao_kernel(int64_t P, int N, const double *__restrict__ coord, const double *__re
     const uint64_t bm0 = act_mask & ((1ull << cut0) - 1ull);
     const uint64_t bm1 = act_mask & ~((1ull << cut0) - 1ull) & (cut1 >= 64 ? ~0ull : ((1ull << cut1) - 1ull));
     const uint64_t bm2 = act_mask & (cut1 >= 64 ? 0ull : ~((1ull << cut1) - 1ull));
-    uint32_t fmask = 0;  // bit 3*tile + block: some entry of (row tile, block) is nonzero
+    uint32_t nzt = 0;  // this lane: bit tile = some entry of its AO in 64-row tile `tile` is nonzero
 
     OutT *o = ao.out + (p0 + wave) * NCOMP * ao.ld + k;
     for (int pl = wave; pl < np; pl += kThreads / 64, o += ostep) {
@@ -194,19 +195,24 @@ ao_kernel(int64_t P, int N, const double *__restrict__ coord, const double *__re
       }
       if (track_flags) {
 #pragma unroll
-        for (int c = 0; c < NCOMP; ++c) {
-          const uint64_t nzm = __ballot(w[c] != (OutT)0);
-          const int tile = (pl * NCOMP + c) >> 6;
-          fmask |= ((nzm & bm0) ? 1u : 0u) << (3 * tile);
-          fmask |= ((nzm & bm1) ? 2u : 0u) << (3 * tile);
-          fmask |= ((nzm & bm2) ? 4u : 0u) << (3 * tile);
-        }
+        for (int c = 0; c < NCOMP; ++c) nzt |= (w[c] != (OutT)0 ? 1u : 0u) << ((pl * NCOMP + c) >> 6);
       }
     }
-    if (track_flags && lane == 0 && fmask != 0) {
-      for (int tile = 0; tile < NCOMP; ++tile)
-        for (int b = 0; b < 3; ++b)
-          if (((fmask >> (3 * tile + b)) & 1u) && kb0 + b < ao.nkb) s_flag[tile * ao.nkb + kb0 + b] = 1;
+    if (track_flags) {
+      // one ballot per row tile per chunk: which of the chunk's (up to three)
+      // 32-AO blocks hold a nonzero entry of that tile
+      uint32_t fmask = 0;  // bit 3*tile + block
+#pragma unroll
+      for (int tile = 0; tile < NCOMP; ++tile) {
+        const uint64_t nzm = __ballot(act && ((nzt >> tile) & 1u));
+        fmask |= ((nzm & bm0) ? 1u : 0u) << (3 * tile);
+        fmask |= ((nzm & bm1) ? 2u : 0u) << (3 * tile);
+        fmask |= ((nzm & bm2) ? 4u : 0u) << (3 * tile);
+      }
+      if (lane == 0 && fmask != 0)
+        for (int tile = 0; tile < NCOMP; ++tile)
+          for (int b = 0; b < 3; ++b)
+            if (((fmask >> (3 * tile + b)) & 1u) && kb0 + b < ao.nkb) s_flag[tile * ao.nkb + kb0 + b] = 1;
     }
     __syncthreads();  // phase 2 is done with the radial tile
   }

@@ -139,6 +139,23 @@ def test_tile_choices_and_autotune_keep_results(form):
         assert np.array_equal(c.mo_vgl(P), ref)
 
 
+def test_more_mos_than_one_workgroup_covers(oracle_lib):
+    """mo_num = 450 needs two column tiles in every contraction (320 MOs per
+    workgroup); 8 waters, 4 walkers (320 points), all points checked."""
+    s = S.water_cluster(n_side=2, walk_num=4, mo_num=450, name="water8_450mo")
+    ref = oracle_lib.mo_vgl(s)
+    with Context(0) as c:
+        c.set_system(s)
+        for form in FORMS:
+            e = row_err(run_form(c, form, s.point_num), ref)
+            say(f"water8 450 MOs {form}: {e}")
+            assert max(e.values()) <= BAR[form], (form, e)
+        c.call("qmckl_set_mo_basis_precision_device", 0)
+        e = row_err(c.mo_vgl(s.point_num), ref)
+        say(f"water8 450 MOs fp64: {e}")
+        assert max(e.values()) <= 1e-12, e
+
+
 def test_switching_forms_recomputes(chb):
     with Context(0) as c:
         c.set_system(chb)

@@ -408,7 +408,7 @@ qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_batch_device(int64_t count,
  * the AO operand kernel aims at (group_waves 0 automatic, or 1..1024).
  * set: ARG_2 / ARG_3 for values out of range.  get: ARG_2 / ARG_3 for NULL.
  * autotune: times the context's mixed chain on its current points for
- * point_order 0/1 x group_waves 16/32/64/128 and keeps the fastest (a
+ * point_order 0/1 x group_waves 16/32/64/128/256 and keeps the fastest (a
  * process-wide cache reuses the choice for contexts of the same device,
  * point, AO, MO and nucleus counts and form); FP64 form: nothing to choose.
  * Needs the basis and points (NOT_PROVIDED otherwise).  It overwrites the

@@ -318,10 +318,10 @@ int ao_rows_groups(int64_t P, int64_t ld, int group_waves) {
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
     cus = 256;
   const int64_t npt = (P + 63) / 64, nst = ld / 64;
-  // default 64 wavefronts per CU (16 per SIMD: measured on water64 at 128
-  // walkers, 1.31 ms at 16, 1.23 at 32, 1.14 at 64, 1.08 at 128 but with a
-  // slower contraction after it)
-  const int64_t want = (group_waves > 0 ? group_waves : 64) * (int64_t)cus;
+  // default 256 wavefronts per CU (measured on water64 at 128 walkers with
+  // ordered tiles: 1.29 ms at 16, 1.25 at 32, 1.15 at 64, 1.09 at 128, 1.06
+  // at 256; the contraction after it is unaffected)
+  const int64_t want = (group_waves > 0 ? group_waves : 256) * (int64_t)cus;
   int64_t g = (want + npt - 1) / npt;
   if (g > nst) g = nst;
   if (g < 1) g = 1;

@@ -569,9 +569,9 @@ qmckl_exit_code_device qmckl_mo_basis_autotune_device(qmckl_context_device conte
   if (e == hipSuccess) e = hipEventCreate(&e1);
   const bool can_order = round_up(c->ao.ao_num, 64) / 32 <= kMaxFlagBlocks;
   float best = -1.0f;
-  int32_t best_order = 0, best_waves = 64;
+  int32_t best_order = 0, best_waves = 256;
   for (int32_t order = 0; order <= (can_order ? 1 : 0) && e == hipSuccess && rc == QMCKL_SUCCESS_DEVICE; ++order)
-    for (int32_t waves : {16, 32, 64, 128}) {
+    for (int32_t waves : {16, 32, 64, 128, 256}) {
       mo.tile_point_order = order;
       mo.tile_group_waves = waves;
       rc = compute_mo_vgl_mixed(context, c, mo.mo_vgl, fn);  // warm-up

@@ -3,11 +3,10 @@
 //
 // en_distance[p][n] = |r_p - R_n| (src/qmckl_electron.cpp:302-331 calls
 // qmckl_distance_device('T','T') whose kernel is at 536-549; here the plain
-// Euclidean distance).  Coordinates are structure-of-arrays: point x, y, z
-// rows of length P are read with unit stride, the nucleus table (3N doubles)
-// is staged in LDS once per workgroup, and each thread writes consecutive n
-// of one point row so stores are contiguous.  The kernel is HBM-bound:
-// 24 B read + 8N B written per point.
+// Euclidean distance).  Coordinates are structure-of-arrays, the nucleus
+// table (3N doubles) is staged in LDS once per workgroup of 64 points, and
+// each workgroup's stores cover one contiguous run of the output.  The
+// kernel is HBM-bound: 24 B read + 8N B written per point.
 #include <hip/hip_runtime.h>
 
 #include "internal.hpp"
@@ -69,6 +68,11 @@ __global__ void __launch_bounds__(kThreads) strided_copy_kernel(int64_t rows, in
 
 constexpr int kMaxNuclLds = 2048;  // nuclei staged in LDS (at most 48 KiB)
 
+// One workgroup per kEnPoints consecutive points: it stages the nuclei in
+// LDS once and writes its points' rows, a contiguous run of the output,
+// thread t taking elements t, t + 256, ... of the run (the (point, nucleus)
+// pair advanced incrementally, no division in the loop).
+constexpr int kEnPoints = 64;
 __global__ void __launch_bounds__(kThreads) en_distance_kernel(int64_t P, int64_t N,
                                                                const double *__restrict__ coord,
                                                                const double *__restrict__ nucl,
@@ -80,14 +84,20 @@ __global__ void __launch_bounds__(kThreads) en_distance_kernel(int64_t P, int64_
     __syncthreads();
   }
   const double *R = staged ? snucl : nucl;
-  const int64_t total = P * N;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t p = i / N, n = i - p * N;
+  const int64_t pb = (int64_t)blockIdx.x * kEnPoints;
+  const int64_t pe = pb + kEnPoints < P ? pb + kEnPoints : P;
+  const int64_t n_end = (pe - pb) * N;  // elements of this workgroup's run
+  int64_t p = pb + threadIdx.x / N, n = threadIdx.x % N;
+  for (int64_t i = threadIdx.x; i < n_end; i += kThreads) {
     const double x = coord[p] - R[n];
     const double y = coord[P + p] - R[N + n];
     const double z = coord[2 * P + p] - R[2 * N + n];
-    out[i] = sqrt(fma(x, x, fma(y, y, z * z)));
+    out[pb * N + i] = sqrt(fma(x, x, fma(y, y, z * z)));
+    n += kThreads;
+    while (n >= N) {
+      n -= N;
+      ++p;
+    }
   }
 }
 
@@ -276,8 +286,10 @@ hipError_t launch_en_distance(hipStream_t s, int64_t P, int64_t N, const double 
                               const double *nucl, double *out) {
   if (P <= 0 || N <= 0) return hipSuccess;
   const size_t lds = N <= kMaxNuclLds ? sizeof(double) * 3 * (size_t)N : 0;
-  hipLaunchKernelGGL(en_distance_kernel, dim3(grid_for(P * N)), dim3(kThreads), lds, s, P, N, coord,
-                     nucl, out);
+  const int64_t blocks = (P + kEnPoints - 1) / kEnPoints;
+  if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(en_distance_kernel, dim3((unsigned)blocks), dim3(kThreads), lds, s, P, N, coord, nucl,
+                     out);
   return hipGetLastError();
 }
 

@@ -322,13 +322,6 @@ int gemm_tn_frag_splits(int64_t R, int64_t K);
 hipError_t launch_gemm_tn_f64_frag(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const double *A,
                                    int64_t lda, const double *Bf, int64_t nk4, double *out_t,
                                    double *partial, int splits);
-// C[R][Ncol] = A^T B with A stored [K][lda] (rows of A^T contiguous along
-// K-major storage), split over K into `splits` partials (partial holds
-// splits*R*Ncol doubles) reduced in a fixed order.
-int gemm_tn_splits(int64_t K);
-hipError_t launch_gemm_tn_f64(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const double *A,
-                              int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc,
-                              double *partial, int splits);
 hipError_t launch_ao_adjoint(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
                              const double *nucl, const double *ao_factor, const double *G,
                              double *partR, double *partA);

@@ -16,10 +16,10 @@
 // for mo_num <= 320 each AO row is read from HBM once.  With nonzero flags
 // from the AO kernel, K tiles whose AOs are all screened on the 64 rows are
 // skipped (adding exact zeros changes nothing).
-// That kernel serves the low-level launchers and the adjoint's GEMMs; the
-// mo_vgl / mo_value providers use gemm_f64_frag_kernel below, which takes
-// the coefficients prepared at MO finalize in MFMA fragment order (36.0 ->
-// 14.3 ms at 409,600 x 4,160 x 320 on MI355X).
+// That kernel serves the low-level launchers (coefficients given as a plain
+// [A][M] array); the mo_vgl / mo_value providers and the adjoint use
+// gemm_f64_frag_kernel below, which takes its B operand in MFMA fragment
+// order (36.0 -> 14.3 ms at 409,600 x 4,160 x 320 on MI355X).
 #include <hip/hip_runtime.h>
 
 #include "internal.hpp"
@@ -329,43 +329,12 @@ sum_splits_t_kernel(int64_t rows, int64_t cols, int splits, const double *__rest
   }
 }
 
-// out[i] = sum_z part[z][i], z ascending (fixed order: deterministic)
-__global__ void __launch_bounds__(kThreads)
-sum_splits_kernel(int64_t n, int splits, const double *__restrict__ part, double *__restrict__ out) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    double t = 0.0;
-    for (int z = 0; z < splits; ++z) t += part[(int64_t)z * n + i];
-    out[i] = t;
-  }
-}
-
 }  // namespace
 
 hipError_t launch_mo_gemm_f64(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const double *A,
                               int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc,
                               const uint8_t *flags, int nkb) {
   return dispatch<false>(s, R, K, Ncol, A, lda, B, ldb, C, ldc, flags, nkb, 1);
-}
-
-int gemm_tn_splits(int64_t K) {
-  int64_t z = K / 2048;
-  return (int)(z < 1 ? 1 : (z > 64 ? 64 : z));
-}
-
-hipError_t launch_gemm_tn_f64(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const double *A,
-                              int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc,
-                              double *partial, int splits) {
-  if (R <= 0 || Ncol <= 0) return hipSuccess;
-  if (ldc != Ncol) return hipErrorInvalidValue;
-  if (splits <= 1) return dispatch<true>(s, R, K, Ncol, A, lda, B, ldb, C, ldc, nullptr, 0, 1);
-  hipError_t e = dispatch<true>(s, R, K, Ncol, A, lda, B, ldb, partial, ldc, nullptr, 0, splits);
-  if (e != hipSuccess) return e;
-  const int64_t n = R * Ncol;
-  int64_t g = (n + kThreads - 1) / kThreads;
-  if (g > 65536) g = 65536;
-  hipLaunchKernelGGL(sum_splits_kernel, dim3((unsigned)g), dim3(kThreads), 0, s, n, splits, partial, C);
-  return hipGetLastError();
 }
 
 int f64_frag_cbw(int64_t Ncol) {

@@ -194,10 +194,11 @@ KERNEL_SYMBOL = {"en_distance": "en_distance_kernel", "ao_vgl": "ao_kernel<5, fa
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_{workload}.json")
 
 
-def measured_traffic(s, kernel):
-    """HBM bytes per launch of a kernel on this workload, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE measurement (bench.py --traffic), or
-    None when there is none."""
+def measured(s, kernel, key="bytes_per_launch"):
+    """A per-launch figure of a kernel on this workload from the committed
+    rocprofv3 counter measurement (bench.py --traffic): HBM bytes
+    (FETCH_SIZE + WRITE_SIZE, calibrated) or issued MFMA flops; None when
+    there is none."""
     path = TRAFFIC_FILE.format(workload=f"{s.name}_{s.walk_num}w")
     try:
         with open(path) as f:
@@ -205,7 +206,7 @@ def measured_traffic(s, kernel):
     except (OSError, ValueError):
         return None
     k = rec.get("kernels", {}).get(kernel)
-    return None if k is None else k.get("bytes_per_launch")
+    return None if k is None else k.get(key)
 
 
 def bench_adjoint(steps, warmup):
@@ -263,38 +264,53 @@ def roofline(kernels, s):
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
     kind, work = kernel_work(dom, s)
     avg_s = kernels[dom]["avg_ms"] / 1e3
-    traffic = measured_traffic(s, dom)
+    traffic = measured(s, dom)
     if kind == "mfma":
         prec, peak = GEMM_PEAK[dom]
         ach = work / avg_s / 1e12
-        return {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
-                "frac": ach / peak, "traffic": traffic, "kernel": dom, "peak_dtype": prec,
-                "mfma_products_per_fma": PRODUCTS[dom]}
+        rec = {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
+               "frac": ach / peak, "traffic": traffic, "kernel": dom, "peak_dtype": prec,
+               "mfma_products_per_fma": PRODUCTS[dom]}
+        # The kernels skip exact-zero blocks (screened AOs, zero coefficient
+        # blocks), so the dense work above can exceed what the matrix cores
+        # run; the MFMA flops the kernel issued (SQ_INSTS_VALU_MFMA_MOPS_*
+        # x 512, all split products included) give its utilisation.
+        issued = measured(s, dom, "mfma_flops_per_launch")
+        if issued:
+            rec["issued"] = {"flops_per_launch": issued, "achieved": issued / avg_s / 1e12,
+                             "frac": issued / avg_s / 1e12 / peak,
+                             "dense_fraction": issued / (work * PRODUCTS[dom])}
+        return rec
     ach = work / avg_s / 1e9
     return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom}
 
 
-def _pmc_per_kernel(counter, argv):
-    """Runs rocprofv3 --pmc <counter> over a child python process running
-    this file with argv; returns {kernel name: mean value per dispatch}."""
+def _pmc_per_kernel(counters, argv):
+    """Runs rocprofv3 --pmc <counters> (one pass) over a child python
+    process running this file with argv; returns {counter: {kernel name:
+    mean value per dispatch}}."""
     import csv
     import subprocess
     import tempfile
+    counters = counters.split()
     out = tempfile.mkdtemp(prefix="qmckl_pmc_")
-    cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "run", "--",
-           sys.executable, os.path.abspath(__file__)] + argv
+    cmd = ["rocprofv3", "--pmc"] + counters + ["--output-format", "csv", "-d", out, "-o", "run", "--",
+                                               sys.executable, os.path.abspath(__file__)] + argv
     log("running", " ".join(cmd))
     subprocess.run(cmd, check=True, timeout=300, stdout=sys.stderr, stderr=sys.stderr)
-    vals = {}
+    vals = {c: {} for c in counters}
     for dirpath, _, files in os.walk(out):
         for fn in files:
             if fn.endswith("counter_collection.csv"):
                 with open(os.path.join(dirpath, fn)) as f:
                     for r in csv.DictReader(f):
-                        if r["Counter_Name"] == counter:
-                            vals.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
-    return {k: sum(v) / len(v) for k, v in vals.items()}
+                        if r["Counter_Name"] in vals:
+                            # one dispatch can appear once per counter
+                            d = vals[r["Counter_Name"]].setdefault(r["Kernel_Name"], {})
+                            key = r.get("Dispatch_Id", len(d))
+                            d[key] = d.get(key, 0.0) + float(r["Counter_Value"])
+    return {c: {k: sum(v.values()) / len(v) for k, v in per.items()} for c, per in vals.items()}
 
 
 def traffic_profile(args):
@@ -309,25 +325,37 @@ def traffic_profile(args):
     copy_bytes = 1024 * (1 << 20)
     factors, per = {}, {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
-        c = _pmc_per_kernel(counter, cal)
+        c = _pmc_per_kernel(counter, cal)[counter]
         copy_kib = max(v for k, v in c.items() if "copy" in k.lower() or "elementwise" in k)
         factors[counter] = copy_bytes / (copy_kib * 1024.0)
-        per[counter] = _pmc_per_kernel(counter, run)
+        per.update(_pmc_per_kernel(counter, run))
+    mops = ("SQ_INSTS_VALU_MFMA_MOPS_F64", "SQ_INSTS_VALU_MFMA_MOPS_F16", "SQ_INSTS_VALU_MFMA_MOPS_BF16",
+            "SQ_INSTS_VALU_MFMA_MOPS_F32")
+    per.update(_pmc_per_kernel(" ".join(mops), run))
+
+    def mean_of(counter, sym):
+        hits = [v for k, v in per[counter].items() if sym in k.replace("(anonymous namespace)::", "")]
+        return sum(hits) / len(hits) if hits else None
+
     kernels = {}
     for timer, sym in KERNEL_SYMBOL.items():
         rec = {}
         for counter in ("FETCH_SIZE", "WRITE_SIZE"):
-            hits = [v for k, v in per[counter].items() if sym in k.replace("(anonymous namespace)::", "")]
-            if hits:
-                rec[counter.lower() + "_bytes"] = sum(hits) / len(hits) * 1024.0 * factors[counter]
+            v = mean_of(counter, sym)
+            if v is not None:
+                rec[counter.lower() + "_bytes"] = v * 1024.0 * factors[counter]
         if len(rec) == 2:
             rec["bytes_per_launch"] = rec["fetch_size_bytes"] + rec["write_size_bytes"]
             kind, work = kernel_work(timer, s)
             if kind == "hbm":
                 rec["algorithmic_bytes_per_launch"] = work
+            else:
+                rec["mfma_flops_per_launch"] = 512.0 * sum(mean_of(c, sym) or 0.0 for c in mops)
+                rec["algorithmic_flops_per_launch"] = work
             kernels[timer] = rec
     out = {"workload": f"{s.name}_{s.walk_num}w", "form": args.form,
            "counter_scale": factors,
+           "mfma_counters": list(mops),
            "command": "python bench.py --traffic " + " ".join(run[6:]),
            "kernels": kernels}
     path = TRAFFIC_FILE.format(workload=out["workload"])

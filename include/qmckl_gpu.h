@@ -419,6 +419,20 @@ qmckl_exit_code_device qmckl_get_mo_basis_tile_config_device(qmckl_context_devic
                                                              int32_t *group_waves);
 qmckl_exit_code_device qmckl_mo_basis_autotune_device(qmckl_context_device context);
 
+/* ---- MO column plan (new; no counterpart in the reference) ----------------
+ * At MO finalize the library maps the coefficients' zeros in blocks of 16
+ * AOs and orders the MO columns of its prepared coefficient copies in groups
+ * of 16 that share zero blocks; the contraction kernels (FP64 and the split
+ * forms) skip those blocks and store every column at its MO's place, so the
+ * order changes no result.  The plan is kept when at most 90% of the
+ * (16-MO, 16-AO) blocks stay live, else the MO order is used as given.
+ * col_mo (may be NULL): receives mo_num entries, the MO of each prepared
+ * column (0, 1, ... without a plan); live (may be NULL): the fraction of
+ * blocks left live by the planned order (kept or not).
+ * NOT_PROVIDED before the MO basis is finalized. */
+qmckl_exit_code_device qmckl_get_mo_basis_column_plan_device(qmckl_context_device context, int32_t *col_mo,
+                                                             double *live);
+
 /* ---- batching form as a HIP graph (new; no counterpart in the reference) --
  * create: runs the batch above once (same arguments and errors), then
  * records the same work of all count contexts (every kernel of every

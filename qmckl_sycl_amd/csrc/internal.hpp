@@ -143,12 +143,44 @@ struct AoBasis {
   bool provided = false;
 };
 
+// Column plan of the MO contraction.  Column j of the prepared coefficient
+// copies holds MO col_mo[j] (null: MO j), so MOs whose nonzero coefficients
+// sit on the same AOs share 16-column blocks.  The order only permutes the
+// MOs within each group of `group` consecutive MOs (the columns of one
+// workgroup of the kernel that uses it), so a workgroup still owns one MO
+// range and writes it as contiguous runs.  Bit s of blk[b * nw + s / 64] is
+// set when column block b (columns [16b, 16b + 16)) has a nonzero
+// coefficient among the AOs [16s, 16s + 16); blocks >= nblk hold no MO.
+// blk null: every block is live.
+struct ColPlan {
+  const int32_t *col_mo = nullptr;
+  const uint64_t *blk = nullptr;
+  int64_t nw = 0;
+  int64_t nblk = 0;
+};
+
+// A column plan kept by the MO basis for one group width (device arrays;
+// col_mo null when the planned order leaves too many blocks live).
+struct ColPlanSlot {
+  int64_t group = 0;  // 0: unused
+  int32_t *col_mo = nullptr;
+  uint64_t *blk = nullptr;
+  double live = 1.0;  // live (16-MO, 16-AO) blocks of the planned order over all of them
+};
+
 struct MoBasis {
   int64_t mo_num = 0;
   double *coefficient = nullptr;    // [M][A]
   double *coefficient_t = nullptr;  // [A][M]
-  double *coef_frag = nullptr;      // coefficient_t in FP64 MFMA fragment order
+  double *coef_frag = nullptr;      // coefficient_t in FP64 MFMA fragment order (planned column order)
   int64_t coef_frag_nk4 = 0;
+  // nonzero map of the coefficients made at finalize (host, [M][nz_nw]
+  // words of 16-AO blocks) and the column plans made from it: slot 0 for
+  // the FP64 contraction's workgroup width, slot 1 for another width of the
+  // split forms
+  std::vector<uint64_t> nz_words;
+  int64_t nz_nw = 0;
+  ColPlanSlot col_plans[2];
   double *mo_vgl = nullptr;         // [P][5][M]
   int64_t mo_vgl_cap = 0;
   uint64_t mo_vgl_date = 0;
@@ -310,11 +342,29 @@ hipError_t launch_mo_gemm_f64(hipStream_t s, int64_t rows, int64_t K, int64_t Nc
 // Bf holds (Ncol rounded up to 64 * f64_frag_cbw(Ncol)) x (K rounded up to
 // 16) doubles; *nk4 receives the K steps of 4 per column block.
 int f64_frag_cbw(int64_t Ncol);
+// Column j of Bf takes column col_mo[j] of B (null: column j).
 hipError_t launch_frag_b_f64(hipStream_t s, int64_t K, int64_t Ncol, const double *B, int64_t ldb,
-                             double *Bf, int64_t *nk4);
+                             double *Bf, int64_t *nk4, const int32_t *col_mo = nullptr);
+// With a column plan, Bf is in the plan's column order (launch_frag_b_f64
+// with plan.col_mo), column j of the product is stored at C column
+// plan.col_mo[j], and the MFMAs and coefficient loads of (column block,
+// 16-AO block) pairs whose coefficients are all zero are skipped.
 hipError_t launch_mo_gemm_f64_frag(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const double *A,
                                    int64_t lda, const double *Bf, int64_t nk4, double *C, int64_t ldc,
-                                   const uint8_t *flags = nullptr, int nkb = 0);
+                                   const uint8_t *flags = nullptr, int nkb = 0, ColPlan plan = ColPlan());
+// Nonzero map of coef [M][K] at 16-AO granularity into words [M][nw],
+// nw = ceil(ceil(K / 16) / 64): bit s of words[m * nw + s / 64] is set when
+// MO m has a nonzero (or NaN) coefficient among the AOs [16s, 16s + 16).
+hipError_t launch_coef_blocks(hipStream_t s, int64_t M, int64_t K, const double *coef, int64_t nw,
+                              uint64_t *words);
+// Column order from that map (mo_plan.cpp) for the M MOs whose words start
+// at `words`: the MOs in groups of 16 that each touch few AO blocks, the
+// groups chained so that neighbours touch similar blocks.  Fills col_mo [M]
+// (indices into those M MOs) and blk [ceil(M/16)][nw] and returns the
+// number of live (column block, AO block) pairs over nkb16 = ceil(K/16) AO
+// blocks per column block.
+int64_t plan_mo_columns(const uint64_t *words, int64_t M, int64_t nw, int64_t nkb16,
+                        std::vector<int32_t> &col_mo, std::vector<uint64_t> &blk);
 // out_t[Ncol][R] = (A^T Bf)^T with A stored K-major [K][lda] (element (r, k)
 // at A[k*lda + r]) and B in the fragment order of launch_frag_b_f64, split
 // over K into `splits` partials [splits][R][Ncol] reduced in a fixed order.
@@ -345,15 +395,19 @@ inline int64_t split_cols_padded(int form, int64_t M) {
   if (form == QMCKL_AMD_SPLIT_FP32) return M;
   return round_up(M, split_wg_cols(form, M));
 }
+// Prepared column j (and colinv[j]) holds MO col_mo[j] (null: MO j); the
+// FP32 form takes no column map.
 hipError_t launch_split_coef(hipStream_t s, int form, int64_t M, int64_t K, const double *coef,
-                             int64_t ldb, uint16_t *B, float *colinv);
+                             int64_t ldb, uint16_t *B, float *colinv, const int32_t *col_mo = nullptr);
 // Mixed MO contraction of the blocked AO operand (launch_ao_rows_f32) of P
 // points with the prepared coefficients into mo_vgl C [P][5][Ncol] (ldc);
-// tile row i holds point perm[i] (null: point i).
+// tile row i holds point perm[i] (null: point i).  The split forms take a
+// column plan as launch_mo_gemm_f64_frag does (the coefficients prepared
+// with its col_mo); the FP32 form takes none.
 hipError_t launch_mo_gemm_mixed(hipStream_t s, int form, int64_t P, int64_t K, int64_t Ncol,
                                 const float *A, int64_t lda, const float *rowmax, const void *B, int64_t ldb,
                                 const float *colinv, const uint8_t *flags, int nkb, const int32_t *perm,
-                                double *C, int64_t ldc);
+                                double *C, int64_t ldc, ColPlan plan = ColPlan());
 hipError_t launch_strided_copy(hipStream_t s, int64_t rows, int64_t cols, const double *in,
                                int64_t ld_in, double *out, int64_t ld_out);
 

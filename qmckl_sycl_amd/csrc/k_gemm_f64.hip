@@ -149,22 +149,44 @@ hipError_t dispatch(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const dou
 // owns 64 rows x 64*CBW columns (wavefront w: column blocks w, w+4, ...);
 // K advances in stages of 16 with the next stage's AO slab and coefficient
 // fragments requested while the current stage computes; stages whose 32-AO
-// block is screened on all 64 rows are dropped from the stage list up front
-// (adding exact zeros changes nothing, so results equal the dense kernel's
-// bit for bit).
+// block is screened on all 64 rows are dropped from the stage list up front.
+// With a column plan each listed stage also carries the live bits of the
+// workgroup's column blocks: a wavefront issues neither the loads nor the
+// MFMAs of a column block whose coefficients are zero on the stage's 16
+// AOs, and a stage with no live block is dropped.  Adding exact zeros
+// changes nothing, so results equal the dense kernel's bit for bit; the
+// plan's column order only moves whole columns.
 constexpr int kMaxStages16 = 2 * kMaxFlagBlocks;
+constexpr int kOutRows = 8;  // output rows per pass of the planned-order epilogue
 
 __global__ void __launch_bounds__(kThreads)
 frag_b_f64_kernel(int64_t K, int64_t Ncol, const double *__restrict__ B, int64_t ldb, int64_t nk4,
-                  int64_t total, double *__restrict__ Bf) {
+                  int64_t total, const int32_t *__restrict__ col_mo, double *__restrict__ Bf) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t frag = i >> 6;
     const int l = (int)(i & 63);
     const int64_t cb = frag / nk4, ks = frag - cb * nk4;
     const int64_t k = 4 * ks + l / 16, col = 16 * cb + l % 16;
-    Bf[i] = (k < K && col < Ncol) ? B[k * ldb + col] : 0.0;
+    Bf[i] = (k < K && col < Ncol) ? B[k * ldb + (col_mo != nullptr ? (int64_t)col_mo[col] : col)] : 0.0;
   }
+}
+
+// one wavefront per (MO, 64 AO blocks of 16): lane s tests its 16 AOs
+__global__ void __launch_bounds__(kThreads)
+coef_blocks_kernel(int64_t M, int64_t K, const double *__restrict__ coef, int64_t nw,
+                   uint64_t *__restrict__ words) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wid = (int64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  if (wid >= M * nw) return;  // whole wavefronts
+  const int64_t m = wid / nw, s = (wid - m * nw) * 64 + lane;
+  bool nz = false;
+  for (int j = 0; j < 16; ++j) {
+    const int64_t k = 16 * s + j;
+    if (k < K && !(coef[m * K + k] == 0.0)) nz = true;
+  }
+  const uint64_t b = __ballot(nz);
+  if (lane == 0) words[wid] = b;
 }
 
 // TRANSA: the A operand is stored K-major (A[k*lda + row]).  gridDim.z
@@ -174,30 +196,57 @@ template <int CBW, bool TRANSA>
 __global__ void __launch_bounds__(kThreads, 2)
 gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restrict__ A, int64_t lda,
                      const double *__restrict__ Bf, int64_t nk4, double *__restrict__ C, int64_t ldc,
-                     const uint8_t *__restrict__ flags, int nkb, int64_t kchunk) {
+                     const uint8_t *__restrict__ flags, int nkb, int64_t kchunk, ColPlan plan) {
   __shared__ double As[kBK][kBM + 1];
   __shared__ int s_list[kMaxStages16];
+  __shared__ uint32_t s_live[kMaxStages16];
   __shared__ int s_count;
+  __shared__ double s_out[TRANSA ? 1 : kOutRows * 64 * CBW];  // output rows in MO order (planned columns)
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wv = __builtin_amdgcn_readfirstlane(wave);
   const int li = lane & 15, lk = lane >> 4;
   const int64_t row0 = (int64_t)blockIdx.x * kBM;
-  const int64_t cb0 = (int64_t)blockIdx.y * 4 * CBW + wave;  // this wavefront's first column block
+  const int64_t gb0 = (int64_t)blockIdx.y * 4 * CBW;  // the workgroup's first column block
+  const int64_t cb0 = gb0 + wv;                       // this wavefront's first column block
   const int64_t kbeg = (int64_t)blockIdx.z * kchunk;
   const int64_t kend = kbeg + kchunk < K ? kbeg + kchunk : K;
   const int nst = kend > kbeg ? (int)((kend - kbeg + kBK - 1) / kBK) : 0;
   C += (int64_t)blockIdx.z * R * ldc;
 
+  // stage list: live stages and, per stage, bit b for live column block gb0 + b
+  const bool listed = flags != nullptr || plan.blk != nullptr;
   int count = nst;
-  if (flags != nullptr) {
+  if (listed) {
     if (wave == 0) {
-      const uint8_t *f = flags + (int64_t)blockIdx.x * nkb;
+      const uint8_t *f = flags != nullptr ? flags + (int64_t)blockIdx.x * nkb : nullptr;
+      const int64_t g0 = kbeg / kBK;  // 16-AO block of the chunk's first stage
       int base = 0;
       for (int s0 = 0; s0 < nst; s0 += 64) {
         const int st = s0 + lane;
-        const bool live = st < nst && f[st >> 1] != 0;
+        bool live = st < nst && (f == nullptr || f[st >> 1] != 0);
+        uint32_t lv = 0xffffffffu;
+        if (live && plan.blk != nullptr) {
+          // all loads first (independent, no branches), then the bits
+          const int64_t g = g0 + st;
+          uint64_t w[4 * CBW];
+#pragma unroll
+          for (int b = 0; b < 4 * CBW; ++b) {
+            const int64_t blk = gb0 + b < plan.nblk ? gb0 + b : plan.nblk - 1;
+            w[b] = plan.blk[blk * plan.nw + (g >> 6)];
+          }
+          lv = 0;
+#pragma unroll
+          for (int b = 0; b < 4 * CBW; ++b)
+            lv |= (gb0 + b < plan.nblk ? (uint32_t)((w[b] >> (g & 63)) & 1ull) : 0u) << b;
+          live = lv != 0;
+        }
         const uint64_t m = __ballot(live);
-        if (live) s_list[base + __popcll(m & ((1ull << lane) - 1ull))] = st;
+        if (live) {
+          const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
+          s_list[pos] = st;
+          s_live[pos] = lv;
+        }
         base += __popcll(m);
       }
       if (lane == 0) s_count = base;
@@ -205,6 +254,10 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
     __syncthreads();
     count = s_count;
   }
+  // this wavefront's live column blocks at stage s: bit 4j for block cb0 + 4j
+  auto wave_live = [&](int s) -> uint32_t {
+    return listed ? (uint32_t)__builtin_amdgcn_readfirstlane(s_live[s]) >> wv : 0xffffffffu;
+  };
 
   // A slot: row ar, 4 consecutive K at aq (K-major storage: 4 consecutive
   // rows ar4.. at K aq4); rows past R read zero or row R-1, K past the
@@ -217,7 +270,9 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
 
   double ra[4];
   double b[4][CBW];
-  auto stage_k0 = [&](int s) { return kbeg + (int64_t)(flags != nullptr ? s_list[s] : s) * kBK; };
+  auto stage_k0 = [&](int s) {
+    return kbeg + (int64_t)__builtin_amdgcn_readfirstlane(listed ? s_list[s] : s) * kBK;
+  };
   auto load_a = [&](int64_t k0) {
     if constexpr (TRANSA) {
       const bool kin = k0 + aq4 < kend;
@@ -237,9 +292,27 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
       for (int j = 0; j < 4; ++j) As[aq + j][ar] = ra[j];
     }
   };
-  auto load_b = [&](int64_t k0, int ks) {
+  // The forward product loads its fragments through a buffer resource over
+  // the workgroup's 4*CBW column blocks (lane offset 8*l bytes, block and
+  // step offset in a scalar register).  A dead block gets a lane offset
+  // past the end of the buffer, so its load returns zero without a memory
+  // access, and every stage issues the same loads (the waits on them stay
+  // exact).
+  const int blk_bytes = (int)(nk4 * 512);
+  const __amdgpu_buffer_rsrc_t brs =
+      __builtin_amdgcn_make_buffer_rsrc((void *)(Bf + gb0 * nk4 * 64), (short)0, TRANSA ? 0 : 4 * CBW * blk_bytes,
+                                        0x00020000);
+  auto load_b = [&](int64_t k0, int ks, uint32_t live) {
 #pragma unroll
-    for (int cb = 0; cb < CBW; ++cb) b[ks][cb] = Bp[cb * cb_stride + (k0 / 4 + ks) * 64];
+    for (int cb = 0; cb < CBW; ++cb) {
+      if constexpr (TRANSA) {
+        b[ks][cb] = Bp[cb * cb_stride + (k0 / 4 + ks) * 64];
+      } else {
+        const int vo = (live >> (4 * cb)) & 1u ? 8 * lane : (int)0x80000000;
+        const int so = (wv + 4 * cb) * blk_bytes + (int)(k0 / 4 + ks) * 512;
+        b[ks][cb] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(brs, vo, so, 0));
+      }
+    }
   };
 
   d4 acc[4][CBW];
@@ -248,11 +321,13 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
 #pragma unroll
     for (int cb = 0; cb < CBW; ++cb) acc[rb][cb] = d4{0.0, 0.0, 0.0, 0.0};
 
+  uint32_t live0 = 0;
   if (count > 0) {
     const int64_t k0 = stage_k0(0);
+    live0 = wave_live(0);
     load_a(k0);
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) load_b(k0, ks);
+    for (int ks = 0; ks < 4; ++ks) load_b(k0, ks, live0);
   }
   for (int s = 0; s < count; ++s) {
     __syncthreads();  // every wavefront is done reading the previous AO slab
@@ -260,6 +335,7 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
     __syncthreads();
     const bool more = s + 1 < count;
     const int64_t k1 = more ? stage_k0(s + 1) : 0;
+    const uint32_t live1 = more ? wave_live(s + 1) : 0u;
     if (more) load_a(k1);
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
@@ -268,50 +344,92 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
       for (int rb = 0; rb < 4; ++rb) a[rb] = As[4 * ks + lk][rb * 16 + li];
 #pragma unroll
       for (int cb = 0; cb < CBW; ++cb)
+        if ((live0 >> (4 * cb)) & 1u) {
 #pragma unroll
-        for (int rb = 0; rb < 4; ++rb)
-          acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], b[ks][cb], acc[rb][cb], 0, 0, 0);
-      if (more) load_b(k1, ks);  // this K step's fragments are free again
+          for (int rb = 0; rb < 4; ++rb)
+            acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], b[ks][cb], acc[rb][cb], 0, 0, 0);
+        }
+      if (more) load_b(k1, ks, live1);  // this K step's fragments are free again
     }
+    live0 = live1;
   }
+  if (TRANSA || plan.col_mo == nullptr) {
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb) {
+      const int64_t col = (cb0 + 4 * cb) * 16 + li;
+      if (col >= Ncol) continue;
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int64_t row = row0 + rb * 16 + lk + 4 * j;
+          if (row < R) C[row * ldc + col] = acc[rb][cb][j];
+        }
+    }
+    return;
+  }
+  // Planned column order: the workgroup's columns are the MOs of its group
+  // [g_lo, g_lo + g_n) in another order.  Rows go through LDS kOutRows at a
+  // time, put back in MO order, and are written as contiguous runs.
+  const int64_t g_lo = gb0 * 16;
+  const int g_n = (int)(Ncol - g_lo < 64 * CBW ? Ncol - g_lo : 64 * CBW);
+  int mloc[CBW];
 #pragma unroll
   for (int cb = 0; cb < CBW; ++cb) {
     const int64_t col = (cb0 + 4 * cb) * 16 + li;
-    if (col >= Ncol) continue;
+    mloc[cb] = col < Ncol ? (int)(plan.col_mo[col] - g_lo) : -1;
+  }
+  for (int pass = 0; pass < kBM / kOutRows; ++pass) {
+    __syncthreads();  // the previous pass has been read out
 #pragma unroll
-    for (int rb = 0; rb < 4; ++rb)
+    for (int cb = 0; cb < CBW; ++cb)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int64_t row = row0 + rb * 16 + lk + 4 * j;
-        if (row < R) C[row * ldc + col] = acc[rb][cb][j];
-      }
+      for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int row = rb * 16 + lk + 4 * j;
+          if (mloc[cb] >= 0 && row / kOutRows == pass) s_out[(row % kOutRows) * (64 * CBW) + mloc[cb]] = acc[rb][cb][j];
+        }
+    __syncthreads();
+    for (int rl = 0; rl < kOutRows; ++rl) {
+      const int64_t row = row0 + pass * kOutRows + rl;
+      if (row >= R) break;
+      double *dst = C + row * ldc + g_lo;
+      for (int c = tid; c < g_n; c += kThreads) dst[c] = s_out[rl * (64 * CBW) + c];
+    }
   }
 }
 
 template <int CBW, bool TRANSA>
 hipError_t launch_frag(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const double *A, int64_t lda,
                        const double *Bf, int64_t nk4, double *C, int64_t ldc, const uint8_t *flags,
-                       int nkb, int splits) {
+                       int nkb, int splits, ColPlan plan) {
   const int64_t gx = (R + kBM - 1) / kBM, gy = (Ncol + 64 * CBW - 1) / (64 * CBW);
   if (gx > 0x7fffffff || gy > 65535 || splits < 1 || splits > 65535) return hipErrorInvalidValue;
   if (flags != nullptr && ((int64_t)nkb * 32 < K || nkb > kMaxFlagBlocks || splits != 1))
     return hipErrorInvalidValue;
   const int64_t kchunk = ((K + splits - 1) / splits + kBK - 1) / kBK * kBK;
+  if (plan.blk != nullptr &&
+      ((kchunk + kBK - 1) / kBK > kMaxStages16 || plan.nw * 64 * kBK < K || plan.nblk < 0))
+    return hipErrorInvalidValue;
+  // the forward product addresses one workgroup's column blocks with 32-bit offsets
+  if (!TRANSA && (int64_t)4 * CBW * nk4 * 512 >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
+  if (TRANSA && (plan.col_mo != nullptr || plan.blk != nullptr)) return hipErrorInvalidValue;
   hipLaunchKernelGGL((gemm_f64_frag_kernel<CBW, TRANSA>), dim3((unsigned)gx, (unsigned)gy, (unsigned)splits),
-                     dim3(kThreads), 0, s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, kchunk);
+                     dim3(kThreads), 0, s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, kchunk, plan);
   return hipGetLastError();
 }
 
 template <bool TRANSA>
 hipError_t dispatch_frag(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const double *A, int64_t lda,
                          const double *Bf, int64_t nk4, double *C, int64_t ldc, const uint8_t *flags, int nkb,
-                         int splits) {
+                         int splits, ColPlan plan) {
   switch (f64_frag_cbw(Ncol)) {
-    case 1: return launch_frag<1, TRANSA>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, splits);
-    case 2: return launch_frag<2, TRANSA>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, splits);
-    case 3: return launch_frag<3, TRANSA>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, splits);
-    case 4: return launch_frag<4, TRANSA>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, splits);
-    default: return launch_frag<5, TRANSA>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, splits);
+    case 1: return launch_frag<1, TRANSA>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, splits, plan);
+    case 2: return launch_frag<2, TRANSA>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, splits, plan);
+    case 3: return launch_frag<3, TRANSA>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, splits, plan);
+    case 4: return launch_frag<4, TRANSA>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, splits, plan);
+    default: return launch_frag<5, TRANSA>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, splits, plan);
   }
 }
 
@@ -343,7 +461,7 @@ int f64_frag_cbw(int64_t Ncol) {
 }
 
 hipError_t launch_frag_b_f64(hipStream_t s, int64_t K, int64_t Ncol, const double *B, int64_t ldb,
-                             double *Bf, int64_t *nk4) {
+                             double *Bf, int64_t *nk4, const int32_t *col_mo) {
   *nk4 = (K + kBK - 1) / kBK * (kBK / 4);
   const int64_t npad = round_up(Ncol, 64 * f64_frag_cbw(Ncol));
   const int64_t total = (npad / 16) * (*nk4) * 64;
@@ -351,16 +469,26 @@ hipError_t launch_frag_b_f64(hipStream_t s, int64_t K, int64_t Ncol, const doubl
   int64_t g = (total + kThreads - 1) / kThreads;
   if (g > 65536) g = 65536;
   hipLaunchKernelGGL(frag_b_f64_kernel, dim3((unsigned)g), dim3(kThreads), 0, s, K, Ncol, B, ldb, *nk4, total,
-                     Bf);
+                     col_mo, Bf);
+  return hipGetLastError();
+}
+
+hipError_t launch_coef_blocks(hipStream_t s, int64_t M, int64_t K, const double *coef, int64_t nw,
+                              uint64_t *words) {
+  if (M <= 0 || nw <= 0) return hipSuccess;
+  if (nw * 64 * 16 < K) return hipErrorInvalidValue;
+  const int64_t g = (M * nw + kThreads / 64 - 1) / (kThreads / 64);
+  if (g > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(coef_blocks_kernel, dim3((unsigned)g), dim3(kThreads), 0, s, M, K, coef, nw, words);
   return hipGetLastError();
 }
 
 hipError_t launch_mo_gemm_f64_frag(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const double *A,
                                    int64_t lda, const double *Bf, int64_t nk4, double *C, int64_t ldc,
-                                   const uint8_t *flags, int nkb) {
+                                   const uint8_t *flags, int nkb, ColPlan plan) {
   if (R <= 0 || Ncol <= 0) return hipSuccess;
   if (K <= 0 || nk4 * 4 < K) return hipErrorInvalidValue;
-  return dispatch_frag<false>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, 1);
+  return dispatch_frag<false>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, 1, plan);
 }
 
 int gemm_tn_frag_splits(int64_t R, int64_t K) {
@@ -379,7 +507,7 @@ hipError_t launch_gemm_tn_f64_frag(hipStream_t s, int64_t R, int64_t K, int64_t 
                                    double *partial, int splits) {
   if (R <= 0 || Ncol <= 0) return hipSuccess;
   if (K <= 0 || nk4 * 4 < K || splits < 1) return hipErrorInvalidValue;
-  hipError_t e = dispatch_frag<true>(s, R, K, Ncol, A, lda, Bf, nk4, partial, Ncol, nullptr, 0, splits);
+  hipError_t e = dispatch_frag<true>(s, R, K, Ncol, A, lda, Bf, nk4, partial, Ncol, nullptr, 0, splits, ColPlan());
   if (e != hipSuccess) return e;
   const int64_t n = R * Ncol;
   int64_t g = (n + kThreads - 1) / kThreads;

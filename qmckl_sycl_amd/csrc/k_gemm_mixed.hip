@@ -101,11 +101,16 @@ __device__ inline f4 mfma(const uint4 &a, const uint4 &b, f4 c) {
 // Workgroup tile WR rows (one half of a 64-row tile, or all of it) x
 // 64*CBW columns (all MOs for mo_num <= 320, so each FP32 AO row is read
 // from HBM once), WR/8 wavefronts at two per SIMD: wavefront w owns rows
-// [32*(w/4), +32) and columns [16*CBW*(w%4), +16*CBW), 2 x CBW MFMA tiles.
+// [32*(w/4), +32) and the column blocks w%4, w%4 + 4, ... of the
+// workgroup's 4*CBW blocks of 16, 2 x CBW MFMA tiles.
 // Each row is scaled by the power of two that brings
 // its largest |entry| into [2^13, 2^14).  K advances in stages of 64;
 // stages whose two flags are zero are dropped from the workgroup's stage
-// list up front.  Per stage:
+// list up front.  With a column plan each listed stage carries, per 32-K
+// half, the live bits of the workgroup's column blocks (nonzero
+// coefficients on those AOs and, from the flags, nonzero AO values): the
+// coefficient loads of a dead (block, half) return zeros without touching
+// memory, and a stage with nothing live is dropped.  Per stage:
 //   - the FP32 A slab (WR x 64, loaded into registers while the previous
 //     stage computed) is scaled, split and written to LDS, shared by the
 //     workgroup's wavefronts;
@@ -139,6 +144,7 @@ __device__ inline f4 mfma(const uint4 &a, const uint4 &b, f4 c) {
 constexpr int kStage = 64;        // K per stage
 constexpr int kLDS = kStage + 8;  // LDS row length in halves (bank spread)
 constexpr int kMaxStages = kMaxFlagBlocks / 2;
+constexpr int kOutRows = 8;  // output rows per pass of the planned-order epilogue
 
 template <int SP>
 __device__ inline void split8(const float (&v)[8], uint4 (&out)[Terms<SP>::n]) {
@@ -174,35 +180,64 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(WR * 8, WR * 8), amdg
 gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *__restrict__ A,
                   int64_t lda, const float *__restrict__ rowmax, const uint16_t *__restrict__ Bs,
                   int64_t ldb, const float *__restrict__ colinv, const uint8_t *__restrict__ flags, int nkb,
-                  const int32_t *__restrict__ perm, double *__restrict__ C, int64_t ldc) {
+                  const int32_t *__restrict__ perm, double *__restrict__ C, int64_t ldc, ColPlan plan) {
   constexpr int T = Terms<SP>::n;
   constexpr int RB = 2;            // 16-row blocks per wavefront
   constexpr int KQ = 8;            // K per thread in the A slab
   __shared__ uint16_t sA[T][WR * kLDS];
   __shared__ float s_inv[WR];
   __shared__ int s_list[kMaxStages];
+  __shared__ uint64_t s_live[kMaxStages];
   __shared__ int s_count;
+  __shared__ double s_out[kOutRows * 64 * CBW];  // output rows in MO order (planned columns)
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  const int cq = wave & 3, rh = wave >> 2;  // column quarter, row half
+  const int cq = __builtin_amdgcn_readfirstlane(wave & 3), rh = wave >> 2;  // column quarter, row half
   const int64_t rt = blockIdx.x / (64 / WR);  // row tile: component c, tile rows 64*pt ..
   const int r0 = (int)(blockIdx.x % (64 / WR)) * WR;  // this workgroup's first row of the tile
   const int64_t comp = rt / npt, p0 = (rt - comp * npt) * 64 + r0;
-  const int64_t col0 = (int64_t)blockIdx.y * 64 * CBW + cq * 16 * CBW;  // this wavefront's first column
+  const int64_t gb0 = (int64_t)blockIdx.y * 4 * CBW;  // the workgroup's first column block
   const int nst = (int)((K + kStage - 1) / kStage);
 
-  // stage list
+  // stage list: live stages and, per stage, bit b (first half) and 32 + b
+  // (second half) for live column block gb0 + b
+  const bool listed = flags != nullptr || plan.blk != nullptr;
   int count = nst;
-  if (flags != nullptr) {
+  if (listed) {
     if (wave == 0) {
-      const uint8_t *f = flags + rt * nkb;
+      const uint8_t *f = flags != nullptr ? flags + rt * nkb : nullptr;
       int base = 0;
       for (int s0 = 0; s0 < nst; s0 += 64) {
         const int st = s0 + lane;
-        const bool live = st < nst && (f[2 * st] != 0 || (2 * st + 1 < nkb && f[2 * st + 1] != 0));
+        const bool h0 = st < nst && (f == nullptr || f[2 * st] != 0);
+        const bool h1 = st < nst && (f == nullptr || (2 * st + 1 < nkb && f[2 * st + 1] != 0));
+        uint64_t lv = (h0 ? 0xffffffffull : 0ull) | (h1 ? 0xffffffff00000000ull : 0ull);
+        if ((h0 || h1) && plan.blk != nullptr) {
+          // all loads first (independent, no branches), then the bits
+          const int64_t g = 4 * (int64_t)st;  // first 16-AO block of the stage
+          uint64_t w[4 * CBW];
+#pragma unroll
+          for (int b = 0; b < 4 * CBW; ++b) {
+            const int64_t blk = gb0 + b < plan.nblk ? gb0 + b : plan.nblk - 1;
+            w[b] = plan.blk[blk * plan.nw + (g >> 6)];
+          }
+          uint64_t bits = 0;
+#pragma unroll
+          for (int b = 0; b < 4 * CBW; ++b) {
+            const uint64_t nib = gb0 + b < plan.nblk ? (w[b] >> (g & 63)) & 15ull : 0ull;
+            bits |= ((nib & 3ull) != 0 ? 1ull : 0ull) << b;
+            bits |= ((nib & 12ull) != 0 ? 1ull : 0ull) << (32 + b);
+          }
+          lv &= bits;
+        }
+        const bool live = lv != 0;
         const uint64_t m = __ballot(live);
-        if (live) s_list[base + __popcll(m & ((1ull << lane) - 1ull))] = st;
+        if (live) {
+          const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
+          s_list[pos] = st;
+          s_live[pos] = lv;
+        }
         base += __popcll(m);
       }
       if (lane == 0) s_count = base;
@@ -210,6 +245,14 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
     __syncthreads();
     count = s_count;
   }
+  // this wavefront's live column blocks in half ks of stage s: bit 4j for
+  // block gb0 + cq + 4j
+  auto wave_live = [&](int s, int ks) -> uint32_t {
+    if (!listed) return 0xffffffffu;
+    const uint64_t lv = s_live[s];
+    const uint32_t w = ks == 0 ? (uint32_t)lv : (uint32_t)(lv >> 32);
+    return (uint32_t)__builtin_amdgcn_readfirstlane(w) >> cq;
+  };
 
   // A slot: row ar = lane of the tile, KQ consecutive K at aq = KQ*wave,
   // read as 16-byte pieces of the blocked operand (consecutive lanes,
@@ -223,25 +266,37 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
   // C operand in fragment order (split_coef_kernel): the 16-byte piece of
   // lane l for column tile ct, 32-K chunk kc, term t is at
   // ((ct * nkc + kc) * T + t) * 512 + 8 * l halves, so each load of a
-  // wavefront reads one contiguous kilobyte
+  // wavefront reads one contiguous kilobyte.  The loads go through a buffer
+  // resource over the workgroup's 4*CBW column tiles: lane offset 16*l
+  // bytes, tile and stage offset in a scalar register.  A dead (tile, half)
+  // gets a lane offset past the end of the buffer, so its loads return
+  // zeros without a memory access, and every stage issues the same loads
+  // (the waits on them stay exact).
   const int64_t nkc = ldb / 32;
-  const uint16_t *Bp = Bs + (col0 / 16) * nkc * T * 512 + 8 * lane;
-  const int64_t cb_stride = nkc * T * 512;
+  const int tile_bytes = (int)(nkc * T * 1024);
+  const __amdgpu_buffer_rsrc_t brs = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)(Bs + gb0 * nkc * T * 512), (short)0, 4 * CBW * tile_bytes, 0x00020000);
+  const int lane_off = 16 * lane;
 
   float4 ra[KQ / 4];
   uint4 bq[2][CBW][T];
-  auto stage_k0 = [&](int s) { return (int64_t)(flags != nullptr ? s_list[s] : s) * kStage; };
+  auto stage_k0 = [&](int s) {
+    return (int64_t)__builtin_amdgcn_readfirstlane(listed ? s_list[s] : s) * kStage;
+  };
   auto load_a = [&](int64_t k0) {
     const float *src = Ap + (k0 / 4) * 256;
 #pragma unroll
     for (int q = 0; q < KQ / 4; ++q) ra[q] = *reinterpret_cast<const float4 *>(src + q * 256);
   };
-  auto load_b = [&](int64_t k0, int ks) {
+  auto load_b = [&](int64_t k0, int ks, uint32_t live) {
 #pragma unroll
-    for (int cb = 0; cb < CBW; ++cb)
+    for (int cb = 0; cb < CBW; ++cb) {
+      const int vo = (live >> (4 * cb)) & 1u ? lane_off : (int)0x80000000;
+      const int so = (cq + 4 * cb) * tile_bytes + (int)(k0 / 32 + ks) * T * 1024;
 #pragma unroll
       for (int t = 0; t < T; ++t)
-        bq[ks][cb][t] = *reinterpret_cast<const uint4 *>(Bp + cb * cb_stride + ((k0 / 32 + ks) * T + t) * 512);
+        bq[ks][cb][t] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(brs, vo + t * 1024, so, 0));
+    }
   };
 
   f4 d[RB][CBW];
@@ -256,8 +311,8 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
   if (count > 0) {
     const int64_t k0 = stage_k0(0);
     load_a(k0);
-    load_b(k0, 0);
-    load_b(k0, 1);
+    load_b(k0, 0, wave_live(0, 0));
+    load_b(k0, 1, wave_live(0, 1));
   } else {
     __syncthreads();  // s_inv is read by every wavefront in the epilogue
   }
@@ -276,6 +331,7 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
     __syncthreads();
     const bool more = s + 1 < count;
     const int64_t k1 = more ? stage_k0(s + 1) : 0;
+    const uint32_t live1[2] = {more ? wave_live(s + 1, 0) : 0u, more ? wave_live(s + 1, 1) : 0u};
     if (more) load_a(k1);  // in flight while this stage computes
 
 #pragma unroll
@@ -287,7 +343,9 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
         for (int r = 0; r < RB; ++r)
           a[t][r] = *reinterpret_cast<const uint4 *>(&sA[t][((rh * RB + r) * 16 + li) * kLDS + ks * 32 + 8 * lk]);
       // product by product over all RB x CBW tiles: RB*CBW independent
-      // accumulators between dependent MFMAs
+      // accumulators between dependent MFMAs.  Dead tiles run on the zeros
+      // their loads returned (branching around their MFMAs measured slower:
+      // 3.28 against 2.97 ms on water64).
       constexpr int NP = T == 3 ? 6 : 3;
       constexpr int pa[6] = {0, 0, 1, 0, 2, 1}, pb[6] = {0, 1, 0, 2, 0, 1};  // (A term, C term)
 #pragma unroll
@@ -298,7 +356,7 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
           for (int r = 0; r < RB; ++r)
             d[r][cb] = mfma<SP>(a[pa[pr]][r], bq[ks][cb][pb[pr]],
                                 pr == 0 && ks == 0 ? f4{0.f, 0.f, 0.f, 0.f} : d[r][cb]);
-      if (more) load_b(k1, ks);  // this half-stage's C registers are free again
+      if (more) load_b(k1, ks, live1[ks]);  // this half-stage's C registers are free again
     }
 #pragma unroll
     for (int r = 0; r < RB; ++r)
@@ -307,22 +365,59 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[r][cb][j] += (double)d[r][cb][j];
   }
+  if (plan.col_mo == nullptr) {
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb) {
+      const int64_t col = (gb0 + cq + 4 * cb) * 16 + li;
+      if (col >= Ncol) continue;
+      const double ci = (double)colinv[col];
+#pragma unroll
+      for (int r = 0; r < RB; ++r)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int row = (rh * RB + r) * 16 + 4 * lk + j;
+          const int64_t i = p0 + row;
+          if (i < P) {
+            const int64_t p = perm != nullptr ? (int64_t)perm[i] : i;
+            C[(p * 5 + comp) * ldc + col] = acc[r][cb][j] * ((double)s_inv[row] * ci);
+          }
+        }
+    }
+    return;
+  }
+  // Planned column order: the workgroup's columns are the MOs of its group
+  // [g_lo, g_lo + g_n) in another order.  Rows go through LDS kOutRows at a
+  // time, put back in MO order, and are written as contiguous runs.
+  const int64_t g_lo = gb0 * 16;
+  const int g_n = (int)(Ncol - g_lo < 64 * CBW ? Ncol - g_lo : 64 * CBW);
+  int mloc[CBW];
+  double ci[CBW];
 #pragma unroll
   for (int cb = 0; cb < CBW; ++cb) {
-    const int64_t col = col0 + cb * 16 + li;
-    if (col >= Ncol) continue;
-    const double ci = (double)colinv[col];
+    const int64_t col = (gb0 + cq + 4 * cb) * 16 + li;
+    mloc[cb] = col < Ncol ? (int)(plan.col_mo[col] - g_lo) : -1;
+    ci[cb] = col < Ncol ? (double)colinv[col] : 0.0;
+  }
+  for (int pass = 0; pass < WR / kOutRows; ++pass) {
+    __syncthreads();  // the previous pass has been read out
 #pragma unroll
-    for (int r = 0; r < RB; ++r)
+    for (int cb = 0; cb < CBW; ++cb)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int row = (rh * RB + r) * 16 + 4 * lk + j;
-        const int64_t i = p0 + row;
-        if (i < P) {
-          const int64_t p = perm != nullptr ? (int64_t)perm[i] : i;
-          C[(p * 5 + comp) * ldc + col] = acc[r][cb][j] * ((double)s_inv[row] * ci);
+      for (int r = 0; r < RB; ++r)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int row = (rh * RB + r) * 16 + 4 * lk + j;
+          if (mloc[cb] >= 0 && row / kOutRows == pass)
+            s_out[(row % kOutRows) * (64 * CBW) + mloc[cb]] = acc[r][cb][j] * ((double)s_inv[row] * ci[cb]);
         }
-      }
+    __syncthreads();
+    for (int rl = 0; rl < kOutRows; ++rl) {
+      const int64_t i = p0 + pass * kOutRows + rl;
+      if (i >= P) break;
+      const int64_t p = perm != nullptr ? (int64_t)perm[i] : i;
+      double *dst = C + (p * 5 + comp) * ldc + g_lo;
+      for (int c = tid; c < g_n; c += WR * 8) dst[c] = s_out[rl * (64 * CBW) + c];
+    }
   }
 }
 
@@ -437,11 +532,11 @@ gemm_f32_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *__
 template <int SP>
 __global__ void __launch_bounds__(kThreads)
 split_coef_kernel(int64_t M, int64_t K, const double *__restrict__ coef, int64_t ldb, int frag,
-                  uint16_t *__restrict__ Bs, float *__restrict__ colinv) {
+                  const int32_t *__restrict__ col_mo, uint16_t *__restrict__ Bs, float *__restrict__ colinv) {
   __shared__ float red[kThreads];
-  const int64_t i = blockIdx.x;
+  const int64_t i = blockIdx.x;  // prepared column
   const bool real = i < M;
-  const double *row = coef + (real ? i : 0) * K;
+  const double *row = coef + (real ? (col_mo != nullptr ? (int64_t)col_mo[i] : i) : 0) * K;
   float m = 0.f;
   if (real)
     for (int64_t k = threadIdx.x; k < K; k += kThreads) m = fmaxf(m, fabsf((float)row[k]));
@@ -483,15 +578,19 @@ template <int SP, int CBW>
 hipError_t launch_split(hipStream_t s, int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *A,
                         int64_t lda, const float *rowmax, const void *B, int64_t ldb,
                         const float *colinv, const uint8_t *flags, int nkb, const int32_t *perm, double *C,
-                        int64_t ldc) {
+                        int64_t ldc, ColPlan plan) {
   constexpr int WR = 32;
   const int64_t gx = 5 * npt * (64 / WR), gy = (Ncol + 64 * CBW - 1) / (64 * CBW);
   if (gx > 0x7fffffff || gy > 65535 || (lda % kStage) != 0 || (ldb % kStage) != 0)
     return hipErrorInvalidValue;
   if (flags != nullptr && (nkb > kMaxFlagBlocks || (int64_t)nkb * 32 < K)) return hipErrorInvalidValue;
+  if (plan.blk != nullptr && ((K + kStage - 1) / kStage > kMaxStages || plan.nw * 64 * 16 < K || plan.nblk < 0))
+    return hipErrorInvalidValue;
+  // the coefficient buffer of one workgroup's column tiles is addressed with 32-bit offsets
+  if ((int64_t)4 * CBW * (ldb / 32) * Terms<SP>::n * 1024 >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
   hipLaunchKernelGGL((gemm_split_kernel<SP, CBW, WR>), dim3((unsigned)gx, (unsigned)gy), dim3(WR * 8), 0, s, P, npt,
                      K, Ncol, A, lda, rowmax, (const uint16_t *)B, ldb, colinv, flags, nkb, perm, C,
-                     ldc);
+                     ldc, plan);
   return hipGetLastError();
 }
 
@@ -499,8 +598,9 @@ template <int SP>
 hipError_t launch_form(hipStream_t s, int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *A,
                        int64_t lda, const float *rowmax, const void *B, int64_t ldb,
                        const float *colinv, const uint8_t *flags, int nkb, const int32_t *perm, double *C,
-                       int64_t ldc) {
+                       int64_t ldc, ColPlan plan) {
   if constexpr (SP == QMCKL_AMD_SPLIT_FP32) {
+    if (plan.col_mo != nullptr || plan.blk != nullptr) return hipErrorInvalidValue;
     const int64_t nb = (Ncol + 31) / 32;  // 32-column groups
     if (nb <= 1) return launch_f32<1>(s, P, npt, K, Ncol, A, lda, B, ldb, flags, nkb, perm, C, ldc);
     if (nb <= 2) return launch_f32<2>(s, P, npt, K, Ncol, A, lda, B, ldb, flags, nkb, perm, C, ldc);
@@ -509,12 +609,12 @@ hipError_t launch_form(hipStream_t s, int64_t P, int64_t npt, int64_t K, int64_t
     return launch_f32<5>(s, P, npt, K, Ncol, A, lda, B, ldb, flags, nkb, perm, C, ldc);
   } else {
     const int cbw = split_wg_cols(SP, Ncol) / 64;
-    if (cbw == 1) return launch_split<SP, 1>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc);
-    if (cbw == 2) return launch_split<SP, 2>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc);
-    if (cbw == 3) return launch_split<SP, 3>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc);
+    if (cbw == 1) return launch_split<SP, 1>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc, plan);
+    if (cbw == 2) return launch_split<SP, 2>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc, plan);
+    if (cbw == 3) return launch_split<SP, 3>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc, plan);
     if constexpr (Terms<SP>::n == 2) {
-      if (cbw == 4) return launch_split<SP, 4>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc);
-      if (cbw == 5) return launch_split<SP, 5>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc);
+      if (cbw == 4) return launch_split<SP, 4>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc, plan);
+      if (cbw == 5) return launch_split<SP, 5>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc, plan);
     }
     return hipErrorInvalidValue;
   }
@@ -533,23 +633,23 @@ int split_terms(int form) {
 }
 
 hipError_t launch_split_coef(hipStream_t s, int form, int64_t M, int64_t K, const double *coef,
-                             int64_t ldb, uint16_t *B, float *colinv) {
+                             int64_t ldb, uint16_t *B, float *colinv, const int32_t *col_mo) {
   if (M <= 0) return hipSuccess;
-  if (ldb % 64 != 0) return hipErrorInvalidValue;
+  if (ldb % 64 != 0 || (form == QMCKL_AMD_SPLIT_FP32 && col_mo != nullptr)) return hipErrorInvalidValue;
   const unsigned g = (unsigned)split_cols_padded(form, M);
   const int frag = 16;  // fragment order of gemm_split_kernel's 16x16x32 tiles
   switch (form) {
     case QMCKL_AMD_SPLIT_FP16X2:
-      hipLaunchKernelGGL(split_coef_kernel<QMCKL_AMD_SPLIT_FP16X2>, dim3(g), dim3(kThreads), 0, s, M, K, coef, ldb, frag, B, colinv);
+      hipLaunchKernelGGL(split_coef_kernel<QMCKL_AMD_SPLIT_FP16X2>, dim3(g), dim3(kThreads), 0, s, M, K, coef, ldb, frag, col_mo, B, colinv);
       break;
     case QMCKL_AMD_SPLIT_BF16X2:
-      hipLaunchKernelGGL(split_coef_kernel<QMCKL_AMD_SPLIT_BF16X2>, dim3(g), dim3(kThreads), 0, s, M, K, coef, ldb, frag, B, colinv);
+      hipLaunchKernelGGL(split_coef_kernel<QMCKL_AMD_SPLIT_BF16X2>, dim3(g), dim3(kThreads), 0, s, M, K, coef, ldb, frag, col_mo, B, colinv);
       break;
     case QMCKL_AMD_SPLIT_BF16X3:
-      hipLaunchKernelGGL(split_coef_kernel<QMCKL_AMD_SPLIT_BF16X3>, dim3(g), dim3(kThreads), 0, s, M, K, coef, ldb, frag, B, colinv);
+      hipLaunchKernelGGL(split_coef_kernel<QMCKL_AMD_SPLIT_BF16X3>, dim3(g), dim3(kThreads), 0, s, M, K, coef, ldb, frag, col_mo, B, colinv);
       break;
     case QMCKL_AMD_SPLIT_FP32:
-      hipLaunchKernelGGL(split_coef_kernel<QMCKL_AMD_SPLIT_FP32>, dim3(g), dim3(kThreads), 0, s, M, K, coef, ldb, frag, B, colinv);
+      hipLaunchKernelGGL(split_coef_kernel<QMCKL_AMD_SPLIT_FP32>, dim3(g), dim3(kThreads), 0, s, M, K, coef, ldb, frag, col_mo, B, colinv);
       break;
     default: return hipErrorInvalidValue;
   }
@@ -559,20 +659,20 @@ hipError_t launch_split_coef(hipStream_t s, int form, int64_t M, int64_t K, cons
 hipError_t launch_mo_gemm_mixed(hipStream_t s, int form, int64_t P, int64_t K, int64_t Ncol,
                                 const float *A, int64_t lda, const float *rowmax, const void *B, int64_t ldb,
                                 const float *colinv, const uint8_t *flags, int nkb, const int32_t *perm,
-                                double *C, int64_t ldc) {
+                                double *C, int64_t ldc, ColPlan plan) {
   if (P <= 0 || Ncol <= 0) return hipSuccess;
   if (K <= 0 || (lda % 64) != 0 || (ldb % 64) != 0 || lda < K || ldb < K)
     return hipErrorInvalidValue;
   const int64_t npt = (P + 63) / 64;
   switch (form) {
     case QMCKL_AMD_SPLIT_FP16X2:
-      return launch_form<QMCKL_AMD_SPLIT_FP16X2>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc);
+      return launch_form<QMCKL_AMD_SPLIT_FP16X2>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc, plan);
     case QMCKL_AMD_SPLIT_BF16X2:
-      return launch_form<QMCKL_AMD_SPLIT_BF16X2>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc);
+      return launch_form<QMCKL_AMD_SPLIT_BF16X2>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc, plan);
     case QMCKL_AMD_SPLIT_BF16X3:
-      return launch_form<QMCKL_AMD_SPLIT_BF16X3>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc);
+      return launch_form<QMCKL_AMD_SPLIT_BF16X3>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc, plan);
     case QMCKL_AMD_SPLIT_FP32:
-      return launch_form<QMCKL_AMD_SPLIT_FP32>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc);
+      return launch_form<QMCKL_AMD_SPLIT_FP32>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc, plan);
     default:
       return hipErrorInvalidValue;
   }

@@ -33,6 +33,96 @@ qmckl_exit_code_device ready(qmckl_context_device context, Context *c, const cha
   return QMCKL_SUCCESS_DEVICE;
 }
 
+// Planned column orders (plan_mo_columns) are kept when at most kPlanLive
+// of the (16-MO, 16-AO) blocks stay live; otherwise the MO order is used.
+constexpr double kPlanLive = 0.9;
+constexpr int64_t kPlanMaxAo = 16 * 4096;  // the GEMM kernels' stage lists hold 4,096 blocks of 16 AOs
+
+void free_col_plans(Context *c) {
+  for (ColPlanSlot &p : c->mo.col_plans) {
+    dfree(c, p.col_mo);
+    dfree(c, p.blk);
+    p = ColPlanSlot();
+  }
+}
+
+// The nonzero map of the coefficients at 16-AO granularity, on the host.
+hipError_t map_coef_zeros(Context *c) {
+  MoBasis &mo = c->mo;
+  free_col_plans(c);
+  mo.nz_words.clear();
+  mo.nz_nw = 0;
+  const int64_t A = c->ao.ao_num, M = mo.mo_num;
+  if (A <= 0 || M <= 0 || A > kPlanMaxAo) return hipSuccess;
+  const int64_t nw = ((A + 15) / 16 + 63) / 64;
+  uint64_t *dwords = static_cast<uint64_t *>(dmalloc(c, sizeof(uint64_t) * (size_t)(M * nw)));
+  if (dwords == nullptr) return hipErrorOutOfMemory;
+  std::vector<uint64_t> words((size_t)(M * nw));
+  hipError_t e = launch_coef_blocks(c->stream, M, A, mo.coefficient, nw, dwords);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(words.data(), dwords, sizeof(uint64_t) * words.size(), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = sync(c);
+  dfree(c, dwords);
+  if (e != hipSuccess) return e;
+  mo.nz_words.swap(words);
+  mo.nz_nw = nw;
+  return hipSuccess;
+}
+
+// The column plan for workgroups of `group` columns: the MOs of each group
+// of `group` consecutive MOs are ordered among themselves.  Built on first
+// use after finalize (slot 0 holds the FP64 contraction's width).
+hipError_t col_plan_for(Context *c, int64_t group, ColPlan *out) {
+  MoBasis &mo = c->mo;
+  *out = ColPlan();
+  if (mo.nz_nw <= 0 || group <= 0 || group % 16 != 0) return hipSuccess;
+  ColPlanSlot *slot = nullptr;
+  for (ColPlanSlot &p : mo.col_plans)
+    if (p.group == group) slot = &p;
+  if (slot == nullptr) {
+    slot = mo.col_plans[0].group == 0 ? &mo.col_plans[0] : &mo.col_plans[1];
+    dfree(c, slot->col_mo);
+    dfree(c, slot->blk);
+    *slot = ColPlanSlot();
+    const int64_t M = mo.mo_num, nw = mo.nz_nw, nkb16 = (c->ao.ao_num + 15) / 16;
+    const int64_t nblk = (M + 15) / 16;
+    std::vector<int32_t> order((size_t)M);
+    std::vector<uint64_t> blk((size_t)(nblk * nw), 0);
+    int64_t live = 0;
+    for (int64_t g0 = 0; g0 < M; g0 += group) {
+      const int64_t n = std::min<int64_t>(group, M - g0);
+      std::vector<int32_t> o;
+      std::vector<uint64_t> b;
+      live += plan_mo_columns(mo.nz_words.data() + g0 * nw, n, nw, nkb16, o, b);
+      for (int64_t j = 0; j < n; ++j) order[(size_t)(g0 + j)] = (int32_t)(g0 + o[(size_t)j]);
+      std::copy(b.begin(), b.end(), blk.begin() + (g0 / 16) * nw);
+    }
+    slot->group = group;
+    slot->live = (double)live / ((double)nblk * (double)nkb16);
+    if (slot->live <= kPlanLive) {
+      slot->col_mo = static_cast<int32_t *>(dmalloc(c, sizeof(int32_t) * order.size()));
+      slot->blk = static_cast<uint64_t *>(dmalloc(c, sizeof(uint64_t) * blk.size()));
+      if (slot->col_mo == nullptr || slot->blk == nullptr) return hipErrorOutOfMemory;
+      hipError_t e = hipMemcpyAsync(slot->col_mo, order.data(), sizeof(int32_t) * order.size(),
+                                    hipMemcpyHostToDevice, c->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(slot->blk, blk.data(), sizeof(uint64_t) * blk.size(), hipMemcpyHostToDevice, c->stream);
+      if (e == hipSuccess) e = sync(c);
+      if (e != hipSuccess) return e;
+    }
+  }
+  if (slot->col_mo != nullptr) {
+    out->col_mo = slot->col_mo;
+    out->blk = slot->blk;
+    out->nw = mo.nz_nw;
+    out->nblk = (mo.mo_num + 15) / 16;
+  }
+  return hipSuccess;
+}
+
+// the FP64 contraction's workgroup width (launch_mo_gemm_f64_frag)
+int64_t f64_group(int64_t M) { return 64 * (int64_t)f64_frag_cbw(M); }
+
 const char *gemm_timer_name(qmckl_precision_device form) {
   switch (form) {
     case QMCKL_PRECISION_FP32_DEVICE: return "mo_gemm_fp32";
@@ -78,14 +168,19 @@ qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Contex
                 !grow_bytes(c, &ao.order_ws, &ao.order_ws_cap, point_order_workspace_bytes(P))))
     return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
   const int32_t *perm = order ? ao.point_perm : nullptr;
+  // the split forms use the column plan of their workgroup width; the FP32
+  // form keeps the MO order
+  ColPlan plan;
   hipError_t e = hipSuccess;
+  if (form != QMCKL_PRECISION_FP32_DEVICE) e = col_plan_for(c, split_wg_cols(form, M), &plan);
+  if (e != hipSuccess) return hip_check(context, e, fn);
   if (mo.coef_split == nullptr || mo.coef_split_form != form) {
     dfree(c, mo.coef_split);
     mo.coef_split = static_cast<uint16_t *>(
         dmalloc(c, sizeof(uint16_t) * split_terms(form) * split_cols_padded(form, M) * ld));
     if (!drealloc(c, reinterpret_cast<void **>(&mo.coef_colinv), sizeof(float) * M) || !mo.coef_split)
       return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
-    e = launch_split_coef(c->stream, form, M, A, mo.coefficient, ld, mo.coef_split, mo.coef_colinv);
+    e = launch_split_coef(c->stream, form, M, A, mo.coefficient, ld, mo.coef_split, mo.coef_colinv, plan.col_mo);
     if (e == hipSuccess) mo.coef_split_form = form;
   }
   if (e == hipSuccess && order) {
@@ -103,7 +198,7 @@ qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Contex
   if (e == hipSuccess) {
     KernelTimer t(c, gemm_timer_name(form));
     e = launch_mo_gemm_mixed(c->stream, form, P, A, M, ao.ao_f32, ld, ao.ao_f32_rowmax, mo.coef_split, ld,
-                             mo.coef_colinv, flags, flags ? nkb : 0, perm, dst, M);
+                             mo.coef_colinv, flags, flags ? nkb : 0, perm, dst, M, plan);
     t.stop();
   }
   if (e == hipSuccess) e = sync(c);
@@ -123,8 +218,11 @@ qmckl_exit_code_device compute_mo_vgl(qmckl_context_device context, Context *c, 
   hipError_t e;
   {
     KernelTimer t(c, "mo_gemm_f64");
-    e = launch_mo_gemm_f64_frag(c->stream, 5 * P, A, M, c->ao.ao_vgl, A, c->mo.coef_frag,
-                                c->mo.coef_frag_nk4, dst, M, flags, flags ? nkb : 0);
+    ColPlan plan;
+    e = col_plan_for(c, f64_group(M), &plan);
+    if (e == hipSuccess)
+      e = launch_mo_gemm_f64_frag(c->stream, 5 * P, A, M, c->ao.ao_vgl, A, c->mo.coef_frag,
+                                  c->mo.coef_frag_nk4, dst, M, flags, flags ? nkb : 0, plan);
     t.stop();
   }
   if (e == hipSuccess) e = sync(c);
@@ -143,8 +241,11 @@ qmckl_exit_code_device compute_mo_value(qmckl_context_device context, Context *c
     if (rc != QMCKL_SUCCESS_DEVICE)
       return fail(context, QMCKL_NOT_PROVIDED_DEVICE, "qmckl_ao_basis_ao_value_device", nullptr);
     KernelTimer t(c, "mo_value_gemm_f64");
-    e = launch_mo_gemm_f64_frag(c->stream, P, A, M, c->ao.ao_value, A, c->mo.coef_frag,
-                                c->mo.coef_frag_nk4, dst, M);
+    ColPlan plan;
+    e = col_plan_for(c, f64_group(M), &plan);
+    if (e == hipSuccess)
+      e = launch_mo_gemm_f64_frag(c->stream, P, A, M, c->ao.ao_value, A, c->mo.coef_frag,
+                                  c->mo.coef_frag_nk4, dst, M, nullptr, 0, plan);
     t.stop();
   }
   if (e == hipSuccess) e = sync(c);
@@ -182,9 +283,13 @@ qmckl_exit_code_device qmckl_finalize_mo_basis_device(qmckl_context_device conte
   const int64_t frag_len = round_up(M, 64 * f64_frag_cbw(M)) * round_up(A, 16);
   if (!drealloc(c, reinterpret_cast<void **>(&c->mo.coef_frag), sizeof(double) * frag_len))
     return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
-  hipError_t e = launch_transpose(c->stream, M, A, c->mo.coefficient, c->mo.coefficient_t);
+  ColPlan plan;
+  hipError_t e = map_coef_zeros(c);
+  if (e == hipSuccess) e = col_plan_for(c, f64_group(M), &plan);
+  if (e == hipSuccess) e = launch_transpose(c->stream, M, A, c->mo.coefficient, c->mo.coefficient_t);
   if (e == hipSuccess)
-    e = launch_frag_b_f64(c->stream, A, M, c->mo.coefficient_t, M, c->mo.coef_frag, &c->mo.coef_frag_nk4);
+    e = launch_frag_b_f64(c->stream, A, M, c->mo.coefficient_t, M, c->mo.coef_frag, &c->mo.coef_frag_nk4,
+                          plan.col_mo);
   if (e == hipSuccess) e = sync(c);
   if (e != hipSuccess) return hip_check(context, e, fn);
   c->mo.mo_vgl_date = c->mo.mo_value_date = 0;
@@ -538,6 +643,30 @@ qmckl_exit_code_device qmckl_get_mo_basis_tile_config_device(qmckl_context_devic
   return QMCKL_SUCCESS_DEVICE;
 }
 
+qmckl_exit_code_device qmckl_get_mo_basis_column_plan_device(qmckl_context_device context, int32_t *col_mo,
+                                                             double *live) {
+  Context *c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+  const char *fn = "qmckl_get_mo_basis_column_plan_device";
+  if (!c->mo.provided || c->mo.coef_frag == nullptr) return fail(context, QMCKL_NOT_PROVIDED_DEVICE, fn, nullptr);
+  const int64_t M = c->mo.mo_num;
+  ColPlan plan;
+  hipError_t e = col_plan_for(c, f64_group(M), &plan);
+  if (e != hipSuccess) return hip_check(context, e, fn);
+  double lv = 1.0;
+  for (const ColPlanSlot &p : c->mo.col_plans)
+    if (p.group == f64_group(M)) lv = p.live;
+  if (live != nullptr) *live = lv;
+  if (col_mo == nullptr) return QMCKL_SUCCESS_DEVICE;
+  if (plan.col_mo == nullptr) {
+    for (int64_t j = 0; j < M; ++j) col_mo[j] = (int32_t)j;
+    return QMCKL_SUCCESS_DEVICE;
+  }
+  e = hipMemcpyAsync(col_mo, plan.col_mo, sizeof(int32_t) * (size_t)M, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = sync(c);
+  return hip_check(context, e, fn);
+}
+
 qmckl_exit_code_device qmckl_mo_basis_autotune_device(qmckl_context_device context) {
   Context *c = ctx_of(context);
   if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
@@ -635,7 +764,8 @@ std::vector<uint64_t> batch_signature(const Context *c, const double *out) {
           u(ao.ao_factor), u(ao.ao_vgl), u(ao.ao_vgl_flags), u(ao.ao_f32), u(ao.ao_f32_rowmax),
           u(ao.ao_f32_flags), u(ao.point_perm), u(ao.order_ws), u(b.shells), u(b.chunks), u(b.stage_shell),
           u(b.expo), u(b.coef_norm), u(b.nucl_range), u(b.ao_shell), u(b.ao_abc), u(mo.coef_frag),
-          u(mo.coef_split), u(mo.coef_colinv)};
+          u(mo.coef_split), u(mo.coef_colinv), u(mo.col_plans[0].col_mo), u(mo.col_plans[0].blk),
+          u(mo.col_plans[1].col_mo), u(mo.col_plans[1].blk)};
 }
 
 }  // namespace

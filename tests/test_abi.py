@@ -318,6 +318,9 @@ def test_tile_config_and_autotune_validation(ctx):
     assert L.qmckl_set_mo_basis_tile_config_device(0, 0, 0) == INVALID_CONTEXT
     assert L.qmckl_mo_basis_autotune_device(ctx.handle) == NOT_PROVIDED
     assert L.qmckl_mo_basis_autotune_device(0) == INVALID_CONTEXT
+    live = ctypes.c_double(0.0)
+    assert L.qmckl_get_mo_basis_column_plan_device(ctx.handle, None, ctypes.byref(live)) == NOT_PROVIDED
+    assert L.qmckl_get_mo_basis_column_plan_device(0, None, None) == INVALID_CONTEXT
     g = ctypes.c_int64(0)
     assert L.qmckl_mo_basis_mo_vgl_batch_graph_launch_device(12345) == 1
     assert L.qmckl_mo_basis_mo_vgl_batch_graph_destroy_device(12345) == 1

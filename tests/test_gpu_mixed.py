@@ -182,3 +182,64 @@ def test_block_skipping_is_exact(oracle_lib):
         c.check(lib().qmckl_compute_mo_basis_mo_vgl_device(c.handle, A, M, P, ct.ptr, ao.ptr, out.ptr))
         dense = out.numpy()
     assert np.array_equal(mo, dense)
+
+
+def column_plan(c, M):
+    col = (ctypes.c_int32 * M)()
+    live = ctypes.c_double(-1.0)
+    c.call("qmckl_get_mo_basis_column_plan_device", col, ctypes.byref(live))
+    return np.frombuffer(col, dtype=np.int32).copy(), live.value
+
+
+def test_column_plan_follows_coefficient_zeros(chb):
+    """Localised MOs (the 64-water system: 70% of the coefficients zero by
+    locality) get a planned column order that leaves at most 90% of the
+    16-MO x 16-AO blocks live; ChBrClF's dense coefficients keep the MO
+    order."""
+    s = S.water64(walk_num=1)
+    with Context(0) as c:
+        c.set_system(s)
+        col, live = column_plan(c, s.mo_num)
+    say(f"water64 column plan: {live:.3f} of the blocks live")
+    assert sorted(col.tolist()) == list(range(s.mo_num))
+    assert not np.array_equal(col, np.arange(s.mo_num))
+    assert live <= 0.9
+    with Context(0) as c:
+        c.set_system(chb)
+        col, live = column_plan(c, chb.mo_num)
+    assert np.array_equal(col, np.arange(chb.mo_num)) and live > 0.9
+
+
+@pytest.mark.parametrize("mo_num", [37, 333])
+def test_planned_columns_ragged_mo_counts(oracle_lib, mo_num):
+    """Planned column order with a partial last 16-column block (37 MOs) and
+    with more MOs than one workgroup's 320 columns (333): FP64 bitwise equal
+    to the dense low-level contraction and within 1e-12 of the oracle, the
+    value-only contraction equal to the oracle's value rows, every split
+    form within its bar."""
+    s = S.water_cluster(n_side=2, walk_num=2, mo_num=mo_num, name=f"water8_{mo_num}mo")
+    ref = oracle_lib.mo_vgl(s)
+    P, A, M = s.point_num, s.ao_num, s.mo_num
+    with Context(0) as c:
+        c.set_system(s)
+        col, live = column_plan(c, M)
+        say(f"water8 {mo_num} MOs: {live:.3f} of the blocks live")
+        assert live <= 0.9 and sorted(col.tolist()) == list(range(M))
+        mv = c.mo_value(P)  # value-only contraction (mo_vgl not computed yet)
+        mo = c.mo_vgl(P)
+        e = row_err(mo, ref)
+        say(f"water8 {mo_num} MOs fp64: {e}")
+        assert max(e.values()) <= 1e-12, e
+        rv = ref[:, 0, :]
+        ev = np.max(np.abs(mv - rv) / np.maximum(np.abs(rv).max(axis=1, keepdims=True), 1e-300))
+        assert ev <= 1e-12, ev
+        ao = DeviceArray(c, (P, 5, A))
+        c.call("qmckl_get_ao_basis_ao_vgl_device", ao.ptr, ao.size)
+        ct = DeviceArray.from_numpy(c, np.ascontiguousarray(s.mo_coef.T))
+        out = DeviceArray(c, (P, 5, M))
+        c.check(lib().qmckl_compute_mo_basis_mo_vgl_device(c.handle, A, M, P, ct.ptr, ao.ptr, out.ptr))
+        assert np.array_equal(mo, out.numpy())
+        for form in ("fp16x2", "bf16x2", "bf16x3"):
+            e = row_err(run_form(c, form, P), ref)
+            say(f"water8 {mo_num} MOs {form}: {e}")
+            assert max(e.values()) <= BAR[form], (form, e)

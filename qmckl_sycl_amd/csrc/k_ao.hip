@@ -29,6 +29,8 @@
 // 40 B (FP64) or 20 B (FP32) per (point, AO) plus 24 B per point.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "internal.hpp"
 
 namespace qmckl_amd {
@@ -38,6 +40,10 @@ namespace {
 constexpr int kTP = 64;            // points per workgroup
 constexpr int kThreads = 256;      // 4 wavefronts
 constexpr int kRS = 5 * kTP + 2;   // LDS stride between shells (bank spread)
+#ifndef QMCKL_AO_TARGET_GROUPS
+#define QMCKL_AO_TARGET_GROUPS 4096
+#endif
+constexpr int64_t kAoTargetGroups = QMCKL_AO_TARGET_GROUPS;  // workgroups a launch aims at
 
 struct Mono {
   double v, gx, gy, gz, lap;
@@ -85,10 +91,13 @@ ao_kernel(int64_t P, int N, const double *__restrict__ coord, const double *__re
           const int32_t *__restrict__ ao_shell, const int32_t *__restrict__ ao_abc,
           const double *__restrict__ ao_factor, const double *__restrict__ expo,
           const double *__restrict__ coef, const double *__restrict__ nucl_range,
-          const double *__restrict__ shell_table, int S_table, AoOut<OutT> ao) {
+          const double *__restrict__ shell_table, int S_table, AoOut<OutT> ao, int cgroups) {
   extern __shared__ unsigned char s_flag[];  // [NCOMP row tiles][nkb] nonzero flags
   __shared__ double rad[kChunkShells * kRS];
   __shared__ double pt[3][kTP];
+  // this workgroup's chunks: group blockIdx.y of cgroups
+  const int ch_begin = (int)((int64_t)blockIdx.y * nchunks / cgroups);
+  const int ch_end = (int)((int64_t)(blockIdx.y + 1) * nchunks / cgroups);
 
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -110,7 +119,7 @@ ao_kernel(int64_t P, int N, const double *__restrict__ coord, const double *__re
   const double px = pt[0][lane], py = pt[1][lane], pz = pt[2][lane];
   const int64_t ostep = (int64_t)(kThreads / 64) * NCOMP * ao.ld;  // stride between a wavefront's points
 
-  for (int ch = 0; ch < nchunks; ++ch) {
+  for (int ch = ch_begin; ch < ch_end; ++ch) {
     const int s0 = chunks[ch];
     const int ns = chunks[ch + 1] - s0;
 
@@ -217,12 +226,16 @@ ao_kernel(int64_t P, int N, const double *__restrict__ coord, const double *__re
     __syncthreads();  // phase 2 is done with the radial tile
   }
   if (track_flags) {
-    // this workgroup owns rows [NCOMP*p0, NCOMP*(p0+np)): whole 64-row tiles
+    // this workgroup's rows [NCOMP*p0, NCOMP*(p0+np)) are whole 64-row
+    // tiles.  One group: it owns their flags and writes them all; several
+    // groups (chunk ranges) share a tile's flags, which the launcher has
+    // zeroed, and each sets only the ones it found (a 32-AO block can
+    // straddle two groups' chunks).
     const int64_t rt0 = (p0 * NCOMP) >> 6;
     const int64_t rt_end = (NCOMP * (p0 + np) + 63) >> 6;
     for (int i = tid; i < NCOMP * ao.nkb; i += kThreads) {
       const int64_t rt = rt0 + i / ao.nkb;
-      if (rt < rt_end) ao.flags[rt * ao.nkb + (i % ao.nkb)] = s_flag[i];
+      if (rt < rt_end && (cgroups == 1 || s_flag[i] != 0)) ao.flags[rt * ao.nkb + (i % ao.nkb)] = s_flag[i];
     }
   }
 }
@@ -262,6 +275,11 @@ shell_vgl_kernel(int64_t P, int S, int N, const double *__restrict__ coord,
   }
 }
 
+__global__ void __launch_bounds__(kThreads) zero_bytes_kernel(int64_t n, uint8_t *__restrict__ p) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
 }  // namespace
 
 template <int NCOMP, bool FROM_TABLE, typename OutT>
@@ -273,10 +291,23 @@ hipError_t launch_ao(hipStream_t s, const DeviceBasis &b, int64_t P, const doubl
   if (blocks > 0x7fffffff || b.chunks == nullptr || b.nchunks <= 0) return hipErrorInvalidValue;
   if (ao.flags != nullptr && (ao.nkb <= 0 || ao.nkb > kMaxFlagBlocks)) return hipErrorInvalidValue;
   const size_t lds = ao.flags != nullptr ? (size_t)NCOMP * ao.nkb : 0;
-  hipLaunchKernelGGL((ao_kernel<NCOMP, FROM_TABLE, OutT>), dim3((unsigned)blocks), dim3(kThreads), lds,
-                     s, P, (int)b.nucl_num, coord, nucl, b.shells, b.chunks, b.nchunks, b.ao_shell,
-                     b.ao_abc, ao_factor, b.expo, b.coef_norm, b.nucl_range, shell_table, (int)S_table,
-                     ao);
+  // Split each 64-point tile's chunks over several workgroups when there are
+  // few tiles, so the launch has enough workgroups to keep every CU busy to
+  // the end (three fit per CU; a tile's chunks are a long serial loop).
+  int64_t groups = (kAoTargetGroups + blocks - 1) / blocks;
+  if (groups > b.nchunks) groups = b.nchunks;
+  if (groups > 65535) groups = 65535;
+  if (groups < 1) groups = 1;
+  if (groups > 1 && ao.flags != nullptr) {
+    // a kernel, not a memset: the launch may be captured into a graph
+    const int64_t n = (NCOMP * P + 63) / 64 * ao.nkb;
+    hipLaunchKernelGGL(zero_bytes_kernel, dim3((unsigned)std::min<int64_t>((n + kThreads - 1) / kThreads, 4096)),
+                       dim3(kThreads), 0, s, n, ao.flags);
+  }
+  hipLaunchKernelGGL((ao_kernel<NCOMP, FROM_TABLE, OutT>), dim3((unsigned)blocks, (unsigned)groups),
+                     dim3(kThreads), lds, s, P, (int)b.nucl_num, coord, nucl, b.shells, b.chunks, b.nchunks,
+                     b.ao_shell, b.ao_abc, ao_factor, b.expo, b.coef_norm, b.nucl_range, shell_table,
+                     (int)S_table, ao, (int)groups);
   return hipGetLastError();
 }
 

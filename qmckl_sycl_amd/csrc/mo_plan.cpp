@@ -128,6 +128,54 @@ int64_t plan_mo_columns(const uint64_t *words, int64_t M, int64_t nw, int64_t nk
   }
   if (nblk > nfull) chain.push_back(nfull);
 
+  // Balance the wavefronts: the FP64 contraction hands column block p of a
+  // workgroup to wavefront p % 4 and waits at every 16-AO stage for the
+  // wavefront with the most live blocks.  Swap full blocks between
+  // wavefronts while that sum of per-stage maxima falls.
+  {
+    const int64_t nmov = nfull;  // the partial block, if any, stays last
+    std::vector<int32_t> load((size_t)(4 * nkb16), 0);
+    auto bit = [&](int64_t pos, int64_t st) {
+      return (int32_t)((unions[(size_t)(chain[(size_t)pos] * nw + st / 64)] >> (st % 64)) & 1ull);
+    };
+    for (int64_t pos = 0; pos < nblk; ++pos)
+      for (int64_t st = 0; st < nkb16; ++st) load[(size_t)((pos % 4) * nkb16 + st)] += bit(pos, st);
+    auto stage_max = [&](int64_t st) {
+      int32_t m = 0;
+      for (int w = 0; w < 4; ++w) m = std::max(m, load[(size_t)(w * nkb16 + st)]);
+      return m;
+    };
+    for (int sweep = 0; sweep < 8; ++sweep) {
+      bool improved = false;
+      for (int64_t p1 = 0; p1 < nmov; ++p1)
+        for (int64_t p2 = p1 + 1; p2 < nmov; ++p2) {
+          const int64_t w1 = p1 % 4, w2 = p2 % 4;
+          if (w1 == w2) continue;
+          int64_t delta = 0;
+          for (int64_t st = 0; st < nkb16; ++st) {
+            const int32_t d = bit(p2, st) - bit(p1, st);  // w1 gains d, w2 loses d
+            if (d == 0) continue;
+            const int32_t before = stage_max(st);
+            load[(size_t)(w1 * nkb16 + st)] += d;
+            load[(size_t)(w2 * nkb16 + st)] -= d;
+            delta += stage_max(st) - before;
+            load[(size_t)(w1 * nkb16 + st)] -= d;
+            load[(size_t)(w2 * nkb16 + st)] += d;
+          }
+          if (delta < 0) {
+            for (int64_t st = 0; st < nkb16; ++st) {
+              const int32_t d = bit(p2, st) - bit(p1, st);
+              load[(size_t)(w1 * nkb16 + st)] += d;
+              load[(size_t)(w2 * nkb16 + st)] -= d;
+            }
+            std::swap(chain[(size_t)p1], chain[(size_t)p2]);
+            improved = true;
+          }
+        }
+      if (!improved) break;
+    }
+  }
+
   int64_t live = 0;
   for (int64_t k = 0; k < nblk; ++k) {
     const int64_t g = chain[(size_t)k];

@@ -1,7 +1,9 @@
 // Differentiation form: adjoint of a weighted sum of mo_vgl with respect to
 // nuclear coordinates, primitive exponents and MO coefficients.
 //   G      = w C                  (FP64 MFMA GEMM, [5P][A], C in fragment
-//                                  order)
+//                                  order; with a column plan, K runs over
+//                                  the MOs in the plan's order and zero
+//                                  coefficient blocks are skipped)
 //   dL/dC  = (AO^T w)^T           (FP64 MFMA GEMM with AO read K-major and w
 //                                  in fragment order, split over points,
 //                                  fixed-order reduction)
@@ -10,6 +12,8 @@
 // There is no counterpart in the reference; the forward arithmetic it
 // differentiates is src/qmckl_ao_sycl.cpp:40-95 and 291-422 and
 // src/qmckl_mo_sycl.cpp:23-50.
+#include <vector>
+
 #include "internal.hpp"
 
 using namespace qmckl_amd;
@@ -51,18 +55,55 @@ qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_adjoint_device(qmckl_context_de
         !grow_d(c, &mo.adj_partR, &mo.adj_partR_cap, nblk * S * 3) ||
         !grow_d(c, &mo.adj_partA, &mo.adj_partA_cap, nblk * Np))
       return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
-    if (!mo.adj_cfrag_ok) {  // the coefficients [M][A] as the B operand, fragment order
+    ColPlan plan;
+    const std::vector<uint64_t> *host_blk = nullptr;
+    e = mo_f64_column_plan(c, &plan, &host_blk);
+    if (e != hipSuccess) return hip_check(context, e, fn);
+    if (!mo.adj_cfrag_ok) {
+      // The coefficients [M][A] as the B operand in fragment order.  With a
+      // column plan, K runs over the MOs in the plan's order (rows gathered),
+      // and the plan's live bits, transposed to (16-AO column block, 16-MO K
+      // block), let the contraction skip the zero blocks.
+      mo.adj_planned = plan.col_mo != nullptr && host_blk != nullptr;
       const int64_t len = round_up(A, 64 * f64_frag_cbw(A)) * round_up(M, 16);
       if (!grow_d(c, &mo.adj_cfrag, &mo.adj_cfrag_cap, len))
         return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
-      e = launch_frag_b_f64(c->stream, M, A, mo.coefficient, A, mo.adj_cfrag, &mo.adj_cfrag_nk4);
+      e = launch_frag_b_f64(c->stream, M, A, mo.coefficient, A, mo.adj_cfrag, &mo.adj_cfrag_nk4, nullptr,
+                            mo.adj_planned ? plan.col_mo : nullptr);
+      if (e == hipSuccess && mo.adj_planned) {
+        const int64_t nmb = (M + 15) / 16, nab = (A + 15) / 16, nwt = (nmb + 63) / 64;
+        std::vector<uint64_t> t((size_t)(nab * nwt), 0);
+        for (int64_t mb = 0; mb < nmb; ++mb)
+          for (int64_t ab = 0; ab < nab; ++ab)
+            if (((*host_blk)[(size_t)(mb * plan.nw + ab / 64)] >> (ab % 64)) & 1ull)
+              t[(size_t)(ab * nwt + mb / 64)] |= 1ull << (mb % 64);
+        dfree(c, mo.adj_blk);
+        mo.adj_blk = static_cast<uint64_t *>(dmalloc(c, sizeof(uint64_t) * t.size()));
+        if (mo.adj_blk == nullptr) return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
+        e = hipMemcpyAsync(mo.adj_blk, t.data(), sizeof(uint64_t) * t.size(), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = sync(c);
+        mo.adj_blk_nw = nwt;
+      }
       if (e != hipSuccess) return hip_check(context, e, fn);
       mo.adj_cfrag_ok = true;
     }
+    if (mo.adj_planned && !grow_d(c, &mo.adj_wperm, &mo.adj_wperm_cap, 5 * P * M))
+      return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
     {
       KernelTimer t(c, "adjoint_g_gemm_f64");
-      e = launch_mo_gemm_f64_frag(c->stream, 5 * P, M, A, weight, M, mo.adj_cfrag, mo.adj_cfrag_nk4, mo.adj_G,
-                                  A);
+      if (mo.adj_planned) {
+        // w's columns in the plan's MO order, then G = w' C' skipping zero blocks
+        e = launch_gather_cols(c->stream, 5 * P, M, weight, M, plan.col_mo, mo.adj_wperm, M);
+        ColPlan gp;
+        gp.blk = mo.adj_blk;
+        gp.nw = mo.adj_blk_nw;
+        gp.nblk = (A + 15) / 16;
+        if (e == hipSuccess)
+          e = launch_mo_gemm_f64_frag(c->stream, 5 * P, M, A, mo.adj_wperm, M, mo.adj_cfrag, mo.adj_cfrag_nk4,
+                                      mo.adj_G, A, nullptr, 0, gp);
+      } else {
+        e = launch_mo_gemm_f64_frag(c->stream, 5 * P, M, A, weight, M, mo.adj_cfrag, mo.adj_cfrag_nk4, mo.adj_G, A);
+      }
       t.stop();
     }
     // primitives that belong to no shell keep a zero partial

@@ -166,6 +166,7 @@ struct ColPlanSlot {
   int32_t *col_mo = nullptr;
   uint64_t *blk = nullptr;
   double live = 1.0;  // live (16-MO, 16-AO) blocks of the planned order over all of them
+  std::vector<uint64_t> host_blk;  // host copy of blk (when kept)
 };
 
 struct MoBasis {
@@ -216,6 +217,15 @@ struct MoBasis {
   bool adj_cfrag_ok = false;
   double *adj_wfrag = nullptr;
   int64_t adj_wfrag_cap = 0;
+  // with a column plan, G = w C runs with K over the MOs in the plan's
+  // order: the weights' columns gathered into that order (per call), the
+  // coefficient fragments built with their rows in it, and the live bits of
+  // (16-AO column block, 16-MO K block) pairs [ceil(A/16)][adj_blk_nw]
+  double *adj_wperm = nullptr;
+  int64_t adj_wperm_cap = 0;
+  uint64_t *adj_blk = nullptr;
+  int64_t adj_blk_nw = 0;
+  bool adj_planned = false;
   int32_t uninitialized = (1 << 2) - 1;
   bool provided = false;
 };
@@ -342,9 +352,14 @@ hipError_t launch_mo_gemm_f64(hipStream_t s, int64_t rows, int64_t K, int64_t Nc
 // Bf holds (Ncol rounded up to 64 * f64_frag_cbw(Ncol)) x (K rounded up to
 // 16) doubles; *nk4 receives the K steps of 4 per column block.
 int f64_frag_cbw(int64_t Ncol);
-// Column j of Bf takes column col_mo[j] of B (null: column j).
+// Column j of Bf takes column col_mo[j] of B (null: column j), K row k
+// takes row row_map[k] of B (null: row k).
 hipError_t launch_frag_b_f64(hipStream_t s, int64_t K, int64_t Ncol, const double *B, int64_t ldb,
-                             double *Bf, int64_t *nk4, const int32_t *col_mo = nullptr);
+                             double *Bf, int64_t *nk4, const int32_t *col_mo = nullptr,
+                             const int32_t *row_map = nullptr);
+// out[r][j] = in[r][map[j]] for r < rows, j < cols.
+hipError_t launch_gather_cols(hipStream_t s, int64_t rows, int64_t cols, const double *in, int64_t ld_in,
+                              const int32_t *map, double *out, int64_t ld_out);
 // With a column plan, Bf is in the plan's column order (launch_frag_b_f64
 // with plan.col_mo), column j of the product is stored at C column
 // plan.col_mo[j], and the MFMAs and coefficient loads of (column block,
@@ -357,6 +372,10 @@ hipError_t launch_mo_gemm_f64_frag(hipStream_t s, int64_t R, int64_t K, int64_t 
 // MO m has a nonzero (or NaN) coefficient among the AOs [16s, 16s + 16).
 hipError_t launch_coef_blocks(hipStream_t s, int64_t M, int64_t K, const double *coef, int64_t nw,
                               uint64_t *words);
+// The column plan of the FP64 contraction (mo.cpp; built on first use after
+// finalize): *plan is empty when the MO order is kept; *host_blk points to
+// the host copy of its live bits.
+hipError_t mo_f64_column_plan(Context *c, ColPlan *plan, const std::vector<uint64_t> **host_blk);
 // Column order from that map (mo_plan.cpp) for the M MOs whose words start
 // at `words`: the MOs in groups of 16 that each touch few AO blocks, the
 // groups chained so that neighbours touch similar blocks.  Fills col_mo [M]

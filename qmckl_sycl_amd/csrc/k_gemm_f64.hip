@@ -161,14 +161,28 @@ constexpr int kOutRows = 8;  // output rows per pass of the planned-order epilog
 
 __global__ void __launch_bounds__(kThreads)
 frag_b_f64_kernel(int64_t K, int64_t Ncol, const double *__restrict__ B, int64_t ldb, int64_t nk4,
-                  int64_t total, const int32_t *__restrict__ col_mo, double *__restrict__ Bf) {
+                  int64_t total, const int32_t *__restrict__ col_mo, const int32_t *__restrict__ row_map,
+                  double *__restrict__ Bf) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t frag = i >> 6;
     const int l = (int)(i & 63);
     const int64_t cb = frag / nk4, ks = frag - cb * nk4;
     const int64_t k = 4 * ks + l / 16, col = 16 * cb + l % 16;
-    Bf[i] = (k < K && col < Ncol) ? B[k * ldb + (col_mo != nullptr ? (int64_t)col_mo[col] : col)] : 0.0;
+    Bf[i] = (k < K && col < Ncol) ? B[(row_map != nullptr ? (int64_t)row_map[k] : k) * ldb +
+                                      (col_mo != nullptr ? (int64_t)col_mo[col] : col)]
+                                  : 0.0;
+  }
+}
+
+// out[r][j] = in[r][map[j]]
+__global__ void __launch_bounds__(kThreads)
+gather_cols_kernel(int64_t rows, int64_t cols, const double *__restrict__ in, int64_t ld_in,
+                   const int32_t *__restrict__ map, double *__restrict__ out, int64_t ld_out) {
+  const int64_t n = rows * cols;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / cols, j = i - r * cols;
+    out[r * ld_out + j] = in[r * ld_in + map[j]];
   }
 }
 
@@ -461,7 +475,7 @@ int f64_frag_cbw(int64_t Ncol) {
 }
 
 hipError_t launch_frag_b_f64(hipStream_t s, int64_t K, int64_t Ncol, const double *B, int64_t ldb,
-                             double *Bf, int64_t *nk4, const int32_t *col_mo) {
+                             double *Bf, int64_t *nk4, const int32_t *col_mo, const int32_t *row_map) {
   *nk4 = (K + kBK - 1) / kBK * (kBK / 4);
   const int64_t npad = round_up(Ncol, 64 * f64_frag_cbw(Ncol));
   const int64_t total = (npad / 16) * (*nk4) * 64;
@@ -469,7 +483,18 @@ hipError_t launch_frag_b_f64(hipStream_t s, int64_t K, int64_t Ncol, const doubl
   int64_t g = (total + kThreads - 1) / kThreads;
   if (g > 65536) g = 65536;
   hipLaunchKernelGGL(frag_b_f64_kernel, dim3((unsigned)g), dim3(kThreads), 0, s, K, Ncol, B, ldb, *nk4, total,
-                     col_mo, Bf);
+                     col_mo, row_map, Bf);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_cols(hipStream_t s, int64_t rows, int64_t cols, const double *in, int64_t ld_in,
+                              const int32_t *map, double *out, int64_t ld_out) {
+  const int64_t n = rows * cols;
+  if (n <= 0) return hipSuccess;
+  int64_t g = (n + kThreads - 1) / kThreads;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(gather_cols_kernel, dim3((unsigned)g), dim3(kThreads), 0, s, rows, cols, in, ld_in, map, out,
+                     ld_out);
   return hipGetLastError();
 }
 

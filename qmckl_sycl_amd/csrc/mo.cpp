@@ -109,6 +109,7 @@ hipError_t col_plan_for(Context *c, int64_t group, ColPlan *out) {
         e = hipMemcpyAsync(slot->blk, blk.data(), sizeof(uint64_t) * blk.size(), hipMemcpyHostToDevice, c->stream);
       if (e == hipSuccess) e = sync(c);
       if (e != hipSuccess) return e;
+      slot->host_blk.swap(blk);
     }
   }
   if (slot->col_mo != nullptr) {
@@ -122,6 +123,23 @@ hipError_t col_plan_for(Context *c, int64_t group, ColPlan *out) {
 
 // the FP64 contraction's workgroup width (launch_mo_gemm_f64_frag)
 int64_t f64_group(int64_t M) { return 64 * (int64_t)f64_frag_cbw(M); }
+
+}  // namespace
+
+namespace qmckl_amd {
+
+hipError_t mo_f64_column_plan(Context *c, ColPlan *plan, const std::vector<uint64_t> **host_blk) {
+  const hipError_t e = col_plan_for(c, f64_group(c->mo.mo_num), plan);
+  *host_blk = nullptr;
+  if (e == hipSuccess && plan->col_mo != nullptr)
+    for (const ColPlanSlot &p : c->mo.col_plans)
+      if (p.col_mo == plan->col_mo) *host_blk = &p.host_blk;
+  return e;
+}
+
+}  // namespace qmckl_amd
+
+namespace {
 
 const char *gemm_timer_name(qmckl_precision_device form) {
   switch (form) {

@@ -15,7 +15,9 @@
 // linearly through three per-shell sums U0, U1, U2.
 //
 // The primitives are recomputed here, not stored by the forward pass.
-// Work: workgroup = 64 points, wavefront = one shell at a time, lane =
+// Work: workgroup = 64 points and an interleaved share of the shells (the
+// shells of a point block are split over several workgroups so the launch
+// has enough wavefronts), wavefront = one shell at a time, lane =
 // point; per (point, shell) the lane evaluates g..g''', loops over the
 // shell's AOs (monomials to third order), and reduces over the wavefront in
 // a fixed shuffle order.  Each (workgroup, shell) writes one partial dR and
@@ -63,7 +65,7 @@ ao_adjoint_kernel(int64_t P, int S, int A, int N, const double *__restrict__ coo
                   const int32_t *__restrict__ ao_abc, const double *__restrict__ ao_factor,
                   const double *__restrict__ expo, const double *__restrict__ coef,
                   const double *__restrict__ nucl_range, const double *__restrict__ G,
-                  double *__restrict__ partR, double *__restrict__ partA, int64_t Np) {
+                  double *__restrict__ partR, double *__restrict__ partA, int64_t Np, int sgroups) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t blk = blockIdx.x;
   const int64_t p = blk * kTP + lane;
@@ -74,7 +76,9 @@ ao_adjoint_kernel(int64_t P, int S, int A, int N, const double *__restrict__ coo
   double *myR = partR + blk * (int64_t)S * 3;
   double *myA = partA + blk * Np;
 
-  for (int si = wave; si < S; si += kThreads / 64) {
+  // shells of this workgroup: every (kThreads/64 * sgroups)-th from its
+  // group's first, one wavefront each
+  for (int si = (int)blockIdx.y * (kThreads / 64) + wave; si < S; si += (kThreads / 64) * sgroups) {
     const ShellDesc d = shells[si];
     const double dx = rx - nucl[d.nucl];
     const double dy = ry - nucl[N + d.nucl];
@@ -197,9 +201,15 @@ hipError_t launch_ao_adjoint(hipStream_t s, const DeviceBasis &b, int64_t P, con
                              double *partR, double *partA) {
   if (P <= 0) return hipSuccess;
   const int64_t blocks = (P + kTP - 1) / kTP;
-  hipLaunchKernelGGL(ao_adjoint_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, P,
+  // about 4,096 workgroups, at most one wavefront per shell
+  int64_t groups = (4096 + blocks - 1) / blocks;
+  const int64_t gmax = (b.shell_num + kThreads / 64 - 1) / (kThreads / 64);
+  if (groups > gmax) groups = gmax;
+  if (groups < 1) groups = 1;
+  if (blocks > 0x7fffffff || groups > 65535) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(ao_adjoint_kernel, dim3((unsigned)blocks, (unsigned)groups), dim3(kThreads), 0, s, P,
                      (int)b.shell_num, (int)b.ao_num, (int)b.nucl_num, coord, nucl, b.shells, b.ao_abc,
-                     ao_factor, b.expo, b.coef_norm, b.nucl_range, G, partR, partA, b.prim_num);
+                     ao_factor, b.expo, b.coef_norm, b.nucl_range, G, partR, partA, b.prim_num, (int)groups);
   return hipGetLastError();
 }
 

@@ -256,6 +256,8 @@ qmckl_exit_code_device qmckl_context_destroy_device(qmckl_context_device context
   c->device_allocs.clear();
   for (auto &kv : c->host_allocs) std::free(kv.first);
   c->host_allocs.clear();
+  if (c->batch_event) (void)hipEventDestroy(c->batch_event);
+  c->batch_event = nullptr;
   if (c->stream) (void)hipStreamDestroy(c->stream);
   c->stream = nullptr;
   c->tag = kInvalidTag;

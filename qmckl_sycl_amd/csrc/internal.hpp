@@ -169,6 +169,25 @@ struct ColPlanSlot {
   std::vector<uint64_t> host_blk;  // host copy of blk (when kept)
 };
 
+// One molecule's split contraction in a persistent batch launch (the
+// arguments of launch_mo_gemm_mixed, the tile width and its work items).
+struct MixedGemmJob {
+  int64_t P, npt, K, Ncol;
+  const float *A;
+  int64_t lda;
+  const float *rowmax;
+  const uint16_t *Bs;
+  int64_t ldb;
+  const float *colinv;
+  const uint8_t *flags;
+  int nkb;
+  const int32_t *perm;
+  double *C;
+  int64_t ldc;
+  ColPlan plan;
+  int cbw;                     // tile width: 64 * cbw MOs
+  int64_t item0, gx, items;    // first item, row-tile halves, items (gx x column groups)
+};
 struct MoBasis {
   int64_t mo_num = 0;
   double *coefficient = nullptr;    // [M][A]
@@ -248,6 +267,16 @@ struct Context {
   bool defer_sync = false;
   // set while a batch graph is captured: kernel timers record nothing
   bool capturing = false;
+  // set while an eager batch enqueues this context: a split-form
+  // contraction of tile width <= 3 is appended here instead of launched (the
+  // batch runs all of them in persistent launches)
+  std::vector<MixedGemmJob> *collect_jobs = nullptr;
+  // the batch's job table and work counters (device), and an event marking
+  // the end of this context's part of a batch
+  MixedGemmJob *batch_jobs = nullptr;
+  int64_t batch_jobs_cap = 0;
+  std::vector<MixedGemmJob> batch_jobs_host;  // the last eager batch's table, as uploaded
+  hipEvent_t batch_event = nullptr;
   struct TimerAcc {
     int64_t launches = 0;
     double total_ms = 0.0;
@@ -427,6 +456,18 @@ hipError_t launch_mo_gemm_mixed(hipStream_t s, int form, int64_t P, int64_t K, i
                                 const float *A, int64_t lda, const float *rowmax, const void *B, int64_t ldb,
                                 const float *colinv, const uint8_t *flags, int nkb, const int32_t *perm,
                                 double *C, int64_t ldc, ColPlan plan = ColPlan());
+// Fills *job for a split form (the checks of launch_mo_gemm_mixed); item0 is
+// left to the caller.  The FP32 form has no persistent launch.
+hipError_t make_mixed_gemm_job(int form, int64_t P, int64_t K, int64_t Ncol, const float *A, int64_t lda,
+                               const float *rowmax, const void *B, int64_t ldb, const float *colinv,
+                               const uint8_t *flags, int nkb, const int32_t *perm, double *C, int64_t ldc,
+                               ColPlan plan, MixedGemmJob *job);
+// One launch for every job of the same split form and tile width cbw: jobs
+// [njobs] in device memory with consecutive item ranges covering [0, total),
+// counter a device int the launch zeroes first (a kernel, so it can be
+// captured).
+hipError_t launch_mo_gemm_mixed_persistent(hipStream_t s, int form, int cbw, const MixedGemmJob *jobs, int njobs,
+                                           int64_t total, int *counter);
 hipError_t launch_strided_copy(hipStream_t s, int64_t rows, int64_t cols, const double *in,
                                int64_t ld_in, double *out, int64_t ld_out);
 

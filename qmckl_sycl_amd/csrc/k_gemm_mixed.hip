@@ -175,29 +175,29 @@ __device__ inline void split8(const float (&v)[8], uint4 (&out)[Terms<SP>::n]) {
   }
 }
 
+// One 32-row x 64*CBW-column tile (bx, by) of a split contraction; the
+// workgroup's LDS comes in as pointers so that one kernel can hold tiles of
+// several widths (the persistent batch kernel below).
 template <int SP, int CBW, int WR>
-__global__ void __attribute__((amdgpu_flat_work_group_size(WR * 8, WR * 8), amdgpu_waves_per_eu(2, 2)))
-gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *__restrict__ A,
-                  int64_t lda, const float *__restrict__ rowmax, const uint16_t *__restrict__ Bs,
-                  int64_t ldb, const float *__restrict__ colinv, const uint8_t *__restrict__ flags, int nkb,
-                  const int32_t *__restrict__ perm, double *__restrict__ C, int64_t ldc, ColPlan plan) {
+__device__ __forceinline__ void split_tile(int64_t bx, int64_t by, int64_t P, int64_t npt, int64_t K,
+                                           int64_t Ncol, const float *__restrict__ A, int64_t lda,
+                                           const float *__restrict__ rowmax, const uint16_t *__restrict__ Bs,
+                                           int64_t ldb, const float *__restrict__ colinv,
+                                           const uint8_t *__restrict__ flags, int nkb,
+                                           const int32_t *__restrict__ perm, double *__restrict__ C, int64_t ldc,
+                                           ColPlan plan, uint16_t *sA, float *s_inv, int *s_list, uint64_t *s_live,
+                                           int *s_count, double *s_out) {
   constexpr int T = Terms<SP>::n;
   constexpr int RB = 2;            // 16-row blocks per wavefront
   constexpr int KQ = 8;            // K per thread in the A slab
-  __shared__ uint16_t sA[T][WR * kLDS];
-  __shared__ float s_inv[WR];
-  __shared__ int s_list[kMaxStages];
-  __shared__ uint64_t s_live[kMaxStages];
-  __shared__ int s_count;
-  __shared__ double s_out[kOutRows * 64 * CBW];  // output rows in MO order (planned columns)
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const int cq = __builtin_amdgcn_readfirstlane(wave & 3), rh = wave >> 2;  // column quarter, row half
-  const int64_t rt = blockIdx.x / (64 / WR);  // row tile: component c, tile rows 64*pt ..
-  const int r0 = (int)(blockIdx.x % (64 / WR)) * WR;  // this workgroup's first row of the tile
+  const int64_t rt = bx / (64 / WR);  // row tile: component c, tile rows 64*pt ..
+  const int r0 = (int)(bx % (64 / WR)) * WR;  // this workgroup's first row of the tile
   const int64_t comp = rt / npt, p0 = (rt - comp * npt) * 64 + r0;
-  const int64_t gb0 = (int64_t)blockIdx.y * 4 * CBW;  // the workgroup's first column block
+  const int64_t gb0 = by * 4 * CBW;  // the workgroup's first column block
   const int nst = (int)((K + kStage - 1) / kStage);
 
   // stage list: live stages and, per stage, bit b (first half) and 32 + b
@@ -240,10 +240,10 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
         }
         base += __popcll(m);
       }
-      if (lane == 0) s_count = base;
+      if (lane == 0) *s_count = base;
     }
     __syncthreads();
-    count = s_count;
+    count = *s_count;
   }
   // this wavefront's live column blocks in half ks of stage s: bit 4j for
   // block gb0 + cq + 4j
@@ -326,7 +326,7 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
       uint4 o[T];
       split8<SP>(v, o);
 #pragma unroll
-      for (int t = 0; t < T; ++t) *reinterpret_cast<uint4 *>(&sA[t][ar * kLDS + aq + 8 * h]) = o[t];
+      for (int t = 0; t < T; ++t) *reinterpret_cast<uint4 *>(&sA[t * (WR * kLDS) + ar * kLDS + aq + 8 * h]) = o[t];
     }
     __syncthreads();
     const bool more = s + 1 < count;
@@ -341,7 +341,7 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
       for (int t = 0; t < T; ++t)
 #pragma unroll
         for (int r = 0; r < RB; ++r)
-          a[t][r] = *reinterpret_cast<const uint4 *>(&sA[t][((rh * RB + r) * 16 + li) * kLDS + ks * 32 + 8 * lk]);
+          a[t][r] = *reinterpret_cast<const uint4 *>(&sA[t * (WR * kLDS) + ((rh * RB + r) * 16 + li) * kLDS + ks * 32 + 8 * lk]);
       // product by product over all RB x CBW tiles: RB*CBW independent
       // accumulators between dependent MFMAs.  Dead tiles run on the zeros
       // their loads returned (branching around their MFMAs measured slower:
@@ -418,6 +418,69 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
       double *dst = C + (p * 5 + comp) * ldc + g_lo;
       for (int c = tid; c < g_n; c += WR * 8) dst[c] = s_out[rl * (64 * CBW) + c];
     }
+  }
+}
+
+
+template <int SP, int CBW, int WR>
+__global__ void __attribute__((amdgpu_flat_work_group_size(WR * 8, WR * 8), amdgpu_waves_per_eu(2, 2)))
+gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *__restrict__ A,
+                  int64_t lda, const float *__restrict__ rowmax, const uint16_t *__restrict__ Bs,
+                  int64_t ldb, const float *__restrict__ colinv, const uint8_t *__restrict__ flags, int nkb,
+                  const int32_t *__restrict__ perm, double *__restrict__ C, int64_t ldc, ColPlan plan) {
+  __shared__ uint16_t sA[Terms<SP>::n * WR * kLDS];
+  __shared__ float s_inv[WR];
+  __shared__ int s_list[kMaxStages];
+  __shared__ uint64_t s_live[kMaxStages];
+  __shared__ int s_count;
+  __shared__ double s_out[kOutRows * 64 * CBW];  // output rows in MO order (planned columns)
+  split_tile<SP, CBW, WR>(blockIdx.x, blockIdx.y, P, npt, K, Ncol, A, lda, rowmax, Bs, ldb, colinv, flags, nkb, perm,
+                          C, ldc, plan, sA, s_inv, s_list, s_live, &s_count, s_out);
+}
+
+// Persistent form over a batch: 2 workgroups per CU take (molecule, row
+// tile, column group) items from an atomic counter until the batch is done;
+// jobs[j] describes molecule j's contraction (all of tile width CBW) and
+// owns items [item0, item0 + items), x-major over its gx row-tile halves.
+// Every workgroup leaves the loop once the counter passes `total`.
+template <int SP, int CBW>
+__global__ void __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(2, 2)))
+gemm_split_persistent_kernel(const MixedGemmJob *__restrict__ jobs, int njobs, int64_t total, int *counter) {
+  constexpr int WR = 32;
+  __shared__ uint16_t sA[Terms<SP>::n * WR * kLDS];
+  __shared__ float s_inv[WR];
+  __shared__ int s_list[kMaxStages];
+  __shared__ uint64_t s_live[kMaxStages];
+  __shared__ int s_count;
+  __shared__ double s_out[kOutRows * 64 * CBW];
+  __shared__ int64_t s_item;
+  __shared__ int s_job;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      const int64_t item = atomicAdd(counter, 1);
+      int j = -1;
+      if (item < total) {  // the last job whose first item is <= item
+        int lo = 0, hi = njobs - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) / 2;
+          if (jobs[mid].item0 <= item) lo = mid;
+          else hi = mid - 1;
+        }
+        j = lo;
+      }
+      s_item = item;
+      s_job = j;
+    }
+    __syncthreads();  // also: the previous tile is finished with the LDS
+    const int64_t item = s_item;
+    const int j = __builtin_amdgcn_readfirstlane(s_job);
+    __syncthreads();  // s_item is read before the next fetch overwrites it
+    if (item >= total) return;
+    const MixedGemmJob &g = jobs[j];
+    const int64_t local = item - g.item0;
+    const int64_t bx = local % g.gx, by = local / g.gx;
+    split_tile<SP, CBW, WR>(bx, by, g.P, g.npt, g.K, g.Ncol, g.A, g.lda, g.rowmax, g.Bs, g.ldb, g.colinv, g.flags,
+                            g.nkb, g.perm, g.C, g.ldc, g.plan, sA, s_inv, s_list, s_live, &s_count, s_out);
   }
 }
 
@@ -653,6 +716,83 @@ hipError_t launch_split_coef(hipStream_t s, int form, int64_t M, int64_t K, cons
       break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t make_mixed_gemm_job(int form, int64_t P, int64_t K, int64_t Ncol, const float *A, int64_t lda,
+                               const float *rowmax, const void *B, int64_t ldb, const float *colinv,
+                               const uint8_t *flags, int nkb, const int32_t *perm, double *C, int64_t ldc,
+                               ColPlan plan, MixedGemmJob *job) {
+  if (form != QMCKL_AMD_SPLIT_FP16X2 && form != QMCKL_AMD_SPLIT_BF16X2 && form != QMCKL_AMD_SPLIT_BF16X3)
+    return hipErrorInvalidValue;
+  if (P <= 0 || Ncol <= 0 || K <= 0 || (lda % 64) != 0 || (ldb % 64) != 0 || lda < K || ldb < K)
+    return hipErrorInvalidValue;
+  if (flags != nullptr && (nkb > kMaxFlagBlocks || (int64_t)nkb * 32 < K)) return hipErrorInvalidValue;
+  if (plan.blk != nullptr && ((K + kStage - 1) / kStage > kMaxStages || plan.nw * 64 * 16 < K || plan.nblk < 0))
+    return hipErrorInvalidValue;
+  const int T = form == QMCKL_AMD_SPLIT_BF16X3 ? 3 : 2;
+  const int cbw = split_wg_cols(form, Ncol) / 64;
+  if ((int64_t)4 * cbw * (ldb / 32) * T * 1024 >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
+  const int64_t npt = (P + 63) / 64;
+  job->P = P;
+  job->npt = npt;
+  job->K = K;
+  job->Ncol = Ncol;
+  job->A = A;
+  job->lda = lda;
+  job->rowmax = rowmax;
+  job->Bs = static_cast<const uint16_t *>(B);
+  job->ldb = ldb;
+  job->colinv = colinv;
+  job->flags = flags;
+  job->nkb = nkb;
+  job->perm = perm;
+  job->C = C;
+  job->ldc = ldc;
+  job->plan = plan;
+  job->cbw = cbw;
+  job->item0 = 0;
+  job->gx = 5 * npt * 2;  // 32-row halves of the 64-row tiles
+  job->items = job->gx * ((Ncol + 64 * cbw - 1) / (64 * cbw));
+  return hipSuccess;
+}
+
+namespace {
+__global__ void zero_int_kernel(int *p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
+}
+}  // namespace
+
+hipError_t launch_mo_gemm_mixed_persistent(hipStream_t s, int form, int cbw, const MixedGemmJob *jobs, int njobs,
+                                           int64_t total, int *counter) {
+  if (njobs <= 0 || total <= 0) return hipSuccess;
+  if (total >= 0x7fffffff - 65536) return hipErrorInvalidValue;  // the counter is an int
+  int dev = 0, cus = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  const unsigned grid = (unsigned)(2 * (cus > 0 ? cus : 1));  // two workgroups per CU
+  hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, s, counter);
+#define QMCKL_PERSISTENT(SPV, W)                                                                            \
+  hipLaunchKernelGGL((gemm_split_persistent_kernel<SPV, W>), dim3(grid), dim3(256), 0, s, jobs, njobs, total, \
+                     counter)
+  switch (form * 8 + cbw) {
+    case QMCKL_AMD_SPLIT_FP16X2 * 8 + 1: QMCKL_PERSISTENT(QMCKL_AMD_SPLIT_FP16X2, 1); break;
+    case QMCKL_AMD_SPLIT_FP16X2 * 8 + 2: QMCKL_PERSISTENT(QMCKL_AMD_SPLIT_FP16X2, 2); break;
+    case QMCKL_AMD_SPLIT_FP16X2 * 8 + 3: QMCKL_PERSISTENT(QMCKL_AMD_SPLIT_FP16X2, 3); break;
+    case QMCKL_AMD_SPLIT_FP16X2 * 8 + 4: QMCKL_PERSISTENT(QMCKL_AMD_SPLIT_FP16X2, 4); break;
+    case QMCKL_AMD_SPLIT_FP16X2 * 8 + 5: QMCKL_PERSISTENT(QMCKL_AMD_SPLIT_FP16X2, 5); break;
+    case QMCKL_AMD_SPLIT_BF16X2 * 8 + 1: QMCKL_PERSISTENT(QMCKL_AMD_SPLIT_BF16X2, 1); break;
+    case QMCKL_AMD_SPLIT_BF16X2 * 8 + 2: QMCKL_PERSISTENT(QMCKL_AMD_SPLIT_BF16X2, 2); break;
+    case QMCKL_AMD_SPLIT_BF16X2 * 8 + 3: QMCKL_PERSISTENT(QMCKL_AMD_SPLIT_BF16X2, 3); break;
+    case QMCKL_AMD_SPLIT_BF16X2 * 8 + 4: QMCKL_PERSISTENT(QMCKL_AMD_SPLIT_BF16X2, 4); break;
+    case QMCKL_AMD_SPLIT_BF16X2 * 8 + 5: QMCKL_PERSISTENT(QMCKL_AMD_SPLIT_BF16X2, 5); break;
+    case QMCKL_AMD_SPLIT_BF16X3 * 8 + 1: QMCKL_PERSISTENT(QMCKL_AMD_SPLIT_BF16X3, 1); break;
+    case QMCKL_AMD_SPLIT_BF16X3 * 8 + 2: QMCKL_PERSISTENT(QMCKL_AMD_SPLIT_BF16X3, 2); break;
+    case QMCKL_AMD_SPLIT_BF16X3 * 8 + 3: QMCKL_PERSISTENT(QMCKL_AMD_SPLIT_BF16X3, 3); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef QMCKL_PERSISTENT
   return hipGetLastError();
 }
 

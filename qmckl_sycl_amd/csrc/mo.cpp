@@ -8,6 +8,7 @@
 // is one GEMM with 5P rows, run by the FP64 MFMA kernel on a copy of the
 // coefficients in MFMA fragment order made at finalize.
 #include <algorithm>
+#include <cstring>
 #include <mutex>
 #include <unordered_map>
 #include <vector>
@@ -213,7 +214,17 @@ qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Contex
                            groups, ao.ao_f32, ao.ao_f32_rowmax, flags, flags ? nkb : 0);
     t.stop();
   }
-  if (e == hipSuccess) {
+  // In an eager batch, split forms of tile width <= 3 leave their
+  // contraction to the batch's persistent launches (wider tiles need more
+  // registers than the persistent kernel has to spare).
+  const bool collect = c->collect_jobs != nullptr && form != QMCKL_PRECISION_FP32_DEVICE &&
+                       split_wg_cols(form, M) <= 192;
+  if (e == hipSuccess && collect) {
+    MixedGemmJob job;
+    e = make_mixed_gemm_job(form, P, A, M, ao.ao_f32, ld, ao.ao_f32_rowmax, mo.coef_split, ld, mo.coef_colinv,
+                            flags, flags ? nkb : 0, perm, dst, M, plan, &job);
+    if (e == hipSuccess) c->collect_jobs->push_back(job);
+  } else if (e == hipSuccess) {
     KernelTimer t(c, gemm_timer_name(form));
     e = launch_mo_gemm_mixed(c->stream, form, P, A, M, ao.ao_f32, ld, ao.ao_f32_rowmax, mo.coef_split, ld,
                              mo.coef_colinv, flags, flags ? nkb : 0, perm, dst, M, plan);
@@ -557,8 +568,100 @@ qmckl_exit_code_device qmckl_mo_basis_select_mo_device(qmckl_context_device cont
   return qmckl_finalize_mo_basis_device(context);
 }
 
+}  // extern "C"
+
+namespace {
+
+// The split-form contractions a batch collected, grouped by (form, tile
+// width): one persistent launch per group, items numbered from 0 within each.
+struct PersistentGroup {
+  int form, cbw;
+  size_t first, n;
+  int64_t total;
+};
+struct PersistentPlan {
+  std::vector<MixedGemmJob> sorted;
+  std::vector<PersistentGroup> groups;
+  size_t table_bytes() const { return sizeof(MixedGemmJob) * sorted.size() + sizeof(int) * groups.size(); }
+};
+
+PersistentPlan plan_persistent(const std::vector<MixedGemmJob> &jobs, const std::vector<Context *> &job_ctx) {
+  PersistentPlan pp;
+  std::vector<size_t> order(jobs.size());
+  for (size_t i = 0; i < jobs.size(); ++i) order[i] = i;
+  // one job per collecting context, in context order: its form is that
+  // context's precision
+  auto key = [&](size_t i) { return (int)job_ctx[i]->mo.precision * 8 + jobs[i].cbw; };
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return key(x) < key(y); });
+  for (size_t k = 0; k < order.size(); ++k) {
+    MixedGemmJob j = jobs[order[k]];
+    const int form = (int)job_ctx[order[k]]->mo.precision;
+    if (pp.groups.empty() || pp.groups.back().form != form || pp.groups.back().cbw != j.cbw)
+      pp.groups.push_back(PersistentGroup{form, j.cbw, k, 0, 0});
+    PersistentGroup &g = pp.groups.back();
+    j.item0 = g.total;
+    g.total += j.items;
+    g.n += 1;
+    pp.sorted.push_back(j);
+  }
+  return pp;
+}
+
+bool same_job(const MixedGemmJob &x, const MixedGemmJob &y) {
+  return x.P == y.P && x.npt == y.npt && x.K == y.K && x.Ncol == y.Ncol && x.A == y.A && x.lda == y.lda &&
+         x.rowmax == y.rowmax && x.Bs == y.Bs && x.ldb == y.ldb && x.colinv == y.colinv && x.flags == y.flags &&
+         x.nkb == y.nkb && x.perm == y.perm && x.C == y.C && x.ldc == y.ldc && x.plan.col_mo == y.plan.col_mo &&
+         x.plan.blk == y.plan.blk && x.plan.nw == y.plan.nw && x.plan.nblk == y.plan.nblk && x.cbw == y.cbw &&
+         x.item0 == y.item0 && x.gx == y.gx && x.items == y.items;
+}
+
+// Enqueues the plan's launches on c0's stream after every collecting
+// context's work; table holds the sorted jobs followed by one int counter per
+// group (already on the device).
+hipError_t launch_persistent(Context *c0, const PersistentPlan &pp, MixedGemmJob *table,
+                             const std::vector<Context *> &job_ctx) {
+  int *counters = reinterpret_cast<int *>(reinterpret_cast<char *>(table) + sizeof(MixedGemmJob) * pp.sorted.size());
+  hipError_t e = hipSuccess;
+  for (Context *c : job_ctx) {
+    if (e != hipSuccess || c == c0) continue;
+    if (c->batch_event == nullptr) e = hipEventCreateWithFlags(&c->batch_event, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(c->batch_event, c->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(c0->stream, c->batch_event, 0);
+  }
+  for (size_t g = 0; g < pp.groups.size() && e == hipSuccess; ++g) {
+    KernelTimer t(c0, "mo_gemm_batch_persistent");
+    e = launch_mo_gemm_mixed_persistent(c0->stream, pp.groups[g].form, pp.groups[g].cbw, table + pp.groups[g].first,
+                                        (int)pp.groups[g].n, pp.groups[g].total, counters + g);
+    t.stop();
+  }
+  return e;
+}
+
+// Eager batch: the table goes to c0's memory (its host copy stays in c0 for
+// a graph built from this batch), the launches complete before returning.
+hipError_t run_persistent_contractions(Context *c0, const std::vector<MixedGemmJob> &jobs,
+                                       const std::vector<Context *> &job_ctx) {
+  const PersistentPlan pp = plan_persistent(jobs, job_ctx);
+  if (!grow_bytes(c0, reinterpret_cast<void **>(&c0->batch_jobs), &c0->batch_jobs_cap,
+                  (int64_t)pp.table_bytes()))
+    return hipErrorOutOfMemory;
+  hipError_t e = hipMemcpyAsync(c0->batch_jobs, pp.sorted.data(), sizeof(MixedGemmJob) * pp.sorted.size(),
+                                hipMemcpyHostToDevice, c0->stream);
+  if (e == hipSuccess) e = launch_persistent(c0, pp, c0->batch_jobs, job_ctx);
+  // the job table is read by the launches: wait before it can be reused
+  if (e == hipSuccess) e = hipStreamSynchronize(c0->stream);
+  if (e == hipSuccess) c0->batch_jobs_host = pp.sorted;
+  return e;
+}
+
+}  // namespace
+
+extern "C" {
+
 // Batching form: each context's chain is enqueued on its own stream with the
 // chain's synchronisations deferred, then every stream is waited for once.
+// The contractions of split-form contexts (tile width <= 3) run as one
+// persistent launch per form and width over all of them.
 qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_batch_device(int64_t count,
                                                               const qmckl_context_device *contexts,
                                                               double *const *mo_vgl,
@@ -585,17 +688,28 @@ qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_batch_device(int64_t count,
   }
   qmckl_exit_code_device first = QMCKL_SUCCESS_DEVICE;
   int64_t enqueued = 0;
+  std::vector<MixedGemmJob> jobs;
+  std::vector<Context *> job_ctx;
+  c0->batch_jobs_host.clear();
   for (; enqueued < count; ++enqueued) {
     Context *c = ctx_of(contexts[enqueued]);
     c->defer_sync = true;
+    const size_t before = jobs.size();
+    c->collect_jobs = &jobs;
     qmckl_exit_code_device rc = qmckl_context_touch_device(contexts[enqueued]);
     if (rc == QMCKL_SUCCESS_DEVICE) rc = compute_mo_vgl(contexts[enqueued], c, mo_vgl[enqueued], fn);
+    c->collect_jobs = nullptr;
     c->defer_sync = false;
+    if (jobs.size() > before) job_ctx.push_back(c);
     if (rc != QMCKL_SUCCESS_DEVICE) {
       first = rc;
       ++enqueued;
       break;
     }
+  }
+  if (first == QMCKL_SUCCESS_DEVICE && !jobs.empty()) {
+    const hipError_t e = run_persistent_contractions(c0, jobs, job_ctx);
+    if (e != hipSuccess) first = hip_check(contexts[0], e, fn);
   }
   for (int64_t i = 0; i < enqueued; ++i) {
     Context *c = ctx_of(contexts[i]);
@@ -763,6 +877,7 @@ struct BatchGraph {
   hipGraphExec_t exec = nullptr;
   std::vector<qmckl_context_device> contexts;
   std::vector<std::vector<uint64_t>> signature;
+  MixedGemmJob *table = nullptr;  // the persistent launches' job table and counters (owned)
 };
 
 std::mutex g_graph_mutex;
@@ -808,6 +923,15 @@ qmckl_exit_code_device qmckl_mo_basis_mo_vgl_batch_graph_create_device(int64_t c
   g->contexts.assign(contexts, contexts + count);
   hipError_t e = hipSetDevice(g->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+  // the eager run's persistent job table, copied to memory the graph owns
+  // (a later batch may reuse the context's table)
+  const std::vector<MixedGemmJob> eager_jobs = c0->batch_jobs_host;
+  if (e == hipSuccess && !eager_jobs.empty()) {
+    const size_t bytes = sizeof(MixedGemmJob) * eager_jobs.size() + 64 * sizeof(int);
+    e = hipMalloc(reinterpret_cast<void **>(&g->table), bytes);
+    if (e == hipSuccess)
+      e = hipMemcpy(g->table, eager_jobs.data(), sizeof(MixedGemmJob) * eager_jobs.size(), hipMemcpyHostToDevice);
+  }
   hipEvent_t fork = nullptr;
   std::vector<hipEvent_t> joins((size_t)count, nullptr);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
@@ -819,19 +943,40 @@ qmckl_exit_code_device qmckl_mo_basis_mo_vgl_batch_graph_create_device(int64_t c
   }
   if (e == hipSuccess) e = hipEventRecord(fork, g->stream);
   // every context's stream joins the capture, enqueues its chain, and is
-  // joined back
+  // joined back; split-form contractions are collected as in the eager run
+  // and recorded as its persistent launches on the first context's stream
+  std::vector<MixedGemmJob> jobs;
+  std::vector<Context *> job_ctx;
   for (int64_t i = 0; i < count && e == hipSuccess && rc == QMCKL_SUCCESS_DEVICE; ++i) {
     Context *c = ctx_of(contexts[i]);
     e = hipStreamWaitEvent(c->stream, fork, 0);
     if (e != hipSuccess) break;
     c->defer_sync = true;
     c->capturing = true;
+    const size_t before = jobs.size();
+    c->collect_jobs = &jobs;
     rc = qmckl_context_touch_device(contexts[i]);
     if (rc == QMCKL_SUCCESS_DEVICE) rc = compute_mo_vgl(contexts[i], c, mo_vgl[i], fn);
+    c->collect_jobs = nullptr;
     c->defer_sync = false;
     c->capturing = false;
+    if (jobs.size() > before) job_ctx.push_back(c);
     if (rc == QMCKL_SUCCESS_DEVICE) e = hipEventRecord(joins[(size_t)i], c->stream);
     if (e == hipSuccess) e = hipStreamWaitEvent(g->stream, joins[(size_t)i], 0);
+  }
+  hipEvent_t pjoin = nullptr;
+  if (e == hipSuccess && rc == QMCKL_SUCCESS_DEVICE && !jobs.empty()) {
+    const PersistentPlan pp = plan_persistent(jobs, job_ctx);
+    // the capture must record exactly the table the eager run uploaded
+    bool same = g->table != nullptr && pp.sorted.size() == eager_jobs.size() && pp.groups.size() <= 64;
+    for (size_t k = 0; same && k < pp.sorted.size(); ++k) same = same_job(pp.sorted[k], eager_jobs[k]);
+    if (!same) e = hipErrorInvalidValue;
+    c0->capturing = true;  // no kernel timers inside the capture
+    if (e == hipSuccess) e = launch_persistent(c0, pp, g->table, job_ctx);
+    c0->capturing = false;
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&pjoin, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(pjoin, c0->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(g->stream, pjoin, 0);
   }
   if (capturing) {
     const hipError_t e2 = hipStreamEndCapture(g->stream, &g->graph);
@@ -839,6 +984,7 @@ qmckl_exit_code_device qmckl_mo_basis_mo_vgl_batch_graph_create_device(int64_t c
   }
   if (e == hipSuccess && rc == QMCKL_SUCCESS_DEVICE) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
   if (fork) (void)hipEventDestroy(fork);
+  if (pjoin) (void)hipEventDestroy(pjoin);
   for (hipEvent_t j : joins)
     if (j) (void)hipEventDestroy(j);
   if (e == hipSuccess && rc == QMCKL_SUCCESS_DEVICE) {
@@ -851,6 +997,7 @@ qmckl_exit_code_device qmckl_mo_basis_mo_vgl_batch_graph_create_device(int64_t c
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
   if (g->stream) (void)hipStreamDestroy(g->stream);
+  if (g->table) (void)hipFree(g->table);
   delete g;
   (void)hipGetLastError();
   if (rc != QMCKL_SUCCESS_DEVICE) return rc;
@@ -895,6 +1042,7 @@ qmckl_exit_code_device qmckl_mo_basis_mo_vgl_batch_graph_destroy_device(int64_t 
   (void)hipGraphExecDestroy(g->exec);
   (void)hipGraphDestroy(g->graph);
   (void)hipStreamDestroy(g->stream);
+  if (g->table) (void)hipFree(g->table);
   delete g;
   return QMCKL_SUCCESS_DEVICE;
 }

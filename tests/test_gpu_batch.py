@@ -60,8 +60,10 @@ def test_batch_equals_single_fp64(molecules, oracle_lib):
 
 
 def test_batch_mixed_forms_and_moving_points(molecules):
-    mols = molecules[:24]
-    ctxs, outs = setup(mols, forms=[0, 2, 3, 1])
+    # the split forms' contractions run as persistent launches over the
+    # batch; every molecule must still equal its own single-context result
+    mols = molecules[:25]
+    ctxs, outs = setup(mols, forms=[0, 2, 3, 1, 4])
     try:
         rng = np.random.default_rng(5)
         for step in range(2):

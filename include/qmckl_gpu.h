@@ -391,6 +391,10 @@ qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_adjoint_device(qmckl_context_de
  * precision form, written to mo_vgl[i] [P_i][5][M_i]), but the work of every
  * context is enqueued on that context's own stream before any of them is
  * waited for, so molecules of different sizes run concurrently on the GPU.
+ * The MO contractions of contexts in a split precision form (up to 192 MOs)
+ * run as persistent launches over all of them (an atomic counter hands out
+ * (molecule, row tile, MO tile) items), one per form and tile width, on
+ * contexts[0]'s stream after every context's AO work.
  * Returns when every output is complete.  contexts, mo_vgl and size_max are
  * host arrays of count entries (mo_vgl[i] a device pointer).  Errors: count
  * <= 0 -> ARG_1; a NULL array -> ARG_2/3/4; an invalid context ->

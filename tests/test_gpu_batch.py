@@ -151,3 +151,41 @@ def lib_rc_launch(handle):
     from qmckl_sycl_amd.context import lib
     return lib().qmckl_mo_basis_mo_vgl_batch_graph_launch_device(handle)
 
+
+
+def test_batch_persistent_and_per_context_contractions_mixed():
+    """One batch mixing contexts whose split-form contraction goes into the
+    persistent launches (up to 192 MOs) with one that keeps its own launch
+    (300 MOs, tile width 5), a three-term form and an FP64 context: every
+    molecule equals its single-context result bitwise, eagerly and on graph
+    replays after the points move."""
+    mols = [S.water_cluster(n_side=2, walk_num=2, mo_num=300, name="water8_300mo"),
+            S.water_cluster(n_side=2, walk_num=2, mo_num=100, seed=7, name="water8_100mo"),
+            S.n2_like(walk_num=3, seed=11),
+            S.water_cluster(n_side=2, walk_num=1, mo_num=40, seed=9, name="water8_40mo")]
+    forms = [2, 4, 2, 0]
+    ctxs, outs = [], []
+    for m, f in zip(mols, forms):
+        c = Context(0)
+        c.set_system(m)
+        c.call("qmckl_set_mo_basis_precision_device", f)
+        ctxs.append(c)
+        outs.append(DeviceArray(c, (m.point_num, 5, m.mo_num)))
+    g = None
+    try:
+        batch_mo_vgl(ctxs, outs)
+        for c, m, o in zip(ctxs, mols, outs):
+            assert np.array_equal(o.numpy(), single(c, m)), m.name
+        g = BatchGraph(ctxs, outs)
+        rng = np.random.default_rng(3)
+        for rep in range(2):
+            _move(ctxs, mols, rng)
+            assert g.launch() == 0
+            for c, m, o in zip(ctxs, mols, outs):
+                assert np.array_equal(o.numpy(), single(c, m)), (rep, m.name)
+        say("persistent + per-context batch: eager and 2 graph replays equal to single calls")
+    finally:
+        if g is not None:
+            g.destroy()
+        for c in ctxs:
+            c.destroy()

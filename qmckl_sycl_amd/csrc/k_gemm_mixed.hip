@@ -130,10 +130,12 @@ __device__ inline f4 mfma(const uint4 &a, const uint4 &b, f4 c) {
 // SIMD (the FP64 accumulators then fit 256 registers) from 4.3 to 3.7 ms;
 // WR = 32 (two independent 4-wavefront workgroups per CU) equals WR = 64
 // for FP16X2 and is 2.5% faster for BF16X3.
-// With the points ordered into compact tiles the kernel runs 3.2 ms, and
-// 1.9 ms with its global loads taken out (a timing variant): 32 cycles per
-// 16x16x32 MFMA per SIMD, so the loop is MFMA-bound plus what the two
-// wavefronts per SIMD cannot hide of the operand loads.
+// With the points ordered into compact tiles the kernel ran 3.2 ms, and
+// 1.9 ms with its global loads taken out (a timing variant): 32 cycles of
+// wall time per 16x16x32 MFMA per SIMD, twice the 16 cycles each keeps the
+// matrix pipe busy (SQ_VALU_MFMA_BUSY_CYCLES), so the loop leaves the pipe
+// idle half the time (FP64 accumulation, A split, LDS reads) and the
+// operand loads add what the two wavefronts per SIMD cannot hide.
 // Measured slower: 64 x 96 tiles of v_mfma_f32_32x32x16 per wavefront at
 // one per SIMD (MOs padded to 384; 4.0 ms, 2.7 ms without loads: 75 cycles
 // per 32x32x16), 32 x 96 tiles of it at two per SIMD (3.9 ms), software-pipelined single-barrier loops (with and without

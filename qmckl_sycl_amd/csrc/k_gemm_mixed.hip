@@ -347,7 +347,8 @@ __device__ __forceinline__ void split_tile(int64_t bx, int64_t by, int64_t P, in
       // product by product over all RB x CBW tiles: RB*CBW independent
       // accumulators between dependent MFMAs.  Dead tiles run on the zeros
       // their loads returned (branching around their MFMAs measured slower:
-      // 3.28 against 2.97 ms on water64).
+      // 3.28 against 2.97 ms on water64, and around the FP64 additions of
+      // stage-dead tiles below 3.10 ms).
       constexpr int NP = T == 3 ? 6 : 3;
       constexpr int pa[6] = {0, 0, 1, 0, 2, 1}, pb[6] = {0, 1, 0, 2, 0, 1};  // (A term, C term)
 #pragma unroll
@@ -768,6 +769,9 @@ __global__ void zero_int_kernel(int *p) {
 hipError_t launch_mo_gemm_mixed_persistent(hipStream_t s, int form, int cbw, const MixedGemmJob *jobs, int njobs,
                                            int64_t total, int *counter) {
   if (njobs <= 0 || total <= 0) return hipSuccess;
+  // tile widths up to 192 MOs: wider persistent tiles need more registers
+  // than two workgroups per CU leave (the 320-wide one spills to scratch)
+  if (cbw < 1 || cbw > 3) return hipErrorInvalidValue;
   if (total >= 0x7fffffff - 65536) return hipErrorInvalidValue;  // the counter is an int
   int dev = 0, cus = 0;
   hipError_t e = hipGetDevice(&dev);
@@ -782,13 +786,9 @@ hipError_t launch_mo_gemm_mixed_persistent(hipStream_t s, int form, int cbw, con
     case QMCKL_AMD_SPLIT_FP16X2 * 8 + 1: QMCKL_PERSISTENT(QMCKL_AMD_SPLIT_FP16X2, 1); break;
     case QMCKL_AMD_SPLIT_FP16X2 * 8 + 2: QMCKL_PERSISTENT(QMCKL_AMD_SPLIT_FP16X2, 2); break;
     case QMCKL_AMD_SPLIT_FP16X2 * 8 + 3: QMCKL_PERSISTENT(QMCKL_AMD_SPLIT_FP16X2, 3); break;
-    case QMCKL_AMD_SPLIT_FP16X2 * 8 + 4: QMCKL_PERSISTENT(QMCKL_AMD_SPLIT_FP16X2, 4); break;
-    case QMCKL_AMD_SPLIT_FP16X2 * 8 + 5: QMCKL_PERSISTENT(QMCKL_AMD_SPLIT_FP16X2, 5); break;
     case QMCKL_AMD_SPLIT_BF16X2 * 8 + 1: QMCKL_PERSISTENT(QMCKL_AMD_SPLIT_BF16X2, 1); break;
     case QMCKL_AMD_SPLIT_BF16X2 * 8 + 2: QMCKL_PERSISTENT(QMCKL_AMD_SPLIT_BF16X2, 2); break;
     case QMCKL_AMD_SPLIT_BF16X2 * 8 + 3: QMCKL_PERSISTENT(QMCKL_AMD_SPLIT_BF16X2, 3); break;
-    case QMCKL_AMD_SPLIT_BF16X2 * 8 + 4: QMCKL_PERSISTENT(QMCKL_AMD_SPLIT_BF16X2, 4); break;
-    case QMCKL_AMD_SPLIT_BF16X2 * 8 + 5: QMCKL_PERSISTENT(QMCKL_AMD_SPLIT_BF16X2, 5); break;
     case QMCKL_AMD_SPLIT_BF16X3 * 8 + 1: QMCKL_PERSISTENT(QMCKL_AMD_SPLIT_BF16X3, 1); break;
     case QMCKL_AMD_SPLIT_BF16X3 * 8 + 2: QMCKL_PERSISTENT(QMCKL_AMD_SPLIT_BF16X3, 2); break;
     case QMCKL_AMD_SPLIT_BF16X3 * 8 + 3: QMCKL_PERSISTENT(QMCKL_AMD_SPLIT_BF16X3, 3); break;

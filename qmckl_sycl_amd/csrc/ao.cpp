@@ -208,6 +208,23 @@ std::string build_basis_tables(Context *c, DeviceBasis &b, int64_t N, int64_t S,
 
 void release_basis_tables(Context *c, DeviceBasis &b) { free_tables(c, b); }
 
+hipError_t tile_point_order(Context *c, const int32_t **perm) {
+  *perm = nullptr;
+  const int64_t P = c->point.num, ld = round_up(c->ao.ao_num, 64);
+  const int32_t cfg = c->mo.tile_point_order;
+  if (!(cfg >= 0 ? cfg == 1 : ld >= 1024 && P >= 4096)) return hipSuccess;
+  AoBasis &ao = c->ao;
+  if (!grow_bytes(c, reinterpret_cast<void **>(&ao.point_perm), &ao.point_perm_cap, (int64_t)sizeof(int32_t) * P) ||
+      !grow_bytes(c, &ao.order_ws, &ao.order_ws_cap, point_order_workspace_bytes(P)))
+    return hipErrorOutOfMemory;
+  KernelTimer t(c, "point_order");
+  const hipError_t e =
+      launch_point_order(c->stream, P, c->point.coord, c->nucleus.num, c->nucleus.coord, ao.order_ws, ao.point_perm);
+  t.stop();
+  if (e == hipSuccess) *perm = ao.point_perm;
+  return e;
+}
+
 }  // namespace qmckl_amd
 
 extern "C" {
@@ -564,15 +581,18 @@ qmckl_exit_code_device qmckl_provide_ao_basis_shell_vgl_device(qmckl_context_dev
 }
 
 // Compute ao_vgl (ncomp = 5) or ao_value (ncomp = 1) for the current points
-// into `dst`.
+// into `dst`.  ao_vgl is evaluated over spatially compact point tiles where
+// tile_point_order says so (each point's rows are the same either way).
 static qmckl_exit_code_device compute_ao(qmckl_context_device context, Context *c, double *dst,
                                          int ncomp, const char *fn, uint8_t *flags = nullptr,
                                          int nkb = 0) {
-  hipError_t e;
-  {
+  const int32_t *perm = nullptr;
+  hipError_t e = ncomp == 5 ? tile_point_order(c, &perm) : hipSuccess;
+  if (e == hipErrorOutOfMemory) return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
+  if (e == hipSuccess) {
     KernelTimer t(c, ncomp == 5 ? "ao_vgl" : "ao_value");
     e = launch_ao_vgl(c->stream, c->ao.dev, c->point.num, c->point.coord, c->nucleus.coord,
-                      c->ao.ao_factor, dst, ncomp, flags, nkb);
+                      c->ao.ao_factor, dst, ncomp, flags, nkb, perm);
     t.stop();
   }
   if (e == hipSuccess) e = sync(c);

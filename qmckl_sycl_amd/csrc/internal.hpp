@@ -355,9 +355,24 @@ hipError_t launch_en_distance(hipStream_t s, int64_t P, int64_t N, const double 
                               const double *nucl, double *out);
 hipError_t launch_shell_vgl(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
                             const double *nucl, double *shell_vgl);
+// ao_vgl (ncomp 5) or ao_value (ncomp 1); perm (ncomp 5 only, or null) is
+// an order of the points into spatially compact 64-point tiles
 hipError_t launch_ao_vgl(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
                          const double *nucl, const double *ao_factor, double *ao_vgl,
-                         int ncomp, uint8_t *flags = nullptr, int nkb = 0);
+                         int ncomp, uint8_t *flags = nullptr, int nkb = 0, const int32_t *perm = nullptr);
+// FP64 ao_vgl [P][5][A] with one lane per point (k_ao_rows.hip), the
+// points of lane-tile t being perm[64 t ..] (null: 64 t ..), and, if flags
+// is not null, nonzero flags [(5P + 63) / 64][nkb], nkb = ceil(A / 32), per
+// (64 rows of [point * 5 + component], 32 AOs).
+hipError_t launch_ao_vgl_f64(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
+                             const int32_t *perm, const double *nucl, const double *ao_factor, double *out,
+                             uint8_t *flags, int nkb);
+// The point order for spatially compact 64-point tiles of the lane-per-point
+// AO kernels: *perm = ao.point_perm, computed here on the context's stream,
+// or null for the caller's order.  The tile configuration's point_order
+// forces it on (1) or off (0); by default (-1) it is on for 1024 AOs or more
+// and 4096 points or more.  hipErrorOutOfMemory if its buffers cannot grow.
+hipError_t tile_point_order(Context *c, const int32_t **perm);
 // AO operand of the mixed contraction, FP32 in the blocked layout
 // buf[c][64-point tile][AO/4][point in tile][4] (AOs zero-padded to ld, a
 // multiple of 64), tile row i holding point perm[i] (null: point i), row

@@ -105,8 +105,8 @@ struct RowWriter {
 
 // Evaluates the l = L Cartesian AOs of one shell for this lane's point and
 // pushes them in the reference order (a descending, then b descending).
-template <int L>
-__device__ __attribute__((always_inline)) inline void shell_aos(RowWriter &w,
+template <int L, typename W>
+__device__ __attribute__((always_inline)) inline void shell_aos(W &w,
                                                                 const double *__restrict__ ao_factor,
                                                                 double x, double y, double z,
                                                                 const double (&R)[5]) {
@@ -167,7 +167,8 @@ __device__ inline void pow3(double x, int e, double &f0, double &f1, double &f2)
 // Shells of l > 3 (rare): the same AOs with the exponents as (wavefront-
 // uniform) loop counts instead of unrolled code, which keeps the register
 // footprint of the kernel at that of l <= 3.
-__device__ __attribute__((always_inline)) inline void shell_aos_any(RowWriter &w, const double *__restrict__ ao_factor,
+template <typename W>
+__device__ __attribute__((always_inline)) inline void shell_aos_any(W &w, const double *__restrict__ ao_factor,
                                                         double x, double y, double z, const double (&R)[5],
                                                         int l) {
   const int k0 = w.k;
@@ -191,6 +192,74 @@ __device__ __attribute__((always_inline)) inline void shell_aos_any(RowWriter &w
       v[3] = (gz * R[0] + m * R[3]) * f;
       v[4] = (lap * R[0] + m * R[4] + 2.0 * (gx * R[1] + gy * R[2] + gz * R[3])) * f;
       w.push(v);
+    }
+  }
+}
+
+// Is any AO of the shells from s_first that start below k_end nonzero for
+// some point of the wavefront?  Same screen as eval_stage: a shell is zero
+// at r^2 when a * r^2 > cutoff for all of its exponents a (a_min * r^2
+// rounds no higher than a * r^2), or past the nucleus range.
+__device__ __attribute__((always_inline)) inline bool stage_live(int64_t s_first, int64_t S, int k_end, int N,
+                                                                 bool inside, double px, double py, double pz,
+                                                                 const double *__restrict__ nucl,
+                                                                 const ShellDesc *__restrict__ shells,
+                                                                 const double *__restrict__ nucl_range) {
+  bool any = false;
+  for (int64_t s = s_first; s < S; ++s) {
+    const ShellDesc d = shells[s];
+    if (d.ao_start >= k_end) break;
+    const double x = px - nucl[d.nucl];
+    const double y = py - nucl[N + d.nucl];
+    const double z = pz - nucl[2 * N + d.nucl];
+    const double r2 = fma(x, x, fma(y, y, z * z));
+    any |= inside && !(r2 > kCutoff * nucl_range[d.nucl]) && !(d.a_min * r2 > kCutoff);
+  }
+  return __ballot(any) != 0;
+}
+
+// Walks the shells from s_first that start below k_end in AO order for
+// this lane's point and pushes their AOs into w (the writer keeps those of
+// its current stage).
+template <typename W>
+__device__ __attribute__((always_inline)) inline void eval_stage(W &w, int64_t s_first, int64_t S, int k_end,
+                                                                 int N, bool inside, double px, double py,
+                                                                 double pz, const double *__restrict__ nucl,
+                                                                 const ShellDesc *__restrict__ shells,
+                                                                 const double *__restrict__ ao_factor,
+                                                                 const double *__restrict__ expo,
+                                                                 const double *__restrict__ coef,
+                                                                 const double *__restrict__ nucl_range) {
+  for (int64_t s = s_first; s < S; ++s) {
+    const ShellDesc d = shells[s];
+    if (d.ao_start >= k_end) break;
+    const double x = px - nucl[d.nucl];
+    const double y = py - nucl[N + d.nucl];
+    const double z = pz - nucl[2 * N + d.nucl];
+    const double r2 = fma(x, x, fma(y, y, z * z));
+    const bool live = inside && !(r2 > kCutoff * nucl_range[d.nucl]);
+    double R[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const double *ex = expo + d.prim_start;
+    const double *cf = coef + d.prim_start;
+    for (int j = 0; j < d.prim_num; ++j) {
+      const double a = ex[j];
+      const double ar2 = a * r2;
+      if (live && !(ar2 > kCutoff)) {
+        const double v = cf[j] * exp(-ar2);
+        const double two_a = -2.0 * a * v;
+        R[0] += v;
+        R[1] += two_a * x;
+        R[2] += two_a * y;
+        R[3] += two_a * z;
+        R[4] += two_a * (3.0 - 2.0 * ar2);
+      }
+    }
+    switch (d.l) {
+      case 0: shell_aos<0>(w, ao_factor, x, y, z, R); break;
+      case 1: shell_aos<1>(w, ao_factor, x, y, z, R); break;
+      case 2: shell_aos<2>(w, ao_factor, x, y, z, R); break;
+      case 3: shell_aos<3>(w, ao_factor, x, y, z, R); break;
+      default: shell_aos_any(w, ao_factor, x, y, z, R, d.l); break;
     }
   }
 }
@@ -231,15 +300,15 @@ ao_rows_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord, co
   w.ring = s_ring[wave];
 
   // Stage by stage; a shell that straddles a stage boundary is evaluated
-  // for each stage it reaches (each writes its own AOs).
+  // for each stage it reaches (each writes its own AOs).  The screen and the
+  // shell walk are the same as stage_live and eval_stage, written out here:
+  // through those functions the kernel takes 105 VGPRs instead of 94, four
+  // wavefronts per SIMD instead of five, and 1.32 instead of 1.06 ms on
+  // water64 at 128 walkers.
   for (int64_t st = st_lo; st < st_hi; ++st) {
     const int k_end = (int)(64 * (st + 1));
     const int64_t s_first = st < nst_a ? stage_shell[st] : S;
     if (flags != nullptr) {
-      // Is any AO of the stage nonzero for some point of the tile?  Same
-      // screen as the evaluation below: a shell is zero at r^2 when
-      // a * r^2 > cutoff for all of its exponents a (a_min * r^2 rounds no
-      // higher than a * r^2), or past the nucleus range.
       bool any = false;
       for (int64_t s = s_first; s < S; ++s) {
         const ShellDesc d = shells[s];
@@ -250,7 +319,7 @@ ao_rows_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord, co
         const double r2 = fma(x, x, fma(y, y, z * z));
         any |= inside && !(r2 > kCutoff * nucl_range[d.nucl]) && !(d.a_min * r2 > kCutoff);
       }
-      if (__ballot(any) == 0) {  // all zero: not written, flagged empty
+      if (__ballot(any) == 0) {
         if (lane < 10) flags[pt * nkb + (lane >> 1) * w.fstride + 2 * st + (lane & 1)] = 0;
         continue;
       }
@@ -310,7 +379,204 @@ __global__ void __launch_bounds__(kThreads) fill_zero_kernel(int64_t n, uint32_t
     p[i] = 0u;
 }
 
+// ---------------------------------------------------------------------------
+// FP64 ao_vgl in the caller's layout ao_vgl[point][component][AO], with the
+// same lane-per-point evaluation.  A lane's five rows are 5 A doubles apart
+// and its own stores would touch one cache line per lane, so the values go
+// through a ring in LDS, [component][AO mod kRing64][point], and every
+// kRing64-th AO the wavefront writes the ring out with lanes re-assigned:
+// lane l moves AO slot l % kRing64 of point l / kRing64 (+ 64 / kRing64 per
+// instruction), so each store instruction writes 64/kRing64 rows of
+// kRing64 * 8 contiguous bytes, as nontemporal stores (the array is far
+// larger than the caches).  The wavefront's 64 points are perm[64 t ..]
+// (spatially compact tiles, the order of the mixed path) or 64 consecutive
+// points; a stage that the screen zeroes for all 64 points is written as
+// zeros without evaluation.  The nonzero flags per (64 rows of the
+// [point * 5 + component] row space, 32-AO block) that the FP64
+// contraction reads are zeroed first; each lane then marks the (at most
+// two) row tiles of its point that hold a nonzero entry of a block.
+// Measured on water64 at 128 walkers (81,920 points x 4,160 AOs, 13.6 GB
+// written): 4.00 ms, against 5.26 ms for the lane-per-AO kernel that wrote
+// this array before.  The same launch without its stores takes 1.93 ms; the
+// stores add the rest because the 20 KB ring leaves room for eight
+// wavefronts per CU, too few to hide them.  Runs of 4 AOs took 10.4 ms,
+// runs of 16 AOs (40 KB ring, four wavefronts per CU) 6.2 ms; holding one
+// 8-AO flush in registers to write whole 128-byte lines spilled to scratch;
+// plain stores instead of nontemporal ones 4.30 ms; the caller's point order
+// instead of compact tiles 4.40 ms.
+#ifndef QMCKL_AO_F64_RING
+#define QMCKL_AO_F64_RING 8
+#endif
+#ifndef QMCKL_AO_F64_GROUP_WAVES
+#define QMCKL_AO_F64_GROUP_WAVES 64
+#endif
+constexpr int kRing64 = QMCKL_AO_F64_RING;  // AOs per ring flush
+constexpr int kAoF64GroupWaves = QMCKL_AO_F64_GROUP_WAVES;  // wavefronts per CU a launch aims at
+// ring row: 64 points, the point index XOR-swizzled by 8 * (slot mod 8) so
+// that the flush reads of one instruction (8 slots of 8 points) fall on
+// distinct bank pairs
+constexpr int kRingLd = 64;
+__device__ __attribute__((always_inline)) inline int ring_col(int slot, int q) { return q ^ (8 * (slot & 7)); }
+static_assert(64 % kRing64 == 0 && kRing64 >= 2 && kRing64 <= 16, "ring length");
+
+struct F64Writer {
+  uint32_t nz;       // bit t: an entry of this lane's rows in row tile t0 + t is nonzero (current 32-AO block)
+  int64_t t0;        // row tile of this lane's first row (point * 5)
+  int cs;            // first component whose row is in tile t0 + 1 (5: none)
+  int k;             // next AO index (wavefront-uniform)
+  int k_lo, k_hi;    // AOs of the current stage
+  double *ring;      // this wavefront's [5][kRing64][kRingLd] LDS ring
+  int p;             // this lane's point (-1 past P)
+  int pq[kRing64];   // the points whose slots this lane moves in a flush
+  double *out;
+  int64_t A;
+  uint8_t *flags;    // zeroed [tiles][nkb], or null
+  int nkb, lane;
+
+  // the ring's first n slots hold AOs kb .. kb + n - 1
+  // the ring's first n slots hold AOs kb .. kb + n - 1
+  __device__ __attribute__((always_inline)) void flush(int kb, int n) {
+    __builtin_amdgcn_wave_barrier();
+    const int s = lane % kRing64;
+    double v[kRing64][5];
+#pragma unroll
+    for (int j = 0; j < kRing64; ++j) {
+      const int q = j * (64 / kRing64) + lane / kRing64;
+#pragma unroll
+      for (int c = 0; c < 5; ++c) v[j][c] = ring[(c * kRing64 + s) * kRingLd + ring_col(s, q)];
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int j = 0; j < kRing64; ++j) {
+      if (pq[j] < 0) continue;
+      double *row = out + (int64_t)pq[j] * 5 * A + s;
+#pragma unroll
+      for (int c = 0; c < 5; ++c) {
+        if (s < n) __builtin_nontemporal_store(v[j][c], row + c * A + kb);
+      }
+    }
+  }
+  __device__ __attribute__((always_inline)) void flag_block(int b) {
+    if (flags != nullptr) {
+      if (nz & 1u) flags[t0 * nkb + b] = 1;
+      if (nz & 2u) flags[(t0 + 1) * nkb + b] = 1;
+    }
+    nz = 0;
+  }
+  __device__ __attribute__((always_inline)) void push(const double (&v)[5]) {
+    if (k < k_lo || k >= k_hi) {  // part of a shell outside the current stage
+      ++k;
+      return;
+    }
+    const int slot = k % kRing64;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      ring[(c * kRing64 + slot) * kRingLd + ring_col(slot, lane)] = v[c];
+      nz |= (v[c] != 0.0 ? 1u : 0u) << (c >= cs ? 1 : 0);
+    }
+    if (slot == kRing64 - 1) flush(k - (kRing64 - 1), kRing64);
+    if ((k & 31) == 31) flag_block(k >> 5);
+    ++k;
+  }
+  // end of a stage whose last AO is k_end - 1: the part-filled ring and
+  // 32-block at the end of the AO range
+  __device__ __attribute__((always_inline)) void finish(int k_end) {
+    if (k_end % kRing64 != 0) flush(k_end - k_end % kRing64, k_end % kRing64);
+    if (k_end % 32 != 0) flag_block(k_end >> 5);
+  }
+  // a stage screened out for every point: zeros (its flags stay zero)
+  __device__ __attribute__((always_inline)) void zeros(int k_end) {
+    const int n = k_end - k_lo;
+    for (int q = 0; q < 64; ++q) {
+      const int pu = __builtin_amdgcn_readlane(p, q);
+      if (pu < 0) break;  // the points past P are the last ones
+      double *row = out + (int64_t)pu * 5 * A + k_lo + lane;
+#pragma unroll
+      for (int c = 0; c < 5; ++c)
+        if (lane < n) __builtin_nontemporal_store(0.0, row + c * A);
+    }
+  }
+};
+
+__global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves_per_eu(2)))
+ao_vgl_f64_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord, const int32_t *__restrict__ perm,
+                  const double *__restrict__ nucl, const ShellDesc *__restrict__ shells,
+                  const int32_t *__restrict__ stage_shell, const double *__restrict__ ao_factor,
+                  const double *__restrict__ expo, const double *__restrict__ coef,
+                  const double *__restrict__ nucl_range, int64_t A, int groups, double *__restrict__ out,
+                  uint8_t *__restrict__ flags, int nkb) {
+  __shared__ double s_ring[5 * kRing64 * kRingLd];
+  const int lane = threadIdx.x;
+  const int64_t pt = blockIdx.x / groups;
+  const int g = (int)(blockIdx.x - pt * groups);
+  const int64_t i = pt * 64 + lane;
+  const bool inside = i < P;
+  const int64_t p = inside ? (perm != nullptr ? (int64_t)perm[i] : i) : 0;
+  const double px = inside ? coord[p] : 0.0;
+  const double py = inside ? coord[P + p] : 0.0;
+  const double pz = inside ? coord[2 * P + p] : 0.0;
+
+  // this wavefront's AOs: 64-AO stages [g*nst/groups, (g+1)*nst/groups)
+  const int64_t nst = (A + 63) / 64;
+  const int64_t st_lo = g * nst / groups, st_hi = (g + 1) * nst / groups;
+  F64Writer w;
+  w.nz = 0;
+  w.t0 = (5 * p) >> 6;
+  w.cs = inside && 64 - (int)((5 * p) & 63) < 5 ? 64 - (int)((5 * p) & 63) : 5;
+  w.ring = s_ring;
+  w.p = inside ? (int)p : -1;
+#pragma unroll
+  for (int j = 0; j < kRing64; ++j) w.pq[j] = __shfl(w.p, j * (64 / kRing64) + lane / kRing64);
+  w.out = out;
+  w.A = A;
+  w.flags = flags;
+  w.nkb = nkb;
+  w.lane = lane;
+
+  for (int64_t st = st_lo; st < st_hi; ++st) {
+    const int k_end = (int)(64 * (st + 1) < A ? 64 * (st + 1) : A);
+    const int64_t s_first = stage_shell[st];
+    w.k_lo = (int)(64 * st);
+    w.k_hi = k_end;
+    if (!stage_live(s_first, S, k_end, N, inside, px, py, pz, nucl, shells, nucl_range)) {
+      w.zeros(k_end);
+      continue;
+    }
+    w.k = shells[s_first].ao_start;
+    eval_stage(w, s_first, S, k_end, N, inside, px, py, pz, nucl, shells, ao_factor, expo, coef, nucl_range);
+    w.finish(k_end);
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) fill_zero_bytes_kernel(int64_t n, uint8_t *__restrict__ p) {
+  for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+    p[i] = 0;
+}
+
 }  // namespace
+
+hipError_t launch_ao_vgl_f64(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
+                             const int32_t *perm, const double *nucl, const double *ao_factor, double *out,
+                             uint8_t *flags, int nkb) {
+  if (P <= 0) return hipSuccess;
+  const int64_t A = b.ao_num;
+  if (b.max_l > kMaxL || A <= 0 || b.stage_shell == nullptr) return hipErrorInvalidValue;
+  if (flags != nullptr && (int64_t)nkb != (A + 31) / 32) return hipErrorInvalidValue;
+  const int64_t npt = (P + 63) / 64;
+  const int groups = ao_rows_groups(P, (A + 63) / 64 * 64, kAoF64GroupWaves);
+  const int64_t blocks = npt * groups;
+  if (blocks > 0x7fffffff || P > 0x7fffffff || A > 0x7fffffff) return hipErrorInvalidValue;
+  if (flags != nullptr) {
+    // a kernel, not a memset: the launch may be captured into a graph
+    const int64_t n = (5 * P + 63) / 64 * nkb;
+    hipLaunchKernelGGL(fill_zero_bytes_kernel, dim3((unsigned)std::min<int64_t>((n + kThreads - 1) / kThreads, 4096)),
+                       dim3(kThreads), 0, s, n, flags);
+  }
+  hipLaunchKernelGGL(ao_vgl_f64_kernel, dim3((unsigned)blocks), dim3(64), 0, s, P, (int)b.nucl_num, b.shell_num,
+                     coord, perm, nucl, b.shells, b.stage_shell, ao_factor, b.expo, b.coef_norm, b.nucl_range, A,
+                     groups, out, flags, nkb);
+  return hipGetLastError();
+}
 
 int ao_rows_groups(int64_t P, int64_t ld, int group_waves) {
   int dev = 0, cus = 0;

@@ -180,13 +180,7 @@ qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Contex
   // Tiles of spatially close points screen out far more 64-AO stages per
   // tile; worth its four small launches for large bases and many points
   // (water64 at 128 walkers: 94% -> 74% of the stages live).
-  const bool order = flags != nullptr && (mo.tile_point_order >= 0 ? mo.tile_point_order == 1
-                                                                    : ld >= 1024 && P >= 4096);
-  if (order && (!grow_bytes(c, reinterpret_cast<void **>(&ao.point_perm), &ao.point_perm_cap,
-                            (int64_t)sizeof(int32_t) * P) ||
-                !grow_bytes(c, &ao.order_ws, &ao.order_ws_cap, point_order_workspace_bytes(P))))
-    return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
-  const int32_t *perm = order ? ao.point_perm : nullptr;
+  const int32_t *perm = nullptr;
   // the split forms use the column plan of their workgroup width; the FP32
   // form keeps the MO order
   ColPlan plan;
@@ -202,11 +196,9 @@ qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Contex
     e = launch_split_coef(c->stream, form, M, A, mo.coefficient, ld, mo.coef_split, mo.coef_colinv, plan.col_mo);
     if (e == hipSuccess) mo.coef_split_form = form;
   }
-  if (e == hipSuccess && order) {
-    KernelTimer t(c, "point_order");
-    e = launch_point_order(c->stream, P, c->point.coord, c->nucleus.num, c->nucleus.coord, ao.order_ws,
-                           ao.point_perm);
-    t.stop();
+  if (e == hipSuccess && flags != nullptr) {
+    e = tile_point_order(c, &perm);
+    if (e == hipErrorOutOfMemory) return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
   }
   if (e == hipSuccess) {
     KernelTimer t(c, "ao_vgl_f32");

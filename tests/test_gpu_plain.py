@@ -144,6 +144,60 @@ def test_ragged_shapes(oracle_lib, P_walk, mo_num):
     assert np.max(np.abs(out["mo_value"] - out["mo"][:, 0, :])) <= 1e-12 * (1 + np.abs(mo).max())
 
 
+def test_high_l_and_screened_stages_vs_oracle(oracle_lib):
+    """Shells l = 0..6 on two nuclei 80 bohr apart (84 AOs each, 168 in all:
+    not a multiple of 32 or 64), 100 points (not a multiple of 64), the first
+    70 near nucleus 0.  The points of the first 64-point tile see nucleus 1
+    screened, so its AOs 128..167 (a whole, part-filled 64-AO stage) are
+    exact zeros there; l = 4..6 take the kernel's loop-count path; the l = 6
+    shell of each nucleus straddles a 64-AO stage boundary."""
+    rng = np.random.default_rng(77)
+    xyz = np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 80.0]])
+    bb = S._BasisBuilder()
+    for _ in range(2):
+        bb.add_nucleus([(l, int(rng.integers(1, 5))) for l in range(7)], rng, (0.1, 50.0))
+    arr = bb.arrays()
+    A = int(sum(S.ncart(int(l)) for l in arr["shell_ang_mom"]))
+    assert A == 168
+    elec = S._electrons_near(rng, xyz, [0] * 70 + [1] * 30, 1)
+    s = S.System(name="far_pair_l6", charge=np.array([1.0, 1.0]), nucl_coord=np.ascontiguousarray(xyz.T),
+                 mo_coef=rng.normal(0.0, 0.3, size=(40, A)), elec_up_num=50, elec_dn_num=50,
+                 elec_coord=elec, **arr)
+    s.validate()
+    with Context(0) as c:
+        c.set_system(s)
+        out = device_chain(c, s.point_num, want=("ao", "ao_value", "mo"))
+    ao, am = oracle_lib.ao_vgl(s, with_mag=True)
+    assert np.all(ao[:64, :, 128:] == 0.0), "the oracle screens nucleus 1 for the first 64 points"
+    assert np.all(out["ao"][:64, :, 128:] == 0.0)
+    assert_vgl_close(out["ao"], ao, am, what="far pair l<=6 ao_vgl")
+    assert np.max(np.abs(out["ao_value"] - out["ao"][:, 0, :])) <= 1e-12 * np.abs(ao).max()
+    mo, mm = oracle_lib.mo_vgl_from_ao(s.mo_coef, ao, am)
+    assert_vgl_close(out["mo"], mo, mm, what="far pair l<=6 mo_vgl")
+
+
+def test_fp64_point_order_keeps_rows(oracle_lib):
+    """FP64 ao_vgl over spatially ordered 64-point tiles (the default at this
+    size: 4,160 AOs, 5,120 points) against the caller's order: every point's
+    rows, and the MOs after them, are bitwise the same, and a sample of
+    points matches the oracle."""
+    s = S.water64(walk_num=8)
+    P = s.point_num
+    with Context(0) as c:
+        c.set_system(s)
+        c.call("qmckl_set_mo_basis_tile_config_device", 1, 0)
+        ao1, mo1 = c.ao_vgl(P), c.mo_vgl(P)
+        c.call("qmckl_set_mo_basis_tile_config_device", 0, 0)
+        c.touch()
+        ao0, mo0 = c.ao_vgl(P), c.mo_vgl(P)
+    assert np.array_equal(ao1, ao0)
+    assert np.array_equal(mo1, mo0)
+    idx = np.sort(np.random.default_rng(3).choice(P, size=64, replace=False))
+    coord = np.ascontiguousarray(s.coord_T()[:, idx])
+    ao, am = oracle_lib.ao_vgl(s, coord, with_mag=True)
+    assert_vgl_close(ao1[idx], ao, am, what="water64 ordered-tile ao_vgl")
+
+
 def test_walker_count_changes_reallocate(oracle_lib):
     s = S.n2_like(walk_num=2, seed=5)
     with Context(0) as c:

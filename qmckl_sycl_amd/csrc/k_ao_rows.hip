@@ -395,9 +395,14 @@ __global__ void __launch_bounds__(kThreads) fill_zero_kernel(int64_t n, uint32_t
 // [point * 5 + component] row space, 32-AO block) that the FP64
 // contraction reads are zeroed first; each lane then marks the (at most
 // two) row tiles of its point that hold a nonzero entry of a block.
+// The host uses it for bases of 1024 AOs or more whose rows are whole 64-byte
+// runs (A % 8 == 0); the lane-per-AO kernel of k_ao.hip, which stores 512-byte
+// runs of a row per instruction, writes the others.  On unaligned rows each
+// 8-AO run touches two partial lines: on the 48-molecule ragged batch
+// (30-620 AOs) this kernel took 22.3 ms of AO time, 15.4 with plain stores,
+// the lane-per-AO kernel 10.0.
 // Measured on water64 at 128 walkers (81,920 points x 4,160 AOs, 13.6 GB
-// written): 4.00 ms, against 5.26 ms for the lane-per-AO kernel that wrote
-// this array before.  The same launch without its stores takes 1.93 ms; the
+// written): 4.00 ms, against 5.26 ms for the lane-per-AO kernel.  The same launch without its stores takes 1.93 ms; the
 // stores add the rest because the 20 KB ring leaves room for eight
 // wavefronts per CU, too few to hide them.  Runs of 4 AOs took 10.4 ms,
 // runs of 16 AOs (40 KB ring, four wavefronts per CU) 6.2 ms; holding one

@@ -144,23 +144,28 @@ def test_ragged_shapes(oracle_lib, P_walk, mo_num):
     assert np.max(np.abs(out["mo_value"] - out["mo"][:, 0, :])) <= 1e-12 * (1 + np.abs(mo).max())
 
 
-def test_high_l_and_screened_stages_vs_oracle(oracle_lib):
-    """Shells l = 0..6 on two nuclei 80 bohr apart (84 AOs each, 168 in all:
-    not a multiple of 32 or 64), 100 points (not a multiple of 64), the first
-    70 near nucleus 0.  The points of the first 64-point tile see nucleus 1
-    screened, so its AOs 128..167 (a whole, part-filled 64-AO stage) are
-    exact zeros there; l = 4..6 take the kernel's loop-count path; the l = 6
-    shell of each nucleus straddles a 64-AO stage boundary."""
+@pytest.mark.parametrize("nucl_num", [2, 14])
+def test_high_l_and_screened_stages_vs_oracle(oracle_lib, nucl_num):
+    """Shells l = 0..6 on nuclei 80 bohr apart along z (84 AOs each: 168 AOs,
+    not a multiple of 32 or 64, for the lane-per-AO ao_vgl kernel; 1,176 AOs,
+    a multiple of 8 above 1024, for the lane-per-point one), 100 points (not a
+    multiple of 64), the first 70 near nucleus 0 and the rest near the last
+    nucleus.  The points of the first 64-point tile see every other nucleus
+    screened, so its AOs from 84 on (whole 64-AO stages, the last one
+    part-filled) are exact zeros there; l = 4..6 take the kernels'
+    loop-count paths; the l = 6 shell of nucleus 0 straddles a 64-AO stage
+    boundary."""
     rng = np.random.default_rng(77)
-    xyz = np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 80.0]])
+    xyz = np.zeros((nucl_num, 3))
+    xyz[:, 2] = 80.0 * np.arange(nucl_num)
     bb = S._BasisBuilder()
-    for _ in range(2):
+    for _ in range(nucl_num):
         bb.add_nucleus([(l, int(rng.integers(1, 5))) for l in range(7)], rng, (0.1, 50.0))
     arr = bb.arrays()
     A = int(sum(S.ncart(int(l)) for l in arr["shell_ang_mom"]))
-    assert A == 168
-    elec = S._electrons_near(rng, xyz, [0] * 70 + [1] * 30, 1)
-    s = S.System(name="far_pair_l6", charge=np.array([1.0, 1.0]), nucl_coord=np.ascontiguousarray(xyz.T),
+    assert A == 84 * nucl_num
+    elec = S._electrons_near(rng, xyz, [0] * 70 + [nucl_num - 1] * 30, 1)
+    s = S.System(name="far_chain_l6", charge=np.ones(nucl_num), nucl_coord=np.ascontiguousarray(xyz.T),
                  mo_coef=rng.normal(0.0, 0.3, size=(40, A)), elec_up_num=50, elec_dn_num=50,
                  elec_coord=elec, **arr)
     s.validate()
@@ -168,12 +173,12 @@ def test_high_l_and_screened_stages_vs_oracle(oracle_lib):
         c.set_system(s)
         out = device_chain(c, s.point_num, want=("ao", "ao_value", "mo"))
     ao, am = oracle_lib.ao_vgl(s, with_mag=True)
-    assert np.all(ao[:64, :, 128:] == 0.0), "the oracle screens nucleus 1 for the first 64 points"
-    assert np.all(out["ao"][:64, :, 128:] == 0.0)
-    assert_vgl_close(out["ao"], ao, am, what="far pair l<=6 ao_vgl")
+    assert np.all(ao[:64, :, 84:] == 0.0), "the oracle screens the far nuclei for the first 64 points"
+    assert np.all(out["ao"][:64, :, 84:] == 0.0)
+    assert_vgl_close(out["ao"], ao, am, what=f"far chain of {nucl_num}, l<=6 ao_vgl")
     assert np.max(np.abs(out["ao_value"] - out["ao"][:, 0, :])) <= 1e-12 * np.abs(ao).max()
     mo, mm = oracle_lib.mo_vgl_from_ao(s.mo_coef, ao, am)
-    assert_vgl_close(out["mo"], mo, mm, what="far pair l<=6 mo_vgl")
+    assert_vgl_close(out["mo"], mo, mm, what=f"far chain of {nucl_num}, l<=6 mo_vgl")
 
 
 def test_fp64_point_order_keeps_rows(oracle_lib):

@@ -187,7 +187,9 @@ def bench_batch(forms, steps, warmup, scale=1.0):
 
 
 # device kernel name (as the profiler lists it) of each timed kernel
-KERNEL_SYMBOL = {"en_distance": "en_distance_kernel", "ao_vgl": "ao_kernel<5, false, double>",
+# (the FP64 ao_vgl kernel depends on the basis: lane-per-point for 1024 AOs
+# or more in whole 64-byte rows, lane-per-AO otherwise)
+KERNEL_SYMBOL = {"en_distance": "en_distance_kernel", "ao_vgl": ("ao_vgl_f64_kernel", "ao_kernel<5, false, double>"),
                  "ao_vgl_f32": "ao_rows_kernel", "mo_gemm_f64": "gemm_f64_frag_kernel<",
                  "mo_gemm_fp32": "gemm_f32_kernel<", "mo_gemm_fp16x2": "gemm_split_kernel<2,",
                  "mo_gemm_bf16x2": "gemm_split_kernel<3,", "mo_gemm_bf16x3": "gemm_split_kernel<4,"}
@@ -334,7 +336,9 @@ def traffic_profile(args):
     per.update(_pmc_per_kernel(" ".join(mops), run))
 
     def mean_of(counter, sym):
-        hits = [v for k, v in per[counter].items() if sym in k.replace("(anonymous namespace)::", "")]
+        syms = (sym,) if isinstance(sym, str) else sym
+        hits = [v for k, v in per[counter].items()
+                if any(x in k.replace("(anonymous namespace)::", "") for x in syms)]
         return sum(hits) / len(hits) if hits else None
 
     kernels = {}

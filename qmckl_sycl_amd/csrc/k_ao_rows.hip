@@ -438,8 +438,8 @@ struct F64Writer {
   uint8_t *flags;    // zeroed [tiles][nkb], or null
   int nkb, lane;
 
-  // the ring's first n slots hold AOs kb .. kb + n - 1
-  // the ring's first n slots hold AOs kb .. kb + n - 1
+  // the ring's first n slots hold AOs kb .. kb + n - 1: read them all out
+  // (lane l: slot l % kRing64 of kRing64 points), then store
   __device__ __attribute__((always_inline)) void flush(int kb, int n) {
     __builtin_amdgcn_wave_barrier();
     const int s = lane % kRing64;

@@ -408,7 +408,9 @@ __global__ void __launch_bounds__(kThreads) fill_zero_kernel(int64_t n, uint32_t
 // runs of 16 AOs (40 KB ring, four wavefronts per CU) 6.2 ms; holding one
 // 8-AO flush in registers to write whole 128-byte lines spilled to scratch;
 // plain stores instead of nontemporal ones 4.30 ms; the caller's point order
-// instead of compact tiles 4.40 ms.
+// instead of compact tiles 4.40 ms; 16-byte stores (two AOs per lane, half
+// the store instructions) 1% faster, which would need a second path for
+// callers' arrays that are only 8-byte aligned.
 #ifndef QMCKL_AO_F64_RING
 #define QMCKL_AO_F64_RING 8
 #endif

@@ -581,14 +581,14 @@ qmckl_exit_code_device qmckl_provide_ao_basis_shell_vgl_device(qmckl_context_dev
 }
 
 // Compute ao_vgl (ncomp = 5) or ao_value (ncomp = 1) for the current points
-// into `dst`.  The lane-per-point ao_vgl kernel runs over spatially compact
-// point tiles where tile_point_order says so (each point's rows are the
-// same either way).
+// into `dst`.  The lane-per-point kernel runs over spatially compact point
+// tiles where tile_point_order says so (each point's rows are the same
+// either way).
 static qmckl_exit_code_device compute_ao(qmckl_context_device context, Context *c, double *dst,
                                          int ncomp, const char *fn, uint8_t *flags = nullptr,
                                          int nkb = 0) {
   const int32_t *perm = nullptr;
-  hipError_t e = ncomp == 5 && ao_vgl_lane_kernel(c->ao.ao_num) ? tile_point_order(c, &perm) : hipSuccess;
+  hipError_t e = ao_vgl_lane_kernel(c->ao.ao_num) ? tile_point_order(c, &perm) : hipSuccess;
   if (e == hipErrorOutOfMemory) return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
   if (e == hipSuccess) {
     KernelTimer t(c, ncomp == 5 ? "ao_vgl" : "ao_value");

@@ -355,25 +355,26 @@ hipError_t launch_en_distance(hipStream_t s, int64_t P, int64_t N, const double 
                               const double *nucl, double *out);
 hipError_t launch_shell_vgl(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
                             const double *nucl, double *shell_vgl);
-// ao_vgl (ncomp 5) or ao_value (ncomp 1); perm (ncomp 5 with
-// ao_vgl_lane_kernel only, or null) is an order of the points into
-// spatially compact 64-point tiles
+// ao_vgl (ncomp 5) or ao_value (ncomp 1); perm (with ao_vgl_lane_kernel
+// only, or null) is an order of the points into spatially compact 64-point
+// tiles
 hipError_t launch_ao_vgl(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
                          const double *nucl, const double *ao_factor, double *ao_vgl,
                          int ncomp, uint8_t *flags = nullptr, int nkb = 0, const int32_t *perm = nullptr);
-// Which kernel writes FP64 ao_vgl: the lane-per-point one (k_ao_rows.hip)
+// Which kernel writes FP64 ao_vgl and ao_value: the lane-per-point one (k_ao_rows.hip)
 // for bases of 1024 AOs or more whose rows are whole 64-byte runs, the
 // lane-per-AO one (k_ao.hip) otherwise.  Measured on MI355X: water64 (4,160
 // AOs, 81,920 points) 4.00 against 5.26 ms; the 48-molecule ragged batch
 // (30-620 AOs, rows mostly not 64-byte aligned) 22.3 against 10.0 ms in all.
 inline bool ao_vgl_lane_kernel(int64_t A) { return A >= 1024 && A % 8 == 0; }
-// FP64 ao_vgl [P][5][A] with one lane per point (k_ao_rows.hip), the
-// points of lane-tile t being perm[64 t ..] (null: 64 t ..), and, if flags
-// is not null, nonzero flags [(5P + 63) / 64][nkb], nkb = ceil(A / 32), per
-// (64 rows of [point * 5 + component], 32 AOs).
+// FP64 ao_vgl [P][5][A] (ncomp 5) or ao_value [P][A] (ncomp 1) with one
+// lane per point (k_ao_rows.hip), the points of lane-tile t being
+// perm[64 t ..] (null: 64 t ..), and, for ao_vgl if flags is not null,
+// nonzero flags [(5P + 63) / 64][nkb], nkb = ceil(A / 32), per (64 rows of
+// [point * 5 + component], 32 AOs).
 hipError_t launch_ao_vgl_f64(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
                              const int32_t *perm, const double *nucl, const double *ao_factor, double *out,
-                             uint8_t *flags, int nkb);
+                             int ncomp, uint8_t *flags, int nkb);
 // The point order for spatially compact 64-point tiles of the lane-per-point
 // AO kernels: *perm = ao.point_perm, computed here on the context's stream,
 // or null for the caller's order.  The tile configuration's point_order

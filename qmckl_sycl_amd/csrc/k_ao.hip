@@ -314,8 +314,9 @@ hipError_t launch_ao(hipStream_t s, const DeviceBasis &b, int64_t P, const doubl
 hipError_t launch_ao_vgl(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
                          const double *nucl, const double *ao_factor, double *out, int ncomp,
                          uint8_t *flags, int nkb, const int32_t *perm) {
-  if (ncomp == 5 && ao_vgl_lane_kernel(b.ao_num))
-    return launch_ao_vgl_f64(s, b, P, coord, perm, nucl, ao_factor, out, flags, nkb);
+  if (ao_vgl_lane_kernel(b.ao_num))
+    return launch_ao_vgl_f64(s, b, P, coord, perm, nucl, ao_factor, out, ncomp, ncomp == 5 ? flags : nullptr,
+                             ncomp == 5 ? nkb : 0);
   AoOut<double> ao{out, b.ao_num, flags, nkb};
   if (ncomp == 5) return launch_ao<5, false, double>(s, b, P, coord, nucl, ao_factor, nullptr, 0, ao);
   ao.flags = nullptr;

@@ -380,7 +380,9 @@ __global__ void __launch_bounds__(kThreads) fill_zero_kernel(int64_t n, uint32_t
 }
 
 // ---------------------------------------------------------------------------
-// FP64 ao_vgl in the caller's layout ao_vgl[point][component][AO], with the
+// FP64 ao_vgl in the caller's layout ao_vgl[point][component][AO] (NC = 5),
+// or ao_value[point][AO] (NC = 1: a 4 KB ring, five wavefronts per SIMD,
+// 0.93 against 1.86 ms for the lane-per-AO kernel on water64), with the
 // same lane-per-point evaluation.  A lane's five rows are 5 A doubles apart
 // and its own stores would touch one cache line per lane, so the values go
 // through a ring in LDS, [component][AO mod kRing64][point], and every
@@ -426,13 +428,14 @@ constexpr int kRingLd = 64;
 __device__ __attribute__((always_inline)) inline int ring_col(int slot, int q) { return q ^ (8 * (slot & 7)); }
 static_assert(64 % kRing64 == 0 && kRing64 >= 2 && kRing64 <= 16, "ring length");
 
+template <int NC>
 struct F64Writer {
   uint32_t nz;       // bit t: an entry of this lane's rows in row tile t0 + t is nonzero (current 32-AO block)
   int64_t t0;        // row tile of this lane's first row (point * 5)
   int cs;            // first component whose row is in tile t0 + 1 (5: none)
   int k;             // next AO index (wavefront-uniform)
   int k_lo, k_hi;    // AOs of the current stage
-  double *ring;      // this wavefront's [5][kRing64][kRingLd] LDS ring
+  double *ring;      // this wavefront's [NC][kRing64][kRingLd] LDS ring
   int p;             // this lane's point (-1 past P)
   int pq[kRing64];   // the points whose slots this lane moves in a flush
   double *out;
@@ -445,20 +448,20 @@ struct F64Writer {
   __device__ __attribute__((always_inline)) void flush(int kb, int n) {
     __builtin_amdgcn_wave_barrier();
     const int s = lane % kRing64;
-    double v[kRing64][5];
+    double v[kRing64][NC];
 #pragma unroll
     for (int j = 0; j < kRing64; ++j) {
       const int q = j * (64 / kRing64) + lane / kRing64;
 #pragma unroll
-      for (int c = 0; c < 5; ++c) v[j][c] = ring[(c * kRing64 + s) * kRingLd + ring_col(s, q)];
+      for (int c = 0; c < NC; ++c) v[j][c] = ring[(c * kRing64 + s) * kRingLd + ring_col(s, q)];
     }
     __builtin_amdgcn_wave_barrier();
 #pragma unroll
     for (int j = 0; j < kRing64; ++j) {
       if (pq[j] < 0) continue;
-      double *row = out + (int64_t)pq[j] * 5 * A + s;
+      double *row = out + (int64_t)pq[j] * NC * A + s;
 #pragma unroll
-      for (int c = 0; c < 5; ++c) {
+      for (int c = 0; c < NC; ++c) {
         if (s < n) __builtin_nontemporal_store(v[j][c], row + c * A + kb);
       }
     }
@@ -477,9 +480,9 @@ struct F64Writer {
     }
     const int slot = k % kRing64;
 #pragma unroll
-    for (int c = 0; c < 5; ++c) {
+    for (int c = 0; c < NC; ++c) {
       ring[(c * kRing64 + slot) * kRingLd + ring_col(slot, lane)] = v[c];
-      nz |= (v[c] != 0.0 ? 1u : 0u) << (c >= cs ? 1 : 0);
+      if constexpr (NC == 5) nz |= (v[c] != 0.0 ? 1u : 0u) << (c >= cs ? 1 : 0);
     }
     if (slot == kRing64 - 1) flush(k - (kRing64 - 1), kRing64);
     if ((k & 31) == 31) flag_block(k >> 5);
@@ -497,22 +500,23 @@ struct F64Writer {
     for (int q = 0; q < 64; ++q) {
       const int pu = __builtin_amdgcn_readlane(p, q);
       if (pu < 0) break;  // the points past P are the last ones
-      double *row = out + (int64_t)pu * 5 * A + k_lo + lane;
+      double *row = out + (int64_t)pu * NC * A + k_lo + lane;
 #pragma unroll
-      for (int c = 0; c < 5; ++c)
+      for (int c = 0; c < NC; ++c)
         if (lane < n) __builtin_nontemporal_store(0.0, row + c * A);
     }
   }
 };
 
-__global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves_per_eu(2)))
+template <int NC>
+__global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves_per_eu(NC == 1 ? 4 : 2)))
 ao_vgl_f64_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord, const int32_t *__restrict__ perm,
                   const double *__restrict__ nucl, const ShellDesc *__restrict__ shells,
                   const int32_t *__restrict__ stage_shell, const double *__restrict__ ao_factor,
                   const double *__restrict__ expo, const double *__restrict__ coef,
                   const double *__restrict__ nucl_range, int64_t A, int groups, double *__restrict__ out,
                   uint8_t *__restrict__ flags, int nkb) {
-  __shared__ double s_ring[5 * kRing64 * kRingLd];
+  __shared__ double s_ring[NC * kRing64 * kRingLd];
   const int lane = threadIdx.x;
   const int64_t pt = blockIdx.x / groups;
   const int g = (int)(blockIdx.x - pt * groups);
@@ -526,7 +530,7 @@ ao_vgl_f64_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord,
   // this wavefront's AOs: 64-AO stages [g*nst/groups, (g+1)*nst/groups)
   const int64_t nst = (A + 63) / 64;
   const int64_t st_lo = g * nst / groups, st_hi = (g + 1) * nst / groups;
-  F64Writer w;
+  F64Writer<NC> w;
   w.nz = 0;
   w.t0 = (5 * p) >> 6;
   w.cs = inside && 64 - (int)((5 * p) & 63) < 5 ? 64 - (int)((5 * p) & 63) : 5;
@@ -564,11 +568,12 @@ __global__ void __launch_bounds__(kThreads) fill_zero_bytes_kernel(int64_t n, ui
 
 hipError_t launch_ao_vgl_f64(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
                              const int32_t *perm, const double *nucl, const double *ao_factor, double *out,
-                             uint8_t *flags, int nkb) {
+                             int ncomp, uint8_t *flags, int nkb) {
   if (P <= 0) return hipSuccess;
   const int64_t A = b.ao_num;
-  if (b.max_l > kMaxL || A <= 0 || b.stage_shell == nullptr) return hipErrorInvalidValue;
-  if (flags != nullptr && (int64_t)nkb != (A + 31) / 32) return hipErrorInvalidValue;
+  if (b.max_l > kMaxL || A <= 0 || b.stage_shell == nullptr || (ncomp != 5 && ncomp != 1))
+    return hipErrorInvalidValue;
+  if (flags != nullptr && (ncomp != 5 || (int64_t)nkb != (A + 31) / 32)) return hipErrorInvalidValue;
   const int64_t npt = (P + 63) / 64;
   const int groups = ao_rows_groups(P, (A + 63) / 64 * 64, kAoF64GroupWaves);
   const int64_t blocks = npt * groups;
@@ -579,9 +584,14 @@ hipError_t launch_ao_vgl_f64(hipStream_t s, const DeviceBasis &b, int64_t P, con
     hipLaunchKernelGGL(fill_zero_bytes_kernel, dim3((unsigned)std::min<int64_t>((n + kThreads - 1) / kThreads, 4096)),
                        dim3(kThreads), 0, s, n, flags);
   }
-  hipLaunchKernelGGL(ao_vgl_f64_kernel, dim3((unsigned)blocks), dim3(64), 0, s, P, (int)b.nucl_num, b.shell_num,
-                     coord, perm, nucl, b.shells, b.stage_shell, ao_factor, b.expo, b.coef_norm, b.nucl_range, A,
-                     groups, out, flags, nkb);
+  if (ncomp == 5)
+    hipLaunchKernelGGL(ao_vgl_f64_kernel<5>, dim3((unsigned)blocks), dim3(64), 0, s, P, (int)b.nucl_num,
+                       b.shell_num, coord, perm, nucl, b.shells, b.stage_shell, ao_factor, b.expo, b.coef_norm,
+                       b.nucl_range, A, groups, out, flags, nkb);
+  else
+    hipLaunchKernelGGL(ao_vgl_f64_kernel<1>, dim3((unsigned)blocks), dim3(64), 0, s, P, (int)b.nucl_num,
+                       b.shell_num, coord, perm, nucl, b.shells, b.stage_shell, ao_factor, b.expo, b.coef_norm,
+                       b.nucl_range, A, groups, out, nullptr, 0);
   return hipGetLastError();
 }
 

@@ -172,9 +172,14 @@ def test_high_l_and_screened_stages_vs_oracle(oracle_lib, nucl_num):
     with Context(0) as c:
         c.set_system(s)
         out = device_chain(c, s.point_num, want=("ao", "ao_value", "mo"))
+        c.touch()
+        value_only = c.ao_value(s.point_num)  # ao_vgl is stale: the value-only kernel runs
     ao, am = oracle_lib.ao_vgl(s, with_mag=True)
     assert np.all(ao[:64, :, 84:] == 0.0), "the oracle screens the far nuclei for the first 64 points"
     assert np.all(out["ao"][:64, :, 84:] == 0.0)
+    assert np.all(value_only[:64, 84:] == 0.0)
+    assert_vgl_close(value_only[:, None, :], ao[:, :1, :], am[:, :1, :], rtol=1e-12,
+                     what=f"far chain of {nucl_num}, l<=6 ao_value")
     assert_vgl_close(out["ao"], ao, am, what=f"far chain of {nucl_num}, l<=6 ao_vgl")
     assert np.max(np.abs(out["ao_value"] - out["ao"][:, 0, :])) <= 1e-12 * np.abs(ao).max()
     mo, mm = oracle_lib.mo_vgl_from_ao(s.mo_coef, ao, am)
@@ -182,21 +187,28 @@ def test_high_l_and_screened_stages_vs_oracle(oracle_lib, nucl_num):
 
 
 def test_fp64_point_order_keeps_rows(oracle_lib):
-    """FP64 ao_vgl over spatially ordered 64-point tiles (the default at this
-    size: 4,160 AOs, 5,120 points) against the caller's order: every point's
-    rows, and the MOs after them, are bitwise the same, and a sample of
-    points matches the oracle."""
+    """FP64 ao_vgl and ao_value over spatially ordered 64-point tiles (the
+    default at this size: 4,160 AOs, 5,120 points) against the caller's
+    order: every point's rows, and the MOs after them, are bitwise the same,
+    the value-only kernel's rows equal the value rows of ao_vgl, and a sample
+    of points matches the oracle."""
     s = S.water64(walk_num=8)
     P = s.point_num
     with Context(0) as c:
         c.set_system(s)
         c.call("qmckl_set_mo_basis_tile_config_device", 1, 0)
         ao1, mo1 = c.ao_vgl(P), c.mo_vgl(P)
+        c.touch()
+        av1 = c.ao_value(P)  # ao_vgl is stale: the value-only kernel runs
         c.call("qmckl_set_mo_basis_tile_config_device", 0, 0)
+        c.touch()
+        av0 = c.ao_value(P)
         c.touch()
         ao0, mo0 = c.ao_vgl(P), c.mo_vgl(P)
     assert np.array_equal(ao1, ao0)
     assert np.array_equal(mo1, mo0)
+    assert np.array_equal(av1, av0)
+    assert np.array_equal(av1, ao1[:, 0, :])
     idx = np.sort(np.random.default_rng(3).choice(P, size=64, replace=False))
     coord = np.ascontiguousarray(s.coord_T()[:, idx])
     ao, am = oracle_lib.ao_vgl(s, coord, with_mag=True)

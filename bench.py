@@ -211,6 +211,55 @@ def measured(s, kernel, key="bytes_per_launch"):
     return None if k is None else k.get(key)
 
 
+def bench_value_only(steps, warmup):
+    """The value-only path (mo_value; src/qmckl_mo.cpp:185-453) at the C3
+    size, FP64: per step the points are touched and mo_value provided, which
+    computes ao_value and the value-only contraction (mo_vgl is not
+    computed)."""
+    import torch
+    from qmckl_sycl_amd.context import Context, lib
+    L = lib()
+    s = make_system("water64", 0)
+    ctx = Context(0)
+    try:
+        ctx.set_system(s)
+        h = ctx.handle
+
+        def step():
+            ctx.check(L.qmckl_context_touch_device(h))
+            ctx.check(L.qmckl_provide_mo_basis_mo_value_device(h))
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        sec = (time.perf_counter() - t0) / steps
+        res = {"workload": f"water64 at {s.walk_num} walkers ({s.point_num} points), FP64 mo_value "
+                           "(ao_value, then the value-only contraction)",
+               "ms_per_step": sec * 1e3, "value": s.point_num / sec, "unit": "electron-points/s"}
+        ctx.check(L.qmckl_reset_kernel_timing_device(h))
+        ctx.check(L.qmckl_set_kernel_timing_device(h, True))
+        for _ in range(2):
+            step()
+        ctx.check(L.qmckl_set_kernel_timing_device(h, False))
+        kernels = {}
+        for name in ("point_order", "ao_value", "mo_value_gemm_f64"):
+            n, ms = ctypes.c_int64(0), ctypes.c_double(0.0)
+            L.qmckl_get_kernel_timing_device(h, name.encode(), ctypes.byref(n), ctypes.byref(ms))
+            if n.value:
+                kernels[name] = {"launches": int(n.value), "avg_ms": ms.value / n.value}
+        if "ao_value" in kernels:  # algorithmic bytes: 8 per (point, AO) written, 24 per point read
+            kernels["ao_value"]["achieved_gbs"] = (s.point_num * (24.0 + 8.0 * s.ao_num) /
+                                                   (kernels["ao_value"]["avg_ms"] / 1e3) / 1e9)
+        res["kernels"] = kernels
+        log("value_only", json.dumps(res))
+        return res
+    finally:
+        ctx.destroy()
+
+
 def bench_adjoint(steps, warmup):
     """Differentiation form at C5 (water64, 32 walkers, weights w ~ N(0,1)
     shaped like mo_vgl): the forward FP64 mo_vgl then the adjoint (dL/dR,
@@ -497,6 +546,7 @@ def main():
     if not args.no_batch:
         out["batch_48_ragged"] = bench_batch(["plain", args.form], max(2, args.steps // 3), 1)
         out["adjoint_c5"] = bench_adjoint(max(2, args.steps // 3), 1)
+        out["value_only_fp64"] = bench_value_only(args.steps, 1)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(s, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -365,7 +365,9 @@ hipError_t launch_ao_vgl(hipStream_t s, const DeviceBasis &b, int64_t P, const d
 // for bases of 1024 AOs or more whose rows are whole 64-byte runs, the
 // lane-per-AO one (k_ao.hip) otherwise.  Measured on MI355X: water64 (4,160
 // AOs, 81,920 points) 4.00 against 5.26 ms; the 48-molecule ragged batch
-// (30-620 AOs, rows mostly not 64-byte aligned) 22.3 against 10.0 ms in all.
+// (30-620 AOs, rows mostly not 64-byte aligned) 22.3 against 10.0 ms in all;
+// on its molecules with 64-byte rows alone 1.40 against 1.31 ms (the lane
+// kernel ahead only at 560 AOs).
 inline bool ao_vgl_lane_kernel(int64_t A) { return A >= 1024 && A % 8 == 0; }
 // FP64 ao_vgl [P][5][A] (ncomp 5) or ao_value [P][A] (ncomp 1) with one
 // lane per point (k_ao_rows.hip), the points of lane-tile t being

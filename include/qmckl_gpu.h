@@ -305,6 +305,20 @@ qmckl_exit_code_device qmckl_compute_ao_vgl_gaussian_device(
     const int64_t *nucleus_index, const int64_t *nucleus_shell_num,
     const double *nucleus_range, const int32_t *nucleus_max_ang_mom,
     const int32_t *shell_ang_mom, const double *ao_factor, double *shell_vgl, double *ao_vgl);
+/* Allocation-free form of qmckl_compute_ao_basis_shell_gaussian_vgl_device
+ * (same arguments, same results).  It has no counterpart in the reference:
+ * it is for callers that cannot allocate (code under HIP graph capture,
+ * serving loops).  It builds its shell table on the device in a fixed
+ * per-GPU device array instead of allocating, so it calls no hipMalloc or
+ * hipFree.  Limit: shell_num <= 65536 (ARG_3 beyond).  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (other threads, other
+ * contexts) race and must be serialised by the caller.  Invalid index arrays
+ * return QMCKL_FAILURE_DEVICE before the radial kernel runs. */
+qmckl_exit_code_device qmckl_compute_ao_basis_shell_gaussian_vgl_noalloc_device(
+    qmckl_context_device context, int prim_num, int shell_num, int point_num, int nucl_num,
+    const int64_t *nucleus_shell_num, const int64_t *nucleus_index, const double *nucleus_range,
+    const int64_t *shell_prim_index, const int64_t *shell_prim_num, const double *coord,
+    const double *nucl_coord, const double *expo, const double *coef_normalized, double *shell_vgl);
 
 /* ---- MO basis (include/qmckl_gpu.hpp:758-801) ---------------------------- */
 

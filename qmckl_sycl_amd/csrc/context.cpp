@@ -233,6 +233,10 @@ qmckl_context_device qmckl_context_create_device(int device_id) {
       return QMCKL_NULL_CONTEXT_DEVICE;
     }
     c->device_id = device_id;
+    // address of the allocation-free launcher's device scratch on this
+    // device (once per device); on failure that launcher retries and
+    // reports the error itself, every other entry point is unaffected
+    if (noalloc_scratch_init(device_id) != hipSuccess) (void)hipGetLastError();
   }
   {
     std::lock_guard<std::mutex> g(g_live_mutex);

@@ -496,4 +496,21 @@ hipError_t launch_mo_gemm_mixed_persistent(hipStream_t s, int form, int cbw, con
 hipError_t launch_strided_copy(hipStream_t s, int64_t rows, int64_t cols, const double *in,
                                int64_t ld_in, double *out, int64_t ld_out);
 
+// Allocation-free radial launcher (noalloc.hip): the shell table is built on
+// the device into a fixed file-scope device array of kNoAllocMaxShells
+// entries.  noalloc_scratch_init(device) looks up that array's address on
+// `device` (which must be current) once per device; context creation calls
+// it.  launch_shell_vgl_noalloc validates the index arrays on the device,
+// waits for that check on `s` and sets *table_err (bit 1: nucleus ranges,
+// 2: shell coverage, 4: primitive ranges) instead of running the radial
+// kernel when it fails.
+constexpr int64_t kNoAllocMaxShells = 65536;
+hipError_t noalloc_scratch_init(int device);
+hipError_t launch_shell_vgl_noalloc(hipStream_t s, int device, int64_t P, int64_t N, int64_t S,
+                                    int64_t Np, const int64_t *nucleus_shell_num,
+                                    const int64_t *nucleus_index, const double *nucleus_range,
+                                    const int64_t *shell_prim_index, const int64_t *shell_prim_num,
+                                    const double *coord, const double *nucl, const double *expo,
+                                    const double *coef_normalized, double *shell_vgl, int *table_err);
+
 }  // namespace qmckl_amd

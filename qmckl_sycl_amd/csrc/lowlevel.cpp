@@ -69,6 +69,42 @@ qmckl_exit_code_device qmckl_compute_ao_basis_shell_gaussian_vgl_device(
   return hip_check(context, e, fn);
 }
 
+qmckl_exit_code_device qmckl_compute_ao_basis_shell_gaussian_vgl_noalloc_device(
+    qmckl_context_device context, int prim_num, int shell_num, int point_num, int nucl_num,
+    const int64_t *nucleus_shell_num, const int64_t *nucleus_index, const double *nucleus_range,
+    const int64_t *shell_prim_index, const int64_t *shell_prim_num, const double *coord,
+    const double *nucl_coord, const double *expo, const double *coef_normalized, double *shell_vgl) {
+  Context *c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+  const char *fn = "qmckl_compute_ao_basis_shell_gaussian_vgl_noalloc_device";
+  if (prim_num <= 0) return fail(context, QMCKL_INVALID_ARG_2_DEVICE, fn, "prim_num <= 0");
+  if (shell_num <= 0) return fail(context, QMCKL_INVALID_ARG_3_DEVICE, fn, "shell_num <= 0");
+  if ((int64_t)shell_num > kNoAllocMaxShells)
+    return fail(context, QMCKL_INVALID_ARG_3_DEVICE, fn,
+                "shell_num exceeds the 65536 shells of the allocation-free scratch table");
+  if (point_num < 0) return fail(context, QMCKL_INVALID_ARG_4_DEVICE, fn, "point_num < 0");
+  if (nucl_num <= 0) return fail(context, QMCKL_INVALID_ARG_5_DEVICE, fn, "nucl_num <= 0");
+  if (!nucleus_shell_num || !nucleus_index || !nucleus_range || !shell_prim_index ||
+      !shell_prim_num || !coord || !nucl_coord || !expo || !coef_normalized || !shell_vgl)
+    return fail(context, QMCKL_INVALID_ARG_6_DEVICE, fn, "NULL pointer");
+  if (!has_device(c)) return fail(context, QMCKL_FAILURE_DEVICE, fn, "no device bound");
+  if (hipSetDevice(c->device_id) != hipSuccess) return hip_check(context, hipGetLastError(), fn);
+  int table_err = 0;
+  hipError_t e = launch_shell_vgl_noalloc(c->stream, c->device_id, point_num, nucl_num, shell_num, prim_num,
+                                          nucleus_shell_num, nucleus_index, nucleus_range, shell_prim_index,
+                                          shell_prim_num, coord, nucl_coord, expo, coef_normalized, shell_vgl,
+                                          &table_err);
+  if (e == hipSuccess && table_err != 0) {
+    // the messages of the allocating launcher's host-side checks
+    const char *msg = (table_err & 1) ? "nucleus_index/nucleus_shell_num out of range"
+                      : (table_err & 4) ? "shell_prim_index/shell_prim_num out of range"
+                                        : "a shell belongs to no nucleus or to two";
+    return fail(context, QMCKL_FAILURE_DEVICE, fn, msg);
+  }
+  if (e == hipSuccess) e = sync(c);
+  return hip_check(context, e, fn);
+}
+
 qmckl_exit_code_device qmckl_compute_ao_vgl_gaussian_device(
     qmckl_context_device context, int64_t ao_num, int64_t shell_num, int64_t point_num,
     int64_t nucl_num, const double *coord, const double *nucl_coord,

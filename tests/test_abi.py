@@ -209,6 +209,15 @@ def test_low_level_launchers_validate(ctx):
     assert L.qmckl_compute_en_distance_device(h, 0, 3, 1, 1, 1) == 2
     assert L.qmckl_compute_en_distance_device(h, 3, 0, 1, 1, 1) == 3
     assert L.qmckl_compute_en_distance_device(h, 3, 3, 1, 1, 1) == FAILURE
+    na = L.qmckl_compute_ao_basis_shell_gaussian_vgl_noalloc_device
+    ptrs = [1] * 10
+    assert na(h, 0, 4, 4, 1, *ptrs) == 2
+    assert na(h, 4, 0, 4, 1, *ptrs) == 3
+    assert na(h, 4, 65537, 4, 1, *ptrs) == 3       # past the fixed scratch table
+    assert na(h, 4, 4, -1, 1, *ptrs) == 4
+    assert na(h, 4, 4, 4, 0, *ptrs) == 5
+    assert na(h, 4, 4, 4, 1, *([1] * 9 + [None])) == 6
+    assert na(h, 4, 4, 4, 1, *ptrs) == FAILURE       # no device bound
 
 
 def test_destroy_frees_host_allocations():

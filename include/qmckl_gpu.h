@@ -427,9 +427,11 @@ qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_batch_device(int64_t count,
  * per CU the AO operand kernel aims at (group_waves 0 automatic, or 1..1024).
  * set: ARG_2 / ARG_3 for values out of range.  get: ARG_2 / ARG_3 for NULL.
  * autotune: times the context's mixed chain on its current points for
- * point_order 0/1 x group_waves 16/32/64/128/256 and keeps the fastest (a
- * process-wide cache reuses the choice for contexts of the same device,
- * point, AO, MO and nucleus counts and form); FP64 form: nothing to choose.
+ * point_order 0/1 x group_waves 16/32/64/128/256, keeps the fastest, then
+ * times the contraction kernels of qmckl_set_mo_basis_gemm_tile_device
+ * below with it and keeps the fastest (a process-wide cache reuses the
+ * choices for contexts of the same device, point, AO, MO and nucleus counts
+ * and form); FP64 form: nothing to choose.
  * Needs the basis and points (NOT_PROVIDED otherwise).  It overwrites the
  * context's cached mo_vgl, which is recomputed on the next get. */
 qmckl_exit_code_device qmckl_set_mo_basis_tile_config_device(qmckl_context_device context, int32_t point_order,
@@ -437,6 +439,25 @@ qmckl_exit_code_device qmckl_set_mo_basis_tile_config_device(qmckl_context_devic
 qmckl_exit_code_device qmckl_get_mo_basis_tile_config_device(qmckl_context_device context, int32_t *point_order,
                                                              int32_t *group_waves);
 qmckl_exit_code_device qmckl_mo_basis_autotune_device(qmckl_context_device context);
+/* A third tile choice of the split forms (FP16X2, BF16X2, BF16X3): the
+ * contraction kernel.  AUTO (0) is LIVE where blocks can be dead (the MO
+ * column plan is in use, or the basis has 1024 AOs or more) and DENSE
+ * otherwise.  LIVE (1): blocks of zero
+ * coefficients or screened AOs are neither loaded nor multiplied, the AO
+ * operand is streamed with nontemporal loads; LIVE_CACHED (2): the same
+ * with cached AO loads; DENSE (3): every block of a live 64-AO stage is
+ * multiplied, dead ones on zeros; LIVE64 (4): LIVE with 64 rows per
+ * wavefront (one wavefront per SIMD).  Results are the same bit for bit
+ * for every choice.  set: ARG_2 for an unknown value; get: ARG_2 for NULL.
+ * The autotuner above also times the four kernels (after the point order
+ * and AO wavefronts) and keeps the fastest. */
+#define QMCKL_GEMM_TILE_AUTO_DEVICE ((int32_t)0)
+#define QMCKL_GEMM_TILE_LIVE_DEVICE ((int32_t)1)
+#define QMCKL_GEMM_TILE_LIVE_CACHED_DEVICE ((int32_t)2)
+#define QMCKL_GEMM_TILE_DENSE_DEVICE ((int32_t)3)
+#define QMCKL_GEMM_TILE_LIVE64_DEVICE ((int32_t)4)
+qmckl_exit_code_device qmckl_set_mo_basis_gemm_tile_device(qmckl_context_device context, int32_t gemm_tile);
+qmckl_exit_code_device qmckl_get_mo_basis_gemm_tile_device(qmckl_context_device context, int32_t *gemm_tile);
 
 /* ---- MO column plan (new; no counterpart in the reference) ----------------
  * At MO finalize the library maps the coefficients' zeros in blocks of 16

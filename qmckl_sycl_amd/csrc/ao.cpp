@@ -262,6 +262,7 @@ qmckl_exit_code_device qmckl_finalize_ao_basis_device(qmckl_context_device conte
   }
   // A new basis invalidates every cached AO/MO result and the mixed-form
   // operands (their padded row length depends on ao_num).
+  c->basis_generation += 1;
   ao.shell_vgl_date = ao.ao_vgl_date = ao.ao_value_date = 0;
   c->mo.mo_vgl_date = c->mo.mo_value_date = 0;
   dfree(c, ao.ao_f32);

@@ -186,6 +186,7 @@ struct MixedGemmJob {
   int64_t ldc;
   ColPlan plan;
   int cbw;                     // tile width: 64 * cbw MOs
+  int dense;                   // 1: the DENSE contraction kernel, 0: LIVE
   int64_t item0, gx, items;    // first item, row-tile halves, items (gx x column groups)
 };
 struct MoBasis {
@@ -219,6 +220,7 @@ struct MoBasis {
   // reach it); tuned: set by the autotuner
   int32_t tile_point_order = -1;
   int32_t tile_group_waves = 0;
+  int32_t tile_gemm = 0;  // contraction kernel of the split forms (QMCKL_GEMM_TILE_*)
   bool tile_tuned = false;
   // adjoint scratch: G = w C [5P][A], per-block partials, split-K partials
   double *adj_G = nullptr;
@@ -261,6 +263,9 @@ struct Context {
   std::unordered_map<void *, size_t> device_allocs;
   std::unordered_map<void *, size_t> host_allocs;
   uint64_t date = 0;
+  // bumped by every AO or MO finalize: a recorded batch graph compares it
+  // (the device tables a finalize rebuilds may land at the old addresses)
+  uint64_t basis_generation = 0;
   bool timing = false;
   // set while a batch enqueues this context's work: the compute chain's
   // stream synchronisations are skipped and the batch waits once at the end
@@ -480,19 +485,19 @@ hipError_t launch_split_coef(hipStream_t s, int form, int64_t M, int64_t K, cons
 hipError_t launch_mo_gemm_mixed(hipStream_t s, int form, int64_t P, int64_t K, int64_t Ncol,
                                 const float *A, int64_t lda, const float *rowmax, const void *B, int64_t ldb,
                                 const float *colinv, const uint8_t *flags, int nkb, const int32_t *perm,
-                                double *C, int64_t ldc, ColPlan plan = ColPlan());
+                                double *C, int64_t ldc, ColPlan plan = ColPlan(), int gemm_tile = 0);
 // Fills *job for a split form (the checks of launch_mo_gemm_mixed); item0 is
 // left to the caller.  The FP32 form has no persistent launch.
 hipError_t make_mixed_gemm_job(int form, int64_t P, int64_t K, int64_t Ncol, const float *A, int64_t lda,
                                const float *rowmax, const void *B, int64_t ldb, const float *colinv,
                                const uint8_t *flags, int nkb, const int32_t *perm, double *C, int64_t ldc,
-                               ColPlan plan, MixedGemmJob *job);
+                               ColPlan plan, MixedGemmJob *job, int gemm_tile = 0);
 // One launch for every job of the same split form and tile width cbw: jobs
 // [njobs] in device memory with consecutive item ranges covering [0, total),
 // counter a device int the launch zeroes first (a kernel, so it can be
 // captured).
-hipError_t launch_mo_gemm_mixed_persistent(hipStream_t s, int form, int cbw, const MixedGemmJob *jobs, int njobs,
-                                           int64_t total, int *counter);
+hipError_t launch_mo_gemm_mixed_persistent(hipStream_t s, int form, int cbw, int dense, const MixedGemmJob *jobs,
+                                           int njobs, int64_t total, int *counter);
 hipError_t launch_strided_copy(hipStream_t s, int64_t rows, int64_t cols, const double *in,
                                int64_t ld_in, double *out, int64_t ld_out);
 

@@ -21,6 +21,7 @@
 // B[8(l/16)..+7][l%16] and receives D[4(l/16)+j][l%16], j = 0..3.
 #include <hip/hip_runtime.h>
 
+
 #include "internal.hpp"
 
 namespace qmckl_amd {
@@ -441,21 +442,330 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
                           C, ldc, plan, sA, s_inv, s_list, s_live, &s_count, s_out);
 }
 
+// Live-block form of the split tile (the default for the split forms).
+// Same operands, sums, scales and FP64 accumulation per stage as
+// split_tile, so the results are the same bit for bit; what differs is
+// what is issued.  Counters on MI355X (water64, 128 walkers) showed the
+// four-wavefront kernel above bound by the vector-memory return path, 85%
+// busy, nine tenths of its bytes coefficient fragments, and the dead ones
+// (zero blocks of the column plan, AOs screened for the tile) costing as
+// much there as live ones although their loads touch no memory.  Here:
+//   - a (column block, 32-K half) that the stage list marks dead is
+//     neither loaded nor multiplied: its loads are not issued and each
+//     wavefront branches (uniformly) around its MFMAs; the first live half
+//     of a block in a stage starts its FP32 sum from zero, and only blocks
+//     live in the stage are added into the FP64 accumulators;
+//   - the FP32 AO slab is read with nontemporal loads (NT), so that the
+//     streamed operand does not push the fragments out of L2, and is
+//     double-buffered in LDS (one barrier per stage);
+//   - what the epilogue needs per row (the point the row is stored at) and
+//     per column (its place in the MO group, its inverse scale) is fetched
+//     into LDS at the start, so the output stores wait on no load; rows go
+//     out 16 at a time through the idle slab buffer, as 16-byte
+//     nontemporal runs when the rows are 16-byte aligned.
+// Measured at 409,600 x 4,160 x 320 (water64, 128 walkers, FP16X2): 2.36 ms
+// against 2.97 ms for split_tile; 52% of its MFMAs are issued.  RB = 4
+// (the whole 64-row tile per wavefront, one wavefront per SIMD) halves the
+// fragment bytes per MFMA but runs 3.1 ms: one wavefront per SIMD hides
+// too little latency.
+template <int SP, int CBW, int RB, bool NT>
+__device__ __forceinline__ void split_tile_live(int64_t bx, int64_t by, int64_t P, int64_t npt, int64_t K,
+                                                int64_t Ncol, const float *__restrict__ A, int64_t lda,
+                                                const float *__restrict__ rowmax,
+                                                const uint16_t *__restrict__ Bs, int64_t ldb,
+                                                const float *__restrict__ colinv,
+                                                const uint8_t *__restrict__ flags, int nkb,
+                                                const int32_t *__restrict__ perm, double *__restrict__ C,
+                                                int64_t ldc, ColPlan plan, uint16_t *sA, float *s_inv,
+                                                int *s_list, uint64_t *s_live, int *s_count, int32_t *s_perm,
+                                                int32_t *s_mloc, float *s_ci) {
+  constexpr int T = Terms<SP>::n;
+  constexpr int WR = 16 * RB;
+  constexpr int KQ = WR / 4;  // K per thread in the A slab (256 threads, 64 K)
+  constexpr int SLAB = T * WR * kLDS;  // halves per LDS slab
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int cq = __builtin_amdgcn_readfirstlane(wave);  // column quarter
+  const int64_t rt = bx / (64 / WR);
+  const int r0 = (int)(bx % (64 / WR)) * WR;
+  const int64_t comp = rt / npt, p0 = (rt - comp * npt) * 64 + r0;
+  const int64_t gb0 = by * 4 * CBW;
+  const int nst = (int)((K + kStage - 1) / kStage);
+
+  // epilogue tables
+  if (tid < WR) {
+    const int64_t i = p0 + tid;
+    s_perm[tid] = i < P ? (perm != nullptr ? perm[i] : (int32_t)i) : -1;
+  }
+  for (int c = tid; c < 64 * CBW; c += 256) {
+    const int64_t col = gb0 * 16 + c;
+    s_mloc[c] = col < Ncol ? (int32_t)((plan.col_mo != nullptr ? (int64_t)plan.col_mo[col] : col) - gb0 * 16) : -1;
+    s_ci[c] = col < Ncol ? colinv[col] : 0.0f;
+  }
+
+  // stage list (as in split_tile)
+  const bool listed = flags != nullptr || plan.blk != nullptr;
+  int count = nst;
+  if (listed) {
+    if (wave == 0) {
+      const uint8_t *f = flags != nullptr ? flags + rt * nkb : nullptr;
+      int base = 0;
+      for (int s0 = 0; s0 < nst; s0 += 64) {
+        const int st = s0 + lane;
+        const bool h0 = st < nst && (f == nullptr || f[2 * st] != 0);
+        const bool h1 = st < nst && (f == nullptr || (2 * st + 1 < nkb && f[2 * st + 1] != 0));
+        uint64_t lv = (h0 ? 0xffffffffull : 0ull) | (h1 ? 0xffffffff00000000ull : 0ull);
+        if ((h0 || h1) && plan.blk != nullptr) {
+          const int64_t g = 4 * (int64_t)st;
+          uint64_t w[4 * CBW];
+#pragma unroll
+          for (int b = 0; b < 4 * CBW; ++b) {
+            const int64_t blk = gb0 + b < plan.nblk ? gb0 + b : plan.nblk - 1;
+            w[b] = plan.blk[blk * plan.nw + (g >> 6)];
+          }
+          uint64_t bits = 0;
+#pragma unroll
+          for (int b = 0; b < 4 * CBW; ++b) {
+            const uint64_t nib = gb0 + b < plan.nblk ? (w[b] >> (g & 63)) & 15ull : 0ull;
+            bits |= ((nib & 3ull) != 0 ? 1ull : 0ull) << b;
+            bits |= ((nib & 12ull) != 0 ? 1ull : 0ull) << (32 + b);
+          }
+          lv &= bits;
+        }
+        const bool live = lv != 0;
+        const uint64_t m = __ballot(live);
+        if (live) {
+          const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
+          s_list[pos] = st;
+          s_live[pos] = lv;
+        }
+        base += __popcll(m);
+      }
+      if (lane == 0) *s_count = base;
+    }
+    __syncthreads();
+    count = *s_count;
+  }
+  // this wavefront's live blocks in half ks of stage s: bit 4j for block
+  // gb0 + cq + 4j
+  auto wave_live = [&](int s, int ks) -> uint32_t {
+    if (!listed) return 0xffffffffu;
+    const uint64_t lv = s_live[s];
+    const uint32_t w = ks == 0 ? (uint32_t)lv : (uint32_t)(lv >> 32);
+    return (uint32_t)__builtin_amdgcn_readfirstlane(w) >> cq;
+  };
+
+  const int ar = tid % WR, aq = (tid / WR) * KQ;
+  const float *Ap = A + rt * (lda / 4) * 256 + (aq / 4) * 256 + (r0 + ar) * 4;
+  const float sc = pow2_scale(rowmax[rt * 64 + r0 + ar]);
+  if (aq == 0) s_inv[ar] = 1.0f / sc;
+  const int64_t nkc = ldb / 32;
+  const int tile_bytes = (int)(nkc * T * 1024);
+  const __amdgpu_buffer_rsrc_t brs = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)(Bs + gb0 * nkc * T * 512), (short)0, 4 * CBW * tile_bytes, 0x00020000);
+  const int lane_off = 16 * lane;
+
+  float4 ra[KQ / 4];
+  uint4 bq[2][CBW][T];
+  auto stage_k0 = [&](int s) {
+    return (int64_t)__builtin_amdgcn_readfirstlane(listed ? s_list[s] : s) * kStage;
+  };
+  auto load_a = [&](int64_t k0) {
+    const float *src = Ap + (k0 / 4) * 256;
+#pragma unroll
+    for (int q = 0; q < KQ / 4; ++q) {
+      if constexpr (NT) {
+        const f4 v = __builtin_nontemporal_load(reinterpret_cast<const f4 *>(src + q * 256));
+        ra[q] = make_float4(v[0], v[1], v[2], v[3]);
+      } else {
+        ra[q] = *reinterpret_cast<const float4 *>(src + q * 256);
+      }
+    }
+  };
+  // fragments of half ks of the stage at k0 for this wavefront's live blocks
+  auto load_b = [&](int64_t k0, int ks, uint32_t live) {
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb) {
+      if ((live >> (4 * cb)) & 1u) {
+        const int so = (cq + 4 * cb) * tile_bytes + (int)(k0 / 32 + ks) * T * 1024;
+#pragma unroll
+        for (int t = 0; t < T; ++t)
+          bq[ks][cb][t] =
+              __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(brs, lane_off + t * 1024, so, 0));
+      }
+    }
+  };
+  auto write_slab = [&](int buf) {
+    uint16_t *dst = sA + buf * SLAB;
+#pragma unroll
+    for (int h = 0; h < KQ / 8; ++h) {
+      const float v[8] = {ra[2 * h].x * sc,     ra[2 * h].y * sc,     ra[2 * h].z * sc,
+                          ra[2 * h].w * sc,     ra[2 * h + 1].x * sc, ra[2 * h + 1].y * sc,
+                          ra[2 * h + 1].z * sc, ra[2 * h + 1].w * sc};
+      uint4 o[T];
+      split8<SP>(v, o);
+#pragma unroll
+      for (int t = 0; t < T; ++t) *reinterpret_cast<uint4 *>(&dst[t * (WR * kLDS) + ar * kLDS + aq + 8 * h]) = o[t];
+    }
+  };
+
+  f4 d[RB][CBW];
+  double acc[RB][CBW][4];
+#pragma unroll
+  for (int r = 0; r < RB; ++r)
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[r][cb][j] = 0.0;
+
+  if (count > 0) {
+    const int64_t k0 = stage_k0(0);
+    load_a(k0);
+    load_b(k0, 0, wave_live(0, 0));
+    load_b(k0, 1, wave_live(0, 1));
+  } else {
+    __syncthreads();
+  }
+  for (int s = 0; s < count; ++s) {
+    const int buf = s & 1;
+    write_slab(buf);
+    __syncthreads();
+    const bool more = s + 1 < count;
+    const int64_t k1 = more ? stage_k0(s + 1) : 0;
+    const uint32_t live1[2] = {more ? wave_live(s + 1, 0) : 0u, more ? wave_live(s + 1, 1) : 0u};
+    if (more) load_a(k1);
+    const uint32_t live0[2] = {wave_live(s, 0), wave_live(s, 1)};
+    const uint16_t *slab = sA + buf * SLAB;
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      uint4 a[T][RB];
+#pragma unroll
+      for (int t = 0; t < T; ++t)
+#pragma unroll
+        for (int r = 0; r < RB; ++r)
+          a[t][r] = *reinterpret_cast<const uint4 *>(&slab[t * (WR * kLDS) + (r * 16 + li) * kLDS + ks * 32 + 8 * lk]);
+      constexpr int NP = T == 3 ? 6 : 3;
+      constexpr int pa[6] = {0, 0, 1, 0, 2, 1}, pb[6] = {0, 1, 0, 2, 0, 1};  // (A term, C term)
+#pragma unroll
+      for (int cb = 0; cb < CBW; ++cb) {
+        if (!((live0[ks] >> (4 * cb)) & 1u)) continue;
+        const bool fresh = ks == 0 || !((live0[0] >> (4 * cb)) & 1u);  // first live half of the stage
+        if (fresh) {
+#pragma unroll
+          for (int pr = 0; pr < NP; ++pr)
+#pragma unroll
+            for (int r = 0; r < RB; ++r)
+              d[r][cb] = mfma<SP>(a[pa[pr]][r], bq[ks][cb][pb[pr]], pr == 0 ? f4{0.f, 0.f, 0.f, 0.f} : d[r][cb]);
+        } else {
+#pragma unroll
+          for (int pr = 0; pr < NP; ++pr)
+#pragma unroll
+            for (int r = 0; r < RB; ++r) d[r][cb] = mfma<SP>(a[pa[pr]][r], bq[ks][cb][pb[pr]], d[r][cb]);
+        }
+      }
+      if (more) load_b(k1, ks, live1[ks]);  // this half's fragment registers are free again
+    }
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb) {
+      if (!(((live0[0] | live0[1]) >> (4 * cb)) & 1u)) continue;
+#pragma unroll
+      for (int r = 0; r < RB; ++r)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[r][cb][j] += (double)d[r][cb][j];
+    }
+  }
+
+  // epilogue: 16 rows at a time through the slab buffer, in MO order
+  const int64_t g_lo = gb0 * 16;
+  const int g_n = (int)(Ncol - g_lo < 64 * CBW ? Ncol - g_lo : 64 * CBW);
+  constexpr int OR = 16;
+  double *so = reinterpret_cast<double *>(sA);
+  int mloc[CBW];
+  double ci[CBW];
+#pragma unroll
+  for (int cb = 0; cb < CBW; ++cb) {
+    const int c = (cq + 4 * cb) * 16 + li;  // the lane's column within the group
+    mloc[cb] = s_mloc[c];
+    ci[cb] = (double)s_ci[c];
+  }
+  const bool vec = (ldc % 2) == 0 && (g_n % 2) == 0 && (reinterpret_cast<uintptr_t>(C) % 16) == 0;
+#pragma unroll
+  for (int pass = 0; pass < WR / OR; ++pass) {  // pass = 16-row block r
+    __syncthreads();  // the slab (or the previous pass) has been read
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (mloc[cb] >= 0)
+          so[(4 * lk + j) * (64 * CBW) + mloc[cb]] = acc[pass][cb][j] * ((double)s_inv[pass * 16 + 4 * lk + j] * ci[cb]);
+    __syncthreads();
+    for (int row = 0; row < OR; ++row) {
+      const int32_t p = s_perm[pass * OR + row];
+      if (p < 0) break;  // rows past the last point
+      double *dst = C + ((int64_t)p * 5 + comp) * ldc + g_lo;
+      if (vec) {
+        typedef double d2 __attribute__((ext_vector_type(2)));
+        for (int cp = tid; cp < g_n / 2; cp += 256) {
+          const d2 v = {so[row * (64 * CBW) + 2 * cp], so[row * (64 * CBW) + 2 * cp + 1]};
+          __builtin_nontemporal_store(v, reinterpret_cast<d2 *>(dst + 2 * cp));
+        }
+      } else {
+        for (int c = tid; c < g_n; c += 256) dst[c] = so[row * (64 * CBW) + c];
+      }
+    }
+  }
+}
+
+// LDS of one split_tile_live workgroup (the slab buffer doubles as the
+// epilogue's 16-row output buffer)
+template <int SP, int CBW, int RB>
+struct LiveTileLds {
+  static constexpr int kSlab = 2 * Terms<SP>::n * 16 * RB * kLDS;  // halves
+  static constexpr int kOut = 16 * 64 * CBW * 4;                   // halves
+  static constexpr int kA = kSlab > kOut ? kSlab : kOut;
+};
+
+template <int SP, int CBW, int RB, bool NT>
+__global__ void __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(RB == 4 ? 1 : 2, RB == 4 ? 1 : 2)))
+gemm_split_live_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *__restrict__ A, int64_t lda,
+                       const float *__restrict__ rowmax, const uint16_t *__restrict__ Bs, int64_t ldb,
+                       const float *__restrict__ colinv, const uint8_t *__restrict__ flags, int nkb,
+                       const int32_t *__restrict__ perm, double *__restrict__ C, int64_t ldc, ColPlan plan) {
+  using L = LiveTileLds<SP, CBW, RB>;
+  __shared__ __attribute__((aligned(16))) uint16_t sA[L::kA];
+  __shared__ float s_inv[16 * RB];
+  __shared__ int s_list[kMaxStages];
+  __shared__ uint64_t s_live[kMaxStages];
+  __shared__ int s_count;
+  __shared__ int32_t s_perm[16 * RB];
+  __shared__ int32_t s_mloc[64 * CBW];
+  __shared__ float s_ci[64 * CBW];
+  split_tile_live<SP, CBW, RB, NT>(blockIdx.x, blockIdx.y, P, npt, K, Ncol, A, lda, rowmax, Bs, ldb, colinv, flags,
+                                   nkb, perm, C, ldc, plan, sA, s_inv, s_list, s_live, &s_count, s_perm, s_mloc,
+                                   s_ci);
+}
+
 // Persistent form over a batch: 2 workgroups per CU take (molecule, row
 // tile, column group) items from an atomic counter until the batch is done;
 // jobs[j] describes molecule j's contraction (all of tile width CBW) and
 // owns items [item0, item0 + items), x-major over its gx row-tile halves.
-// Every workgroup leaves the loop once the counter passes `total`.
-template <int SP, int CBW>
+// Every workgroup leaves the loop once the counter passes `total`.  Each
+// item runs split_tile_live (LIVE) or split_tile.
+template <int SP, int CBW, bool LIVE>
 __global__ void __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(2, 2)))
 gemm_split_persistent_kernel(const MixedGemmJob *__restrict__ jobs, int njobs, int64_t total, int *counter) {
   constexpr int WR = 32;
-  __shared__ uint16_t sA[Terms<SP>::n * WR * kLDS];
+  using L = LiveTileLds<SP, CBW, 2>;
+  __shared__ __attribute__((aligned(16))) uint16_t sA[LIVE ? L::kA : Terms<SP>::n * WR * kLDS];
   __shared__ float s_inv[WR];
   __shared__ int s_list[kMaxStages];
   __shared__ uint64_t s_live[kMaxStages];
   __shared__ int s_count;
-  __shared__ double s_out[kOutRows * 64 * CBW];
+  __shared__ double s_out[LIVE ? 1 : kOutRows * 64 * CBW];
+  __shared__ int32_t s_perm[LIVE ? WR : 1];
+  __shared__ int32_t s_mloc[LIVE ? 64 * CBW : 1];
+  __shared__ float s_ci[LIVE ? 64 * CBW : 1];
   __shared__ int64_t s_item;
   __shared__ int s_job;
   for (;;) {
@@ -482,8 +792,13 @@ gemm_split_persistent_kernel(const MixedGemmJob *__restrict__ jobs, int njobs, i
     const MixedGemmJob &g = jobs[j];
     const int64_t local = item - g.item0;
     const int64_t bx = local % g.gx, by = local / g.gx;
-    split_tile<SP, CBW, WR>(bx, by, g.P, g.npt, g.K, g.Ncol, g.A, g.lda, g.rowmax, g.Bs, g.ldb, g.colinv, g.flags,
-                            g.nkb, g.perm, g.C, g.ldc, g.plan, sA, s_inv, s_list, s_live, &s_count, s_out);
+    if constexpr (LIVE)
+      split_tile_live<SP, CBW, 2, true>(bx, by, g.P, g.npt, g.K, g.Ncol, g.A, g.lda, g.rowmax, g.Bs, g.ldb,
+                                        g.colinv, g.flags, g.nkb, g.perm, g.C, g.ldc, g.plan, sA, s_inv, s_list,
+                                        s_live, &s_count, s_perm, s_mloc, s_ci);
+    else
+      split_tile<SP, CBW, WR>(bx, by, g.P, g.npt, g.K, g.Ncol, g.A, g.lda, g.rowmax, g.Bs, g.ldb, g.colinv, g.flags,
+                              g.nkb, g.perm, g.C, g.ldc, g.plan, sA, s_inv, s_list, s_live, &s_count, s_out);
   }
 }
 
@@ -660,11 +975,30 @@ hipError_t launch_split(hipStream_t s, int64_t P, int64_t npt, int64_t K, int64_
   return hipGetLastError();
 }
 
+template <int SP, int CBW, int RB, bool NT>
+hipError_t launch_split_live(hipStream_t s, int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *A,
+                             int64_t lda, const float *rowmax, const void *B, int64_t ldb, const float *colinv,
+                             const uint8_t *flags, int nkb, const int32_t *perm, double *C, int64_t ldc,
+                             ColPlan plan) {
+  constexpr int WR = 16 * RB;
+  const int64_t gx = 5 * npt * (64 / WR), gy = (Ncol + 64 * CBW - 1) / (64 * CBW);
+  if (gx > 0x7fffffff || gy > 65535 || (lda % kStage) != 0 || (ldb % kStage) != 0)
+    return hipErrorInvalidValue;
+  if (flags != nullptr && (nkb > kMaxFlagBlocks || (int64_t)nkb * 32 < K)) return hipErrorInvalidValue;
+  if (plan.blk != nullptr && ((K + kStage - 1) / kStage > kMaxStages || plan.nw * 64 * 16 < K || plan.nblk < 0))
+    return hipErrorInvalidValue;
+  if ((int64_t)4 * CBW * (ldb / 32) * Terms<SP>::n * 1024 >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((gemm_split_live_kernel<SP, CBW, RB, NT>), dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, s,
+                     P, npt, K, Ncol, A, lda, rowmax, (const uint16_t *)B, ldb, colinv, flags, nkb, perm, C, ldc,
+                     plan);
+  return hipGetLastError();
+}
+
 template <int SP>
 hipError_t launch_form(hipStream_t s, int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *A,
                        int64_t lda, const float *rowmax, const void *B, int64_t ldb,
                        const float *colinv, const uint8_t *flags, int nkb, const int32_t *perm, double *C,
-                       int64_t ldc, ColPlan plan) {
+                       int64_t ldc, ColPlan plan, int gemm_tile) {
   if constexpr (SP == QMCKL_AMD_SPLIT_FP32) {
     if (plan.col_mo != nullptr || plan.blk != nullptr) return hipErrorInvalidValue;
     const int64_t nb = (Ncol + 31) / 32;  // 32-column groups
@@ -675,6 +1009,26 @@ hipError_t launch_form(hipStream_t s, int64_t P, int64_t npt, int64_t K, int64_t
     return launch_f32<5>(s, P, npt, K, Ncol, A, lda, B, ldb, flags, nkb, perm, C, ldc);
   } else {
     const int cbw = split_wg_cols(SP, Ncol) / 64;
+    if (gemm_tile != QMCKL_GEMM_TILE_DENSE_DEVICE) {
+#define QMCKL_LIVE(W)                                                                                              \
+  switch (gemm_tile) {                                                                                            \
+    case QMCKL_GEMM_TILE_LIVE_CACHED_DEVICE:                                                                     \
+      return launch_split_live<SP, W, 2, false>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc, plan); \
+    case QMCKL_GEMM_TILE_LIVE64_DEVICE:                                                                          \
+      return launch_split_live<SP, W, 4, true>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc, plan); \
+    default:                                                                                                      \
+      return launch_split_live<SP, W, 2, true>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc, plan); \
+  }
+      if (cbw == 1) { QMCKL_LIVE(1) }
+      if (cbw == 2) { QMCKL_LIVE(2) }
+      if (cbw == 3) { QMCKL_LIVE(3) }
+      if constexpr (Terms<SP>::n == 2) {
+        if (cbw == 4) { QMCKL_LIVE(4) }
+        if (cbw == 5) { QMCKL_LIVE(5) }
+      }
+#undef QMCKL_LIVE
+      return hipErrorInvalidValue;
+    }
     if (cbw == 1) return launch_split<SP, 1>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc, plan);
     if (cbw == 2) return launch_split<SP, 2>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc, plan);
     if (cbw == 3) return launch_split<SP, 3>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc, plan);
@@ -725,7 +1079,7 @@ hipError_t launch_split_coef(hipStream_t s, int form, int64_t M, int64_t K, cons
 hipError_t make_mixed_gemm_job(int form, int64_t P, int64_t K, int64_t Ncol, const float *A, int64_t lda,
                                const float *rowmax, const void *B, int64_t ldb, const float *colinv,
                                const uint8_t *flags, int nkb, const int32_t *perm, double *C, int64_t ldc,
-                               ColPlan plan, MixedGemmJob *job) {
+                               ColPlan plan, MixedGemmJob *job, int gemm_tile) {
   if (form != QMCKL_AMD_SPLIT_FP16X2 && form != QMCKL_AMD_SPLIT_BF16X2 && form != QMCKL_AMD_SPLIT_BF16X3)
     return hipErrorInvalidValue;
   if (P <= 0 || Ncol <= 0 || K <= 0 || (lda % 64) != 0 || (ldb % 64) != 0 || lda < K || ldb < K)
@@ -754,6 +1108,7 @@ hipError_t make_mixed_gemm_job(int form, int64_t P, int64_t K, int64_t Ncol, con
   job->ldc = ldc;
   job->plan = plan;
   job->cbw = cbw;
+  job->dense = gemm_tile == QMCKL_GEMM_TILE_DENSE_DEVICE ? 1 : 0;
   job->item0 = 0;
   job->gx = 5 * npt * 2;  // 32-row halves of the 64-row tiles
   job->items = job->gx * ((Ncol + 64 * cbw - 1) / (64 * cbw));
@@ -766,8 +1121,8 @@ __global__ void zero_int_kernel(int *p) {
 }
 }  // namespace
 
-hipError_t launch_mo_gemm_mixed_persistent(hipStream_t s, int form, int cbw, const MixedGemmJob *jobs, int njobs,
-                                           int64_t total, int *counter) {
+hipError_t launch_mo_gemm_mixed_persistent(hipStream_t s, int form, int cbw, int dense, const MixedGemmJob *jobs,
+                                           int njobs, int64_t total, int *counter) {
   if (njobs <= 0 || total <= 0) return hipSuccess;
   // tile widths up to 192 MOs: wider persistent tiles need more registers
   // than two workgroups per CU leave (the 320-wide one spills to scratch)
@@ -779,9 +1134,15 @@ hipError_t launch_mo_gemm_mixed_persistent(hipStream_t s, int form, int cbw, con
   if (e != hipSuccess) return e;
   const unsigned grid = (unsigned)(2 * (cus > 0 ? cus : 1));  // two workgroups per CU
   hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, s, counter);
-#define QMCKL_PERSISTENT(SPV, W)                                                                            \
-  hipLaunchKernelGGL((gemm_split_persistent_kernel<SPV, W>), dim3(grid), dim3(256), 0, s, jobs, njobs, total, \
-                     counter)
+#define QMCKL_PERSISTENT(SPV, W)                                                                                   \
+  do {                                                                                                            \
+    if (dense)                                                                                                    \
+      hipLaunchKernelGGL((gemm_split_persistent_kernel<SPV, W, false>), dim3(grid), dim3(256), 0, s, jobs, njobs, \
+                         total, counter);                                                                         \
+    else                                                                                                          \
+      hipLaunchKernelGGL((gemm_split_persistent_kernel<SPV, W, true>), dim3(grid), dim3(256), 0, s, jobs, njobs,  \
+                         total, counter);                                                                         \
+  } while (0)
   switch (form * 8 + cbw) {
     case QMCKL_AMD_SPLIT_FP16X2 * 8 + 1: QMCKL_PERSISTENT(QMCKL_AMD_SPLIT_FP16X2, 1); break;
     case QMCKL_AMD_SPLIT_FP16X2 * 8 + 2: QMCKL_PERSISTENT(QMCKL_AMD_SPLIT_FP16X2, 2); break;
@@ -801,20 +1162,20 @@ hipError_t launch_mo_gemm_mixed_persistent(hipStream_t s, int form, int cbw, con
 hipError_t launch_mo_gemm_mixed(hipStream_t s, int form, int64_t P, int64_t K, int64_t Ncol,
                                 const float *A, int64_t lda, const float *rowmax, const void *B, int64_t ldb,
                                 const float *colinv, const uint8_t *flags, int nkb, const int32_t *perm,
-                                double *C, int64_t ldc, ColPlan plan) {
+                                double *C, int64_t ldc, ColPlan plan, int gemm_tile) {
   if (P <= 0 || Ncol <= 0) return hipSuccess;
   if (K <= 0 || (lda % 64) != 0 || (ldb % 64) != 0 || lda < K || ldb < K)
     return hipErrorInvalidValue;
   const int64_t npt = (P + 63) / 64;
   switch (form) {
     case QMCKL_AMD_SPLIT_FP16X2:
-      return launch_form<QMCKL_AMD_SPLIT_FP16X2>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc, plan);
+      return launch_form<QMCKL_AMD_SPLIT_FP16X2>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc, plan, gemm_tile);
     case QMCKL_AMD_SPLIT_BF16X2:
-      return launch_form<QMCKL_AMD_SPLIT_BF16X2>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc, plan);
+      return launch_form<QMCKL_AMD_SPLIT_BF16X2>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc, plan, gemm_tile);
     case QMCKL_AMD_SPLIT_BF16X3:
-      return launch_form<QMCKL_AMD_SPLIT_BF16X3>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc, plan);
+      return launch_form<QMCKL_AMD_SPLIT_BF16X3>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc, plan, gemm_tile);
     case QMCKL_AMD_SPLIT_FP32:
-      return launch_form<QMCKL_AMD_SPLIT_FP32>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc, plan);
+      return launch_form<QMCKL_AMD_SPLIT_FP32>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc, plan, gemm_tile);
     default:
       return hipErrorInvalidValue;
   }

@@ -206,6 +206,15 @@ qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Contex
                            groups, ao.ao_f32, ao.ao_f32_rowmax, flags, flags ? nkb : 0);
     t.stop();
   }
+  // The contraction kernel: by default the live-block kernel where blocks
+  // can be dead (a column plan, or bases of 1024 AOs or more whose 64-AO
+  // stages the point order screens per tile), the dense one otherwise.
+  // Measured on MI355X: water64 (4,160 AOs, planned columns) 2.38 against
+  // 2.97 ms; the 48-molecule ragged batch (30-620 AOs, dense coefficients)
+  // 144 M points/s with the dense kernel against 123 M with the live one.
+  int gemm_tile = mo.tile_gemm;
+  if (gemm_tile == QMCKL_GEMM_TILE_AUTO_DEVICE)
+    gemm_tile = plan.col_mo != nullptr || ld >= 1024 ? QMCKL_GEMM_TILE_LIVE_DEVICE : QMCKL_GEMM_TILE_DENSE_DEVICE;
   // In an eager batch, split forms of tile width <= 3 leave their
   // contraction to the batch's persistent launches (wider tiles need more
   // registers than the persistent kernel has to spare).
@@ -214,12 +223,12 @@ qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Contex
   if (e == hipSuccess && collect) {
     MixedGemmJob job;
     e = make_mixed_gemm_job(form, P, A, M, ao.ao_f32, ld, ao.ao_f32_rowmax, mo.coef_split, ld, mo.coef_colinv,
-                            flags, flags ? nkb : 0, perm, dst, M, plan, &job);
+                            flags, flags ? nkb : 0, perm, dst, M, plan, &job, gemm_tile);
     if (e == hipSuccess) c->collect_jobs->push_back(job);
   } else if (e == hipSuccess) {
     KernelTimer t(c, gemm_timer_name(form));
     e = launch_mo_gemm_mixed(c->stream, form, P, A, M, ao.ao_f32, ld, ao.ao_f32_rowmax, mo.coef_split, ld,
-                             mo.coef_colinv, flags, flags ? nkb : 0, perm, dst, M, plan);
+                             mo.coef_colinv, flags, flags ? nkb : 0, perm, dst, M, plan, gemm_tile);
     t.stop();
   }
   if (e == hipSuccess) e = sync(c);
@@ -295,6 +304,7 @@ qmckl_exit_code_device qmckl_get_mo_basis_mo_num_device(qmckl_context_device con
 qmckl_exit_code_device qmckl_finalize_mo_basis_device(qmckl_context_device context) {
   Context *c = ctx_of(context);
   if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+  c->basis_generation += 1;
   const char *fn = "qmckl_finalize_mo_basis_device";
   if (c->mo.coefficient == nullptr) return fail(context, QMCKL_NOT_PROVIDED_DEVICE, fn, nullptr);
   const int64_t A = c->ao.ao_num, M = c->mo.mo_num;
@@ -567,7 +577,7 @@ namespace {
 // The split-form contractions a batch collected, grouped by (form, tile
 // width): one persistent launch per group, items numbered from 0 within each.
 struct PersistentGroup {
-  int form, cbw;
+  int form, cbw, dense;
   size_t first, n;
   int64_t total;
 };
@@ -583,13 +593,14 @@ PersistentPlan plan_persistent(const std::vector<MixedGemmJob> &jobs, const std:
   for (size_t i = 0; i < jobs.size(); ++i) order[i] = i;
   // one job per collecting context, in context order: its form is that
   // context's precision
-  auto key = [&](size_t i) { return (int)job_ctx[i]->mo.precision * 8 + jobs[i].cbw; };
+  auto key = [&](size_t i) { return ((int)job_ctx[i]->mo.precision * 8 + jobs[i].cbw) * 2 + jobs[i].dense; };
   std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return key(x) < key(y); });
   for (size_t k = 0; k < order.size(); ++k) {
     MixedGemmJob j = jobs[order[k]];
     const int form = (int)job_ctx[order[k]]->mo.precision;
-    if (pp.groups.empty() || pp.groups.back().form != form || pp.groups.back().cbw != j.cbw)
-      pp.groups.push_back(PersistentGroup{form, j.cbw, k, 0, 0});
+    if (pp.groups.empty() || pp.groups.back().form != form || pp.groups.back().cbw != j.cbw ||
+        pp.groups.back().dense != j.dense)
+      pp.groups.push_back(PersistentGroup{form, j.cbw, j.dense, k, 0, 0});
     PersistentGroup &g = pp.groups.back();
     j.item0 = g.total;
     g.total += j.items;
@@ -604,6 +615,7 @@ bool same_job(const MixedGemmJob &x, const MixedGemmJob &y) {
          x.rowmax == y.rowmax && x.Bs == y.Bs && x.ldb == y.ldb && x.colinv == y.colinv && x.flags == y.flags &&
          x.nkb == y.nkb && x.perm == y.perm && x.C == y.C && x.ldc == y.ldc && x.plan.col_mo == y.plan.col_mo &&
          x.plan.blk == y.plan.blk && x.plan.nw == y.plan.nw && x.plan.nblk == y.plan.nblk && x.cbw == y.cbw &&
+         x.dense == y.dense &&
          x.item0 == y.item0 && x.gx == y.gx && x.items == y.items;
 }
 
@@ -622,7 +634,8 @@ hipError_t launch_persistent(Context *c0, const PersistentPlan &pp, MixedGemmJob
   }
   for (size_t g = 0; g < pp.groups.size() && e == hipSuccess; ++g) {
     KernelTimer t(c0, "mo_gemm_batch_persistent");
-    e = launch_mo_gemm_mixed_persistent(c0->stream, pp.groups[g].form, pp.groups[g].cbw, table + pp.groups[g].first,
+    e = launch_mo_gemm_mixed_persistent(c0->stream, pp.groups[g].form, pp.groups[g].cbw, pp.groups[g].dense,
+                                        table + pp.groups[g].first,
                                         (int)pp.groups[g].n, pp.groups[g].total, counters + g);
     t.stop();
   }
@@ -732,9 +745,12 @@ struct TuneKeyHash {
     return h;
   }
 };
+struct TuneChoice {
+  int32_t point_order, group_waves, gemm_tile;
+};
 std::mutex g_tune_mutex;
-std::unordered_map<TuneKey, std::pair<int32_t, int32_t>, TuneKeyHash> &tune_cache() {
-  static std::unordered_map<TuneKey, std::pair<int32_t, int32_t>, TuneKeyHash> m;
+std::unordered_map<TuneKey, TuneChoice, TuneKeyHash> &tune_cache() {
+  static std::unordered_map<TuneKey, TuneChoice, TuneKeyHash> m;
   return m;
 }
 
@@ -752,6 +768,26 @@ qmckl_exit_code_device qmckl_set_mo_basis_tile_config_device(qmckl_context_devic
   c->mo.tile_point_order = point_order;
   c->mo.tile_group_waves = group_waves;
   c->mo.tile_tuned = false;
+  return QMCKL_SUCCESS_DEVICE;
+}
+
+qmckl_exit_code_device qmckl_set_mo_basis_gemm_tile_device(qmckl_context_device context, int32_t gemm_tile) {
+  Context *c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+  if (gemm_tile < QMCKL_GEMM_TILE_AUTO_DEVICE || gemm_tile > QMCKL_GEMM_TILE_LIVE64_DEVICE)
+    return fail(context, QMCKL_INVALID_ARG_2_DEVICE, "qmckl_set_mo_basis_gemm_tile_device",
+                "gemm_tile not one of QMCKL_GEMM_TILE_*");
+  c->mo.tile_gemm = gemm_tile;
+  c->mo.tile_tuned = false;
+  return QMCKL_SUCCESS_DEVICE;
+}
+
+qmckl_exit_code_device qmckl_get_mo_basis_gemm_tile_device(qmckl_context_device context, int32_t *gemm_tile) {
+  Context *c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+  if (gemm_tile == nullptr)
+    return fail(context, QMCKL_INVALID_ARG_2_DEVICE, "qmckl_get_mo_basis_gemm_tile_device", "NULL pointer");
+  *gemm_tile = c->mo.tile_gemm;
   return QMCKL_SUCCESS_DEVICE;
 }
 
@@ -808,8 +844,9 @@ qmckl_exit_code_device qmckl_mo_basis_autotune_device(qmckl_context_device conte
     std::lock_guard<std::mutex> lock(g_tune_mutex);
     auto it = tune_cache().find(key);
     if (it != tune_cache().end()) {
-      mo.tile_point_order = it->second.first;
-      mo.tile_group_waves = it->second.second;
+      mo.tile_point_order = it->second.point_order;
+      mo.tile_group_waves = it->second.group_waves;
+      mo.tile_gemm = it->second.gemm_tile;
       mo.tile_tuned = true;
       return QMCKL_SUCCESS_DEVICE;
     }
@@ -821,36 +858,61 @@ qmckl_exit_code_device qmckl_mo_basis_autotune_device(qmckl_context_device conte
   hipError_t e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
   const bool can_order = round_up(c->ao.ao_num, 64) / 32 <= kMaxFlagBlocks;
+  // the time of two evaluations of the mixed chain with the current choices
+  auto time_chain = [&](float *ms) -> bool {
+    rc = compute_mo_vgl_mixed(context, c, mo.mo_vgl, fn);  // warm-up
+    if (rc != QMCKL_SUCCESS_DEVICE) return false;
+    e = hipEventRecord(e0, c->stream);
+    for (int r = 0; r < 2 && rc == QMCKL_SUCCESS_DEVICE; ++r) rc = compute_mo_vgl_mixed(context, c, mo.mo_vgl, fn);
+    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);
+    return e == hipSuccess && rc == QMCKL_SUCCESS_DEVICE;
+  };
+  // point order x AO wavefronts with the default contraction, then the
+  // contraction kernels with the best of those (the FP32 form has one)
   float best = -1.0f;
-  int32_t best_order = 0, best_waves = 256;
-  for (int32_t order = 0; order <= (can_order ? 1 : 0) && e == hipSuccess && rc == QMCKL_SUCCESS_DEVICE; ++order)
+  int32_t best_order = 0, best_waves = 256, best_gemm = QMCKL_GEMM_TILE_AUTO_DEVICE;
+  const int32_t gemm0 = mo.tile_gemm;
+  mo.tile_gemm = QMCKL_GEMM_TILE_AUTO_DEVICE;
+  bool ok = e == hipSuccess;
+  for (int32_t order = 0; order <= (can_order ? 1 : 0) && ok; ++order)
     for (int32_t waves : {16, 32, 64, 128, 256}) {
       mo.tile_point_order = order;
       mo.tile_group_waves = waves;
-      rc = compute_mo_vgl_mixed(context, c, mo.mo_vgl, fn);  // warm-up
-      if (rc != QMCKL_SUCCESS_DEVICE) break;
-      e = hipEventRecord(e0, c->stream);
-      for (int r = 0; r < 2 && rc == QMCKL_SUCCESS_DEVICE; ++r) rc = compute_mo_vgl_mixed(context, c, mo.mo_vgl, fn);
-      if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
-      if (e == hipSuccess) e = hipEventSynchronize(e1);
       float ms = 0.0f;
-      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-      if (e != hipSuccess || rc != QMCKL_SUCCESS_DEVICE) break;
+      if (!(ok = time_chain(&ms))) break;
       if (best < 0.0f || ms < best) {
         best = ms;
         best_order = order;
         best_waves = waves;
       }
     }
+  if (ok && mo.precision != QMCKL_PRECISION_FP32_DEVICE) {
+    mo.tile_point_order = best_order;
+    mo.tile_group_waves = best_waves;
+    for (int32_t g : {QMCKL_GEMM_TILE_LIVE_DEVICE, QMCKL_GEMM_TILE_LIVE_CACHED_DEVICE, QMCKL_GEMM_TILE_DENSE_DEVICE,
+                      QMCKL_GEMM_TILE_LIVE64_DEVICE}) {
+      mo.tile_gemm = g;
+      float ms = 0.0f;
+      if (!(ok = time_chain(&ms))) break;
+      if (ms < best) {
+        best = ms;
+        best_gemm = g;
+      }
+    }
+  }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
+  if (!ok) mo.tile_gemm = gemm0;
   if (rc != QMCKL_SUCCESS_DEVICE) return rc;
   if (e != hipSuccess) return hip_check(context, e, fn);
   mo.tile_point_order = best_order;
   mo.tile_group_waves = best_waves;
+  mo.tile_gemm = best_gemm;
   mo.tile_tuned = true;
   std::lock_guard<std::mutex> lock(g_tune_mutex);
-  tune_cache()[key] = {best_order, best_waves};
+  tune_cache()[key] = {best_order, best_waves, best_gemm};
   return QMCKL_SUCCESS_DEVICE;
 }
 
@@ -884,13 +946,17 @@ std::vector<uint64_t> batch_signature(const Context *c, const double *out) {
   const AoBasis &ao = c->ao;
   const MoBasis &mo = c->mo;
   const DeviceBasis &b = ao.dev;
+  // entry 6 is the output (the launch reads it back); the kernels' launch
+  // arguments that a new basis or other tile choices change come last
   return {(uint64_t)c->point.num, (uint64_t)c->nucleus.num, (uint64_t)ao.ao_num, (uint64_t)mo.mo_num,
           (uint64_t)mo.precision, (uint64_t)mo.coef_split_form, u(out), u(c->point.coord), u(c->nucleus.coord),
           u(ao.ao_factor), u(ao.ao_vgl), u(ao.ao_vgl_flags), u(ao.ao_f32), u(ao.ao_f32_rowmax),
           u(ao.ao_f32_flags), u(ao.point_perm), u(ao.order_ws), u(b.shells), u(b.chunks), u(b.stage_shell),
           u(b.expo), u(b.coef_norm), u(b.nucl_range), u(b.ao_shell), u(b.ao_abc), u(mo.coef_frag),
           u(mo.coef_split), u(mo.coef_colinv), u(mo.col_plans[0].col_mo), u(mo.col_plans[0].blk),
-          u(mo.col_plans[1].col_mo), u(mo.col_plans[1].blk)};
+          u(mo.col_plans[1].col_mo), u(mo.col_plans[1].blk), (uint64_t)ao.shell_num, (uint64_t)b.nchunks,
+          c->basis_generation, (uint64_t)(uint32_t)mo.tile_point_order, (uint64_t)(uint32_t)mo.tile_group_waves,
+          (uint64_t)(uint32_t)mo.tile_gemm};
 }
 
 }  // namespace

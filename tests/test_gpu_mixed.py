@@ -115,11 +115,11 @@ def test_point_order_invariance(form):
     assert np.array_equal(b, a[perm])
 
 
-@pytest.mark.parametrize("form", ["fp16x2", "bf16x3"])
+@pytest.mark.parametrize("form", ["fp16x2", "bf16x2", "bf16x3"])
 def test_tile_choices_and_autotune_keep_results(form):
     """The tile choices (spatial point order on/off, AO-operand wavefronts
-    per CU) change speed only: every choice, and the autotuner's, gives the
-    same rows bitwise."""
+    per CU, the contraction kernel) change speed only: every choice, and the
+    autotuner's, gives the same rows bitwise."""
     s = S.water64(walk_num=8)
     P = s.point_num
     with Context(0) as c:
@@ -131,11 +131,17 @@ def test_tile_choices_and_autotune_keep_results(form):
                 c.touch()
                 assert np.array_equal(c.mo_vgl(P), ref), (order, waves)
         c.call("qmckl_set_mo_basis_tile_config_device", -1, 0)
+        for gemm in (1, 2, 3, 4):  # LIVE, LIVE_CACHED, DENSE, LIVE64
+            c.call("qmckl_set_mo_basis_gemm_tile_device", gemm)
+            c.touch()
+            assert np.array_equal(c.mo_vgl(P), ref), gemm
+        c.call("qmckl_set_mo_basis_gemm_tile_device", 0)
         c.call("qmckl_mo_basis_autotune_device")
-        po, gw = ctypes.c_int32(-5), ctypes.c_int32(-5)
+        po, gw, gt = ctypes.c_int32(-5), ctypes.c_int32(-5), ctypes.c_int32(-5)
         c.call("qmckl_get_mo_basis_tile_config_device", ctypes.byref(po), ctypes.byref(gw))
-        say(f"autotuned {form}: point_order={po.value} group_waves={gw.value}")
-        assert po.value in (0, 1) and gw.value in (16, 32, 64, 128, 256)
+        c.call("qmckl_get_mo_basis_gemm_tile_device", ctypes.byref(gt))
+        say(f"autotuned {form}: point_order={po.value} group_waves={gw.value} gemm_tile={gt.value}")
+        assert po.value in (0, 1) and gw.value in (16, 32, 64, 128, 256) and gt.value in (0, 1, 2, 3, 4)
         assert np.array_equal(c.mo_vgl(P), ref)
 
 

@@ -342,7 +342,9 @@ qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_device(qmckl_context_device con
                                                         double *mo_vgl, int64_t size_max);
 qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_inplace_device(qmckl_context_device context,
                                                                 double *mo_vgl, int64_t size_max);
-/* mo_value [P][M], size_max >= M*P */
+/* mo_value [P][M], size_max >= M*P.  Always the FP64 contraction of
+ * ao_value, whatever the precision form (in the FP64 form a current mo_vgl's
+ * value rows are copied), so it does not depend on the order of calls. */
 qmckl_exit_code_device qmckl_get_mo_basis_mo_value_device(qmckl_context_device context,
                                                           double *mo_value, int64_t size_max);
 qmckl_exit_code_device qmckl_get_mo_basis_mo_value_inplace_device(qmckl_context_device context,

@@ -263,8 +263,12 @@ qmckl_exit_code_device compute_mo_value(qmckl_context_device context, Context *c
                                         const char *fn) {
   const int64_t P = c->point.num, A = c->ao.ao_num, M = c->mo.mo_num;
   hipError_t e;
-  if (c->mo.mo_vgl_date != 0 && c->mo.mo_vgl_date == c->point.date && c->mo.mo_vgl) {
-    // mo_vgl is current: copy its value rows (src/qmckl_mo.cpp:230-256).
+  // mo_value is always the FP64 value-only contraction, whatever the
+  // precision form, so that it does not depend on whether mo_vgl was asked
+  // for first; in the FP64 form a current mo_vgl holds the same sums, and
+  // its value rows are copied (src/qmckl_mo.cpp:230-256)
+  if (c->mo.precision == QMCKL_PRECISION_FP64_DEVICE && c->mo.mo_vgl_date != 0 &&
+      c->mo.mo_vgl_date == c->point.date && c->mo.mo_vgl) {
     e = launch_strided_copy(c->stream, P, M, c->mo.mo_vgl, 5 * M, dst, M);
   } else {
     qmckl_exit_code_device rc = qmckl_provide_ao_basis_ao_value_device(context);

@@ -129,3 +129,106 @@ def test_autograd_harness():
         mo2 = mo_vgl(R, a, C + 1e-3 * dirC, s)
     lin = float(((wt * (mo2 - mo)).sum() / 1e-3))
     assert abs(lin - float((gC * dirC).sum())) <= 1e-8 * max(1.0, abs(lin))
+
+
+
+def _gpu_forward(sys_, ao=False):
+    with Context(0) as c:
+        c.set_system(sys_)
+        mo = c.mo_vgl(sys_.point_num)
+        return (c.ao_vgl(sys_.point_num), mo) if ao else mo
+
+
+def _fd_of_L(w, sp, sm, step):
+    """(L(sp) - L(sm)) / (2 step) with L = sum w mo_vgl, the two forward
+    passes subtracted element by element first, so that the round-off of
+    summing 32.8 M terms of L does not enter the difference."""
+    return float(np.sum(w * (_gpu_forward(sp) - _gpu_forward(sm)))) / (2.0 * step)
+
+
+def test_adjoint_vs_finite_differences_c5():
+    """The differentiation workload itself (64 waters, 32 walkers, 20,480
+    points, w ~ N(0, 1)): central differences (h = 1e-5) of the GPU's FP64
+    forward pass against the GPU adjoint, each within 1e-6 relative:
+
+    - 3 random directions of all 576 nuclear coordinates and 3 MO
+      coefficients, from differences of mo_vgl;
+    - the most diffuse primitive exponent of 4 f shells and 2 d shells
+      (relative step h alpha), from differences of ao_vgl contracted with
+      (w C)[p][c][k] = sum_i w[p][c][i] C[i][k], since L = sum (w C) ao_vgl.
+      One exponent moves only its shell's 7-10 AOs, a gradient of order 1
+      against sum |w mo_vgl| of order 1e5, and differences of mo_vgl pick
+      up the FP64 contraction's round-off (the sums after the changed AOs
+      round differently): measured 6.5e-7 and 5.9e-6 relative for exponents
+      61 and 2437, against 2.5e-9 and 1.6e-8 from the AO differences.  The
+      mo_vgl-level value is printed for the record.  A primitive with a
+      point on its screen alpha r^2 = -ln(1e-12) within +-h is passed over
+      (exponent 234 here: one point 5.3e-6 from the screen, whose step adds
+      3e-6 to the difference quotient)."""
+    s, w = S.water64_diff(walk_num=32)
+    gR, gA, gC = gpu_adjoint(s, w)
+    h = 1e-5
+    rng = np.random.default_rng(21)
+    worst = 0.0
+    for k in range(3):
+        u = rng.normal(size=s.nucl_coord.shape)
+        u /= np.linalg.norm(u)
+        sp = dataclasses.replace(s, nucl_coord=np.ascontiguousarray(s.nucl_coord + h * u))
+        sm = dataclasses.replace(s, nucl_coord=np.ascontiguousarray(s.nucl_coord - h * u))
+        fd = _fd_of_L(w, sp, sm, h)
+        ad = float(np.sum(gR.T * u))
+        e = abs(fd - ad) / abs(ad)
+        say(f"nuclear direction {k}: fd {fd:.12g} adjoint {ad:.12g} rel {e:.2e}")
+        assert e <= 1e-6, (k, fd, ad)
+        worst = max(worst, e)
+    for (i, k) in ((0, 0), (17, 1000), (319, 4159)):
+        cp, cm = s.mo_coef.copy(), s.mo_coef.copy()
+        cp[i, k] += h
+        cm[i, k] -= h
+        fd = _fd_of_L(w, dataclasses.replace(s, mo_coef=cp), dataclasses.replace(s, mo_coef=cm), h)
+        e = abs(fd - gC[i, k]) / abs(gC[i, k])
+        say(f"coefficient ({i}, {k}): fd {fd:.12g} adjoint {gC[i, k]:.12g} rel {e:.2e}")
+        assert e <= 1e-6, (i, k, fd, gC[i, k])
+        worst = max(worst, e)
+    # The primitive screen alpha r^2 > -ln(1e-12) makes L a step function of
+    # alpha where a point sits on it; a primitive is taken only if no point
+    # crosses the screen between alpha (1 - h) and alpha (1 + h).
+    pts = np.asarray(s.elec_coord).reshape(-1, 3)
+    nucleus_of_shell = np.repeat(np.arange(s.nucl_num), s.nucleus_shell_num)
+    cut = -np.log(1e-12)
+    prims = []
+    for l, n in ((3, 4), (2, 2)):
+        shells = [sh for sh in range(s.shell_num) if s.shell_ang_mom[sh] == l]
+        taken = 0
+        for sh in shells[::7]:
+            j = int(s.shell_prim_index[sh] + s.shell_prim_num[sh] - 1)
+            r2 = np.sum((pts - s.nucl_coord[:, nucleus_of_shell[sh]]) ** 2, axis=1)
+            x = s.exponent[j] * r2
+            crossing = int(np.sum((x * (1 - h) <= cut) & (x * (1 + h) > cut)))
+            if crossing:
+                say(f"exponent {j} skipped: {crossing} point(s) cross the primitive screen within +-h")
+                continue
+            prims.append(j)
+            taken += 1
+            if taken == n:
+                break
+    wC = np.matmul(w, s.mo_coef)  # [P][5][A]
+    for j in prims:
+        hj = h * s.exponent[j]
+        ep, em = s.exponent.copy(), s.exponent.copy()
+        ep[j] += hj
+        em[j] -= hj
+        aop, mop = _gpu_forward(dataclasses.replace(s, exponent=ep), ao=True)
+        aom, mom = _gpu_forward(dataclasses.replace(s, exponent=em), ao=True)
+        dao = aop - aom
+        moved = np.nonzero(np.any(dao != 0, axis=(0, 1)))[0]
+        # only the AOs of the shell that owns primitive j may move
+        assert 0 < len(moved) <= 10, moved
+        fd = float(np.sum(dao[:, :, moved] * wC[:, :, moved])) / (2.0 * hj)
+        fd_mo = float(np.sum(w * (mop - mom))) / (2.0 * hj)
+        e = abs(fd - gA[j]) / abs(gA[j])
+        say(f"exponent {j} (alpha {s.exponent[j]:.4g}): fd {fd:.12g} adjoint {gA[j]:.12g} rel {e:.2e}"
+            f" (mo_vgl differences {abs(fd_mo - gA[j]) / abs(gA[j]):.2e})")
+        assert e <= 1e-6, (j, fd, gA[j])
+        worst = max(worst, e)
+    say(f"water64_diff finite differences: worst relative difference {worst:.2e}")

@@ -139,6 +139,10 @@ def test_batch_graph_refuses_stale_launch(molecules):
         assert g.launch() == 101          # QMCKL_FAILURE_DEVICE
         ctxs[1].call("qmckl_set_mo_basis_precision_device", 0)
         assert g.launch() == 0
+        # a finalize rebuilds the device tables (they may land at the old
+        # addresses with other sizes): refused, whatever the addresses
+        ctxs[2].call("qmckl_finalize_ao_basis_device")
+        assert g.launch() == 101
         assert lib_rc_launch(987654321) == 1   # unknown handle: ARG_1
     finally:
         if g is not None:

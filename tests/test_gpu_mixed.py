@@ -249,3 +249,26 @@ def test_planned_columns_ragged_mo_counts(oracle_lib, mo_num):
             e = row_err(run_form(c, form, P), ref)
             say(f"water8 {mo_num} MOs {form}: {e}")
             assert max(e.values()) <= BAR[form], (form, e)
+
+
+@pytest.mark.parametrize("form", ["fp16x2", "bf16x3"])
+def test_mo_value_independent_of_call_order(oracle_lib, form):
+    """In a mixed-precision context mo_value is the FP64 value-only
+    contraction whether or not mo_vgl (in the mixed form) was computed for
+    the same points first: bitwise the same both ways, and within 1e-12 of
+    the oracle's value rows (8 waters, 4 walkers)."""
+    s = S.water_cluster(n_side=2, walk_num=4, name="water8")
+    P = s.point_num
+    ref = oracle_lib.mo_vgl(s)[:, 0, :]
+    with Context(0) as c:
+        c.set_system(s)
+        c.call("qmckl_set_mo_basis_precision_device", FORMS[form])
+        c.touch()
+        first = c.mo_value(P)            # mo_vgl not current
+        c.touch()
+        c.mo_vgl(P)                      # mixed-form mo_vgl current ...
+        second = c.mo_value(P)           # ... and mo_value asked for after it
+    assert np.array_equal(first, second)
+    ev = np.max(np.abs(first - ref) / np.maximum(np.abs(ref).max(axis=1, keepdims=True), 1e-300))
+    say(f"{form} context mo_value vs oracle: {ev:.2e}")
+    assert ev <= 1e-12

@@ -223,8 +223,14 @@ def test_mo_magnitude_bounds_value(oracle_lib, chbrclf):
     assert np.all(np.abs(mo) <= mag * (1 + 1e-12) + 1e-300)
 
 
-def test_adjoint_against_finite_differences(oracle_lib):
-    s = S.n2_like(walk_num=2)
+@pytest.mark.parametrize("system", ["n2_like", "lmax3", "lmax4"])
+def test_adjoint_against_finite_differences(oracle_lib, system):
+    """Central differences (h = 1e-5) of L = sum w mo_vgl against the
+    oracle's adjoint, for every nuclear coordinate, every third primitive
+    exponent and (exactly, L being linear in them) the MO coefficients.  The
+    l = 3 and l = 4 systems exercise the third derivatives of the monomials
+    that the gradient of the Laplacian needs (identically zero for l <= 2)."""
+    s = S.n2_like(walk_num=2) if system == "n2_like" else _fd_system(lmax=int(system[-1]))
     rng = np.random.default_rng(11)
     w = rng.normal(size=(s.point_num, 5, s.mo_num))
 

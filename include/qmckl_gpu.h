@@ -187,6 +187,15 @@ qmckl_exit_code_device qmckl_compute_en_distance_device(qmckl_context_device con
                                                         const double *elec_coord,
                                                         const double *nucl_coord,
                                                         double *en_distance);
+/* include/qmckl_electron.hpp:37-40 (src/qmckl_electron.cpp:373-553).
+ * C[i + j*ldc] = |a_i - b_j|, i < m, j < n, on device arrays; a_i is
+ * (A[3i], A[3i+1], A[3i+2]) for transa 'N' (lda >= 3) and (A[i], A[i+lda],
+ * A[i+2*lda]) for 'T' (lda >= m); b_j likewise from B, transb, n, ldb.
+ * Codes: m <= 0 ARG_4, n <= 0 ARG_5, flag not N/n/T/t ARG_1, lda ARG_7,
+ * ldb ARG_9, ldc < m ARG_11, NULL A/B/C ARG_6/8/10. */
+qmckl_exit_code_device qmckl_distance_device(qmckl_context_device context, char transa, char transb,
+                                             int64_t m, int64_t n, const double *A, int64_t lda,
+                                             const double *B, int64_t ldb, double *C, int64_t ldc);
 /* New getter (the reference has none): en_distance [P][N], size_max >= P*N. */
 qmckl_exit_code_device qmckl_get_electron_en_distance_device(qmckl_context_device context,
                                                              double *en_distance,

@@ -355,6 +355,8 @@ hipError_t launch_transpose(hipStream_t s, int64_t rows, int64_t cols, const dou
 int64_t point_order_workspace_bytes(int64_t P);
 hipError_t launch_point_order(hipStream_t s, int64_t P, const double *coord, int64_t N, const double *nucl,
                               void *workspace, int32_t *perm);
+hipError_t launch_distance(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, const double *A, int64_t lda,
+                           const double *B, int64_t ldb, double *C, int64_t ldc);
 hipError_t launch_ee_distance(hipStream_t s, int64_t E, int64_t W, const double *coord, double *out);
 hipError_t launch_en_distance(hipStream_t s, int64_t P, int64_t N, const double *coord,
                               const double *nucl, double *out);

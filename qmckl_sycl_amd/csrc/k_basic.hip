@@ -68,36 +68,77 @@ __global__ void __launch_bounds__(kThreads) strided_copy_kernel(int64_t rows, in
 
 constexpr int kMaxNuclLds = 2048;  // nuclei staged in LDS (at most 48 KiB)
 
-// One workgroup per kEnPoints consecutive points: it stages the nuclei in
-// LDS once and writes its points' rows, a contiguous run of the output,
-// thread t taking elements t, t + 256, ... of the run (the (point, nucleus)
-// pair advanced incrementally, no division in the loop).
-constexpr int kEnPoints = 64;
+// One workgroup per kEnPoints consecutive points: it stages its points and
+// the nuclei in LDS and writes its points' rows, one contiguous run of the
+// output.  With an even nucleus count and a 16-byte aligned output each
+// thread writes pairs of neighbouring elements (same point, nuclei n and
+// n + 1) as one nontemporal 16-byte store, thread t taking pairs t,
+// t + 256, ...; the (point, nucleus) position advances by a constant step
+// with one wrap test, no division in the loop.  Otherwise single elements.
+constexpr int kEnPoints = 128;
+template <int V>  // elements per store: 2 or 1
 __global__ void __launch_bounds__(kThreads) en_distance_kernel(int64_t P, int64_t N,
                                                                const double *__restrict__ coord,
                                                                const double *__restrict__ nucl,
                                                                double *__restrict__ out) {
-  extern __shared__ double snucl[];  // [3][N] when N <= kMaxNuclLds
+  extern __shared__ double sm[];  // [3][kEnPoints] points, then [3][N] nuclei when N <= kMaxNuclLds
+  double *sp = sm;
   const bool staged = N <= kMaxNuclLds;
-  if (staged) {
-    for (int i = threadIdx.x; i < 3 * N; i += blockDim.x) snucl[i] = nucl[i];
-    __syncthreads();
-  }
-  const double *R = staged ? snucl : nucl;
   const int64_t pb = (int64_t)blockIdx.x * kEnPoints;
-  const int64_t pe = pb + kEnPoints < P ? pb + kEnPoints : P;
-  const int64_t n_end = (pe - pb) * N;  // elements of this workgroup's run
-  int64_t p = pb + threadIdx.x / N, n = threadIdx.x % N;
-  for (int64_t i = threadIdx.x; i < n_end; i += kThreads) {
-    const double x = coord[p] - R[n];
-    const double y = coord[P + p] - R[N + n];
-    const double z = coord[2 * P + p] - R[2 * N + n];
-    out[pb * N + i] = sqrt(fma(x, x, fma(y, y, z * z)));
-    n += kThreads;
-    while (n >= N) {
-      n -= N;
-      ++p;
+  const int np = (int)(P - pb < kEnPoints ? P - pb : kEnPoints);
+  for (int i = threadIdx.x; i < 3 * kEnPoints; i += kThreads) {
+    const int d = i / kEnPoints, j = i - d * kEnPoints;
+    sp[i] = j < np ? coord[d * P + pb + j] : 0.0;
+  }
+  if (staged)
+    for (int i = threadIdx.x; i < 3 * N; i += kThreads) sm[3 * kEnPoints + i] = nucl[i];
+  __syncthreads();
+  const double *R = staged ? sm + 3 * kEnPoints : nucl;
+  const int64_t units = (int64_t)np * N / V;  // stores of this workgroup
+  constexpr int kStep = V * kThreads;         // elements between a thread's stores
+  const int step_p = (int)(kStep / N), step_n = (int)(kStep - (int64_t)step_p * N);
+  int pl = (int)((int64_t)V * threadIdx.x / N);
+  int n = (int)((int64_t)V * threadIdx.x - (int64_t)pl * N);
+  double *o = out + pb * N;
+  for (int64_t u = threadIdx.x; u < units; u += kThreads) {
+    const double px = sp[pl], py = sp[kEnPoints + pl], pz = sp[2 * kEnPoints + pl];
+    if constexpr (V == 2) {
+      typedef double d2 __attribute__((ext_vector_type(2)));
+      const double x0 = px - R[n], y0 = py - R[N + n], z0 = pz - R[2 * N + n];
+      const double x1 = px - R[n + 1], y1 = py - R[N + n + 1], z1 = pz - R[2 * N + n + 1];
+      const d2 v = {sqrt(fma(x0, x0, fma(y0, y0, z0 * z0))), sqrt(fma(x1, x1, fma(y1, y1, z1 * z1)))};
+      __builtin_nontemporal_store(v, reinterpret_cast<d2 *>(o + 2 * u));
+    } else {
+      const double x = px - R[n], y = py - R[N + n], z = pz - R[2 * N + n];
+      __builtin_nontemporal_store(sqrt(fma(x, x, fma(y, y, z * z))), o + u);
     }
+    pl += step_p;
+    n += step_n;
+    if (n >= N) {
+      n -= (int)N;
+      ++pl;
+    }
+  }
+}
+
+// C[i + j ldc] = |a_i - b_j| for i < m, j < n (src/qmckl_electron.cpp:373-553,
+// qmckl_distance_device), where a_i = (A[3i], A[3i+1], A[3i+2]) for 'N' and
+// (A[i], A[i+lda], A[i+2lda]) for 'T', b_j likewise from B and ldb.
+// Threads run along i (the contiguous index of C); each workgroup covers 256
+// values of i and walks the columns j from blockIdx.y in steps of gridDim.y.
+template <bool TA, bool TB>
+__global__ void __launch_bounds__(kThreads) distance_kernel(int64_t m, int64_t n, const double *__restrict__ A,
+                                                            int64_t lda, const double *__restrict__ B, int64_t ldb,
+                                                            double *__restrict__ C, int64_t ldc) {
+  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= m) return;
+  const double ax = TA ? A[i] : A[i * lda], ay = TA ? A[i + lda] : A[i * lda + 1],
+               az = TA ? A[i + 2 * lda] : A[i * lda + 2];
+  for (int64_t j = blockIdx.y; j < n; j += gridDim.y) {
+    const double x = ax - (TB ? B[j] : B[j * ldb]);
+    const double y = ay - (TB ? B[j + ldb] : B[j * ldb + 1]);
+    const double z = az - (TB ? B[j + 2 * ldb] : B[j * ldb + 2]);
+    C[i + j * ldc] = sqrt(fma(x, x, fma(y, y, z * z)));
   }
 }
 
@@ -285,11 +326,36 @@ hipError_t launch_strided_copy(hipStream_t s, int64_t rows, int64_t cols, const 
 hipError_t launch_en_distance(hipStream_t s, int64_t P, int64_t N, const double *coord,
                               const double *nucl, double *out) {
   if (P <= 0 || N <= 0) return hipSuccess;
-  const size_t lds = N <= kMaxNuclLds ? sizeof(double) * 3 * (size_t)N : 0;
+  if (N > 0x7fffffff / kEnPoints) return hipErrorInvalidValue;  // per-workgroup element counts in int
+  const size_t lds = sizeof(double) * (3 * kEnPoints + (N <= kMaxNuclLds ? 3 * (size_t)N : 0));
   const int64_t blocks = (P + kEnPoints - 1) / kEnPoints;
   if (blocks > 0x7fffffff) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(en_distance_kernel, dim3((unsigned)blocks), dim3(kThreads), lds, s, P, N, coord, nucl,
-                     out);
+  if (N % 2 == 0 && reinterpret_cast<uintptr_t>(out) % 16 == 0)
+    hipLaunchKernelGGL(en_distance_kernel<2>, dim3((unsigned)blocks), dim3(kThreads), lds, s, P, N, coord, nucl,
+                       out);
+  else
+    hipLaunchKernelGGL(en_distance_kernel<1>, dim3((unsigned)blocks), dim3(kThreads), lds, s, P, N, coord, nucl,
+                       out);
+  return hipGetLastError();
+}
+
+hipError_t launch_distance(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, const double *A, int64_t lda,
+                           const double *B, int64_t ldb, double *C, int64_t ldc) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  const int64_t bx = (m + kThreads - 1) / kThreads;
+  if (bx > 0x7fffffff) return hipErrorInvalidValue;
+  // enough workgroups to fill the chip, each walking several columns
+  const int64_t want = (4096 + bx - 1) / bx;
+  const unsigned by = (unsigned)(n < want ? n : (want > 65535 ? 65535 : want));
+  const dim3 grid((unsigned)bx, by);
+  if (ta && tb)
+    hipLaunchKernelGGL((distance_kernel<true, true>), grid, dim3(kThreads), 0, s, m, n, A, lda, B, ldb, C, ldc);
+  else if (ta)
+    hipLaunchKernelGGL((distance_kernel<true, false>), grid, dim3(kThreads), 0, s, m, n, A, lda, B, ldb, C, ldc);
+  else if (tb)
+    hipLaunchKernelGGL((distance_kernel<false, true>), grid, dim3(kThreads), 0, s, m, n, A, lda, B, ldb, C, ldc);
+  else
+    hipLaunchKernelGGL((distance_kernel<false, false>), grid, dim3(kThreads), 0, s, m, n, A, lda, B, ldb, C, ldc);
   return hipGetLastError();
 }
 

@@ -300,6 +300,40 @@ qmckl_exit_code_device qmckl_compute_en_distance_device(qmckl_context_device con
   return hip_check(context, e, fn);
 }
 
+// General distance launcher: C[i + j ldc] = |a_i - b_j|, a_i the i-th point of
+// A ('N': A[3i..3i+2], lda >= 3; 'T': A[i], A[i+lda], A[i+2lda], lda >= m), b_j
+// likewise from B with n and ldb (src/qmckl_electron.cpp:373-553).  Exit codes
+// as the reference's: m <= 0 ARG_4, n <= 0 ARG_5, a transposition flag other
+// than N/n/T/t ARG_1, lda ARG_7, ldb ARG_9, ldc < m ARG_11.  Each leading
+// dimension is checked against its own operand's flag; the reference's checks
+// at 430-470 also test lda against transb and ldb against transa, so every
+// call it accepts is accepted here.  Its kernel takes the square root of a
+// whole column from each work item while others still write it; the result
+// here is the distance that kernel is meant to produce (and what its callers,
+// en_distance at 326 and ee_distance at 361, use).
+qmckl_exit_code_device qmckl_distance_device(qmckl_context_device context, char transa, char transb, int64_t m,
+                                             int64_t n, const double *A, int64_t lda, const double *B, int64_t ldb,
+                                             double *C, int64_t ldc) {
+  Context *c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
+  const char *fn = "qmckl_distance_device";
+  if (m <= 0) return QMCKL_INVALID_ARG_4_DEVICE;
+  if (n <= 0) return QMCKL_INVALID_ARG_5_DEVICE;
+  const bool na = transa == 'N' || transa == 'n', ta = transa == 'T' || transa == 't';
+  const bool nb = transb == 'N' || transb == 'n', tb = transb == 'T' || transb == 't';
+  if (!(na || ta) || !(nb || tb)) return QMCKL_INVALID_ARG_1_DEVICE;
+  if (lda < (ta ? m : 3)) return QMCKL_INVALID_ARG_7_DEVICE;
+  if (ldb < (tb ? n : 3)) return QMCKL_INVALID_ARG_9_DEVICE;
+  if (ldc < m) return QMCKL_INVALID_ARG_11_DEVICE;
+  if (!A) return fail(context, QMCKL_INVALID_ARG_6_DEVICE, fn, "NULL pointer");
+  if (!B) return fail(context, QMCKL_INVALID_ARG_8_DEVICE, fn, "NULL pointer");
+  if (!C) return fail(context, QMCKL_INVALID_ARG_10_DEVICE, fn, "NULL pointer");
+  if (!has_device(c)) return fail(context, QMCKL_FAILURE_DEVICE, fn, "no device bound");
+  hipError_t e = launch_distance(c->stream, ta, tb, m, n, A, lda, B, ldb, C, ldc);
+  if (e == hipSuccess) e = sync(c);
+  return hip_check(context, e, fn);
+}
+
 qmckl_exit_code_device qmckl_provide_en_distance_device(qmckl_context_device context) {
   Context *c = ctx_of(context);
   if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;

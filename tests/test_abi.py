@@ -122,6 +122,26 @@ def test_electron_coord_checks(ctx):
     assert ctx.rc("qmckl_get_electron_coord_device", b"N", 1234, 0) == 4
 
 
+def test_distance_checks(ctx):
+    """qmckl_distance_device's codes (src/qmckl_electron.cpp:373-470)."""
+    D = "qmckl_distance_device"
+    assert ctx.rc(D, b"T", b"T", 0, 5, 1234, 4, 1234, 5, 1234, 4) == 4
+    assert ctx.rc(D, b"T", b"T", 4, 0, 1234, 4, 1234, 5, 1234, 4) == 5
+    assert ctx.rc(D, b"X", b"T", 4, 5, 1234, 4, 1234, 5, 1234, 4) == 1
+    assert ctx.rc(D, b"T", b"y", 4, 5, 1234, 4, 1234, 5, 1234, 4) == 1
+    assert ctx.rc(D, b"T", b"T", 4, 5, 1234, 3, 1234, 5, 1234, 4) == 7    # 'T': lda >= m
+    assert ctx.rc(D, b"N", b"T", 4, 5, 1234, 2, 1234, 5, 1234, 4) == 7    # 'N': lda >= 3
+    assert ctx.rc(D, b"T", b"T", 4, 5, 1234, 4, 1234, 4, 1234, 4) == 9    # 'T': ldb >= n
+    assert ctx.rc(D, b"T", b"n", 4, 5, 1234, 4, 1234, 2, 1234, 4) == 9    # 'N': ldb >= 3
+    assert ctx.rc(D, b"T", b"T", 4, 5, 1234, 4, 1234, 5, 1234, 3) == 11
+    assert ctx.rc(D, b"T", b"T", 4, 5, None, 4, 1234, 5, 1234, 4) == 6
+    assert ctx.rc(D, b"T", b"T", 4, 5, 1234, 4, None, 5, 1234, 4) == 8
+    assert ctx.rc(D, b"T", b"T", 4, 5, 1234, 4, 1234, 5, None, 4) == 10
+    # 'N' for A with lda = 3 and m > 3, 'T' for B: accepted (a leading
+    # dimension is checked against its own flag only); no device -> FAILURE
+    assert ctx.rc(D, b"n", b"t", 40, 5, 1234, 3, 1234, 5, 1234, 40) == FAILURE
+
+
 def test_point_checks(ctx):
     assert ctx.rc("qmckl_set_point_device", b"N", 10, 1234, 29) == 4
     assert ctx.rc("qmckl_set_point_device", b"Z", 10, 1234, 30) == 2

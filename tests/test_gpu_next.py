@@ -52,6 +52,46 @@ def test_ee_distance_follows_new_coordinates(oracle_lib):
     assert np.allclose(got, ref, rtol=1e-15, atol=1e-15)
 
 
+@pytest.mark.parametrize("ta,tb", [(b"T", b"T"), (b"N", b"T"), (b"T", b"N"), (b"n", b"n")])
+def test_distance_launcher(oracle_lib, ta, tb):
+    """qmckl_distance_device in its four layouts, with leading dimensions
+    larger than needed (the padding holds NaN): with the nuclei as A and the
+    points as B, C[p][n] is the oracle's en_distance [P][N] in every layout,
+    as the reference's compute_en_distance obtains it with 'T','T'
+    (src/qmckl_electron.cpp:326-328); the padding columns of C stay
+    untouched."""
+    s = S.chbrclf()
+    P, N = s.point_num, s.nucl_num
+    pts = s.coord_T()              # [3][P]
+    nuc = np.asarray(s.nucl_coord)  # [3][N]
+    ref = oracle_lib.en_distance(s)  # [P][N]
+
+    def layout(x, t, ld):  # x [3][k] -> 'T' [3][ld] or 'N' [k][ld]
+        k = x.shape[1]
+        if t in (b"T", b"t"):
+            out = np.full((3, ld), np.nan)
+            out[:, :k] = x
+        else:
+            out = np.full((k, ld), np.nan)
+            out[:, :3] = x.T
+        return np.ascontiguousarray(out)
+
+    lda = N + 3 if ta in (b"T", b"t") else 5
+    ldb = P + 7 if tb in (b"T", b"t") else 4
+    ldc = N + 2
+    with Context(0) as c:
+        A = DeviceArray.from_numpy(c, layout(nuc, ta, lda))
+        B = DeviceArray.from_numpy(c, layout(pts, tb, ldb))
+        C = DeviceArray.from_numpy(c, np.full((P, ldc), -1.0))
+        c.check(lib().qmckl_distance_device(c.handle, ta, tb, N, P, A.ptr, lda, B.ptr, ldb, C.ptr, ldc))
+        got = C.numpy()
+    assert np.all(got[:, N:] == -1.0)  # padding of C untouched
+    d = got[:, :N]
+    rel = np.max(np.abs(d - ref) / np.maximum(ref, 1e-300))
+    say(f"distance {ta.decode()}{tb.decode()}: max rel diff vs oracle {rel:.2e}")
+    assert rel <= 2.3e-16
+
+
 @pytest.mark.parametrize("form", [0, 2])
 def test_select_mo(form):
     s = S.chbrclf()

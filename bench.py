@@ -179,6 +179,7 @@ def bench_batch(forms, steps, warmup, scale=1.0):
                 res[form]["autotuned_graph"] = {"ms_per_step": sec * 1e3, "value": points / sec}
                 for c in ctxs:  # back to the fixed tiles for any later form
                     c.check(L.qmckl_set_mo_basis_tile_config_device(c.handle, -1, 0))
+            res[form]["mo_error_vs_fp64_oracle"] = batch_errors(mols, outs, form)
             log("batch", form, json.dumps(res[form]))
     finally:
         for c in ctxs:
@@ -186,13 +187,31 @@ def bench_batch(forms, steps, warmup, scale=1.0):
     return res
 
 
+def batch_errors(mols, outs, form, n_pts=64):
+    """The batch's outputs against the FP64 oracle on 64 sampled points of
+    eight molecules (the two largest by P A M and six spread over the
+    batch); the worst per-component error over them."""
+    by_work = sorted(range(len(mols)), key=lambda i: mols[i].point_num * mols[i].ao_num * mols[i].mo_num)
+    picks = sorted(set(by_work[-2:]) | set(np.linspace(0, len(mols) - 1, 6).astype(int).tolist()))
+    worst = {"value": 0.0, "gradient": 0.0, "laplacian": 0.0}
+    for i in picks:
+        m = mols[i]
+        idx = sample_points(m, n_pts, seed=i)
+        e = errors_vs_oracle(m, {form: outs[i].numpy()[idx]}, idx)[form]
+        for k in worst:
+            worst[k] = max(worst[k], e[k])
+    return {"molecules_checked": [int(i) for i in picks], "points_per_molecule": n_pts, **worst}
+
+
 # device kernel name (as the profiler lists it) of each timed kernel
 # (the FP64 ao_vgl kernel depends on the basis: lane-per-point for 1024 AOs
 # or more in whole 64-byte rows, lane-per-AO otherwise)
 KERNEL_SYMBOL = {"en_distance": "en_distance_kernel", "ao_vgl": ("ao_vgl_f64_kernel", "ao_kernel<5, false, double>"),
                  "ao_vgl_f32": "ao_rows_kernel", "mo_gemm_f64": "gemm_f64_frag_kernel<",
-                 "mo_gemm_fp32": "gemm_f32_kernel<", "mo_gemm_fp16x2": "gemm_split_kernel<2,",
-                 "mo_gemm_bf16x2": "gemm_split_kernel<3,", "mo_gemm_bf16x3": "gemm_split_kernel<4,"}
+                 "mo_gemm_fp32": "gemm_f32_kernel<",
+                 "mo_gemm_fp16x2": ("gemm_split_kernel<2,", "gemm_split_live_kernel<2,"),
+                 "mo_gemm_bf16x2": ("gemm_split_kernel<3,", "gemm_split_live_kernel<3,"),
+                 "mo_gemm_bf16x3": ("gemm_split_kernel<4,", "gemm_split_live_kernel<4,")}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_{workload}.json")
 
 
@@ -209,6 +228,16 @@ def measured(s, kernel, key="bytes_per_launch"):
         return None
     k = rec.get("kernels", {}).get(kernel)
     return None if k is None else k.get(key)
+
+
+def measured_adjoint(kernel):
+    """The adjoint kernels' record in the committed counter measurement."""
+    path = TRAFFIC_FILE.format(workload=ADJOINT_WORKLOAD)
+    try:
+        with open(path) as f:
+            return json.load(f).get("kernels", {}).get(kernel)
+    except (OSError, ValueError):
+        return None
 
 
 def bench_value_only(steps, warmup):
@@ -260,10 +289,17 @@ def bench_value_only(steps, warmup):
         ctx.destroy()
 
 
-def bench_adjoint(steps, warmup):
+ADJOINT_WORKLOAD = "water64_diff_32w"
+VALU_PEAK = 256 * 4 * 2.4e9 / 4  # wavefront VALU instructions/s: 1024 SIMDs, 4 cycles each at 2.4 GHz
+
+
+def bench_adjoint(steps, warmup, forward=True):
     """Differentiation form at C5 (water64, 32 walkers, weights w ~ N(0,1)
     shaped like mo_vgl): the forward FP64 mo_vgl then the adjoint (dL/dR,
-    dL/dalpha, dL/dC of L = sum w mo_vgl), timed per step."""
+    dL/dalpha, dL/dC of L = sum w mo_vgl), timed per step.  forward=False
+    runs the adjoint alone (the counter runs use it to separate the
+    adjoint's G GEMM from the forward contraction, which is the same
+    kernel)."""
     import torch
     from qmckl_sycl_amd import systems
     from qmckl_sycl_amd.context import Context, DeviceArray, lib
@@ -280,7 +316,8 @@ def bench_adjoint(steps, warmup):
         h = ctx.handle
 
         def step():
-            ctx.check(L.qmckl_get_mo_basis_mo_vgl_inplace_device(h, mo.ptr, mo.size))
+            if forward:
+                ctx.check(L.qmckl_get_mo_basis_mo_vgl_inplace_device(h, mo.ptr, mo.size))
             ctx.check(L.qmckl_get_mo_basis_mo_vgl_adjoint_device(h, wd.ptr, wd.size, dR.ptr, dA.ptr, dC.ptr))
         for _ in range(warmup):
             step()
@@ -290,7 +327,8 @@ def bench_adjoint(steps, warmup):
             step()
         torch.cuda.synchronize()
         sec = (time.perf_counter() - t0) / steps
-        res = {"workload": f"water64 at 32 walkers ({s.point_num} points), FP64 forward + adjoint",
+        res = {"workload": f"water64 at 32 walkers ({s.point_num} points), FP64 "
+                           f"{'forward + ' if forward else ''}adjoint",
                "ms_per_step": sec * 1e3, "value": s.point_num / sec, "unit": "electron-points/s"}
         # per-kernel breakdown from a separate run with the HIP-event timers on
         ctx.check(L.qmckl_reset_kernel_timing_device(h))
@@ -304,6 +342,38 @@ def bench_adjoint(steps, warmup):
             L.qmckl_get_kernel_timing_device(h, name.encode(), ctypes.byref(n), ctypes.byref(ms))
             if n.value:
                 kernels[name] = {"launches": int(n.value), "avg_ms": ms.value / n.value}
+        # Per-kernel rooflines.  The FP64 GEMMs (G = w C over [5P][M] x [M][A],
+        # dL/dC = AO^T w, the forward contraction) on the MFMA flops they
+        # issued, from the committed counter run of the adjoint alone
+        # (they skip zero coefficient blocks and screened AO blocks, so the
+        # dense 10 P A M, also given, can read above the peak); ao_adjoint,
+        # which recomputes the primitives and the product rule per point, on
+        # its VALU issue rate (wavefront VALU instructions per second
+        # against one per 4 cycles on each of the 1,024 SIMDs).  The forward
+        # kernels (ao_vgl, mo_gemm_f64) are the plain form's, whose
+        # rooflines the plain_fp64 record gives.
+        P, A, M = s.point_num, s.ao_num, s.mo_num
+        for name, k in kernels.items():
+            sec_k = k["avg_ms"] / 1e3
+            prof = measured_adjoint(name)
+            if name in ("ao_vgl", "mo_gemm_f64"):
+                continue
+            if name.endswith("gemm_f64"):
+                dense = 10.0 * P * A * M / sec_k / 1e12
+                rec = {"bound": "mfma", "achieved": dense, "peak": 78.6, "unit": "TFLOP/s",
+                       "frac": dense / 78.6, "basis": "dense 10 P A M"}
+                issued = (prof or {}).get("mfma_flops_per_launch")
+                if issued:
+                    rec.update(achieved=issued / sec_k / 1e12, frac=issued / sec_k / 1e12 / 78.6,
+                               basis="issued MFMA flops (live blocks)",
+                               dense={"achieved": dense, "frac": dense / 78.6},
+                               dense_fraction=issued / (10.0 * P * A * M))
+                k["roofline"] = rec
+            elif name == "ao_adjoint" and prof and prof.get("valu_insts_per_launch"):
+                ach = prof["valu_insts_per_launch"] / sec_k
+                k["roofline"] = {"bound": "valu", "achieved": ach, "peak": VALU_PEAK,
+                                 "unit": "wavefront VALU instructions/s", "frac": ach / VALU_PEAK,
+                                 "hbm_bytes_per_launch": prof.get("bytes_per_launch")}
         res["kernels"] = kernels
         log("adjoint", json.dumps(res))
         return res
@@ -311,26 +381,35 @@ def bench_adjoint(steps, warmup):
         ctx.destroy()
 
 
-def roofline(kernels, s):
+def roofline(kernels, s, basis="dense"):
+    """Roofline record of the kernel that takes the most time.  For a GEMM,
+    basis "dense" prices the algorithmic work 10 P A M (logical FMAs counted
+    once) against the pipe's peak; basis "issued" prices the MFMA flops the
+    kernel issued (SQ_INSTS_VALU_MFMA_MOPS_* x 512 from the committed
+    counter run, split products included), its utilisation of the matrix
+    cores.  The kernels skip exact-zero blocks (screened AOs, zero
+    coefficient blocks), so dense work can exceed what the matrix cores run:
+    the FP64 contraction issues 41% of the dense products, and its dense
+    figure would read above 1."""
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
     kind, work = kernel_work(dom, s)
     avg_s = kernels[dom]["avg_ms"] / 1e3
     traffic = measured(s, dom)
     if kind == "mfma":
         prec, peak = GEMM_PEAK[dom]
-        ach = work / avg_s / 1e12
-        rec = {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
-               "frac": ach / peak, "traffic": traffic, "kernel": dom, "peak_dtype": prec,
-               "mfma_products_per_fma": PRODUCTS[dom]}
-        # The kernels skip exact-zero blocks (screened AOs, zero coefficient
-        # blocks), so the dense work above can exceed what the matrix cores
-        # run; the MFMA flops the kernel issued (SQ_INSTS_VALU_MFMA_MOPS_*
-        # x 512, all split products included) give its utilisation.
+        dense = work / avg_s / 1e12
         issued = measured(s, dom, "mfma_flops_per_launch")
+        rec = {"bound": "mfma", "achieved": dense, "peak": peak, "unit": "TFLOP/s",
+               "frac": dense / peak, "traffic": traffic, "kernel": dom, "peak_dtype": prec,
+               "mfma_products_per_fma": PRODUCTS[dom], "basis": "dense 10 P A M"}
         if issued:
             rec["issued"] = {"flops_per_launch": issued, "achieved": issued / avg_s / 1e12,
                              "frac": issued / avg_s / 1e12 / peak,
                              "dense_fraction": issued / (work * PRODUCTS[dom])}
+            if basis == "issued":
+                rec["dense"] = {"flops_per_launch": work, "achieved": dense, "frac": dense / peak}
+                rec.update(achieved=issued / avg_s / 1e12, frac=issued / avg_s / 1e12 / peak,
+                           basis="issued MFMA flops (live blocks x split products)")
         return rec
     ach = work / avg_s / 1e9
     return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -416,14 +495,50 @@ def traffic_profile(args):
         json.dump(out, f, indent=1)
     print(json.dumps(out), flush=True)
 
+    # the adjoint alone (its G GEMM is the forward's kernel, so the forward
+    # stays out of these runs)
+    arun = ["--adjoint-only", "--steps", "2", "--warmup", "0"]
+    aper = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        aper.update(_pmc_per_kernel(counter, arun))
+    aper.update(_pmc_per_kernel(" ".join(mops) + " SQ_INSTS_VALU", arun))
+    per = aper
+    adj = {}
+    for timer, sym in (("adjoint_g_gemm_f64", "gemm_f64_frag_kernel<5, false>"),
+                       ("adjoint_dc_gemm_f64", "gemm_f64_frag_kernel<5, true>"),
+                       ("ao_adjoint", "ao_adjoint_kernel")):
+        rec = {}
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            v = mean_of(counter, sym)
+            if v is not None:
+                rec[counter.lower() + "_bytes"] = v * 1024.0 * factors[counter]
+        if len(rec) == 2:
+            rec["bytes_per_launch"] = rec["fetch_size_bytes"] + rec["write_size_bytes"]
+        f = 512.0 * sum(mean_of(c, sym) or 0.0 for c in mops)
+        if f:
+            rec["mfma_flops_per_launch"] = f
+        v = mean_of("SQ_INSTS_VALU", sym)
+        if v is not None:
+            rec["valu_insts_per_launch"] = v
+        adj[timer] = rec
+    aout = {"workload": ADJOINT_WORKLOAD, "counter_scale": factors, "mfma_counters": list(mops),
+            "command": "python bench.py --traffic (adjoint runs: bench.py " + " ".join(arun) + ")",
+            "kernels": adj}
+    with open(TRAFFIC_FILE.format(workload=ADJOINT_WORKLOAD), "w") as f:
+        json.dump(aout, f, indent=1)
+    print(json.dumps(aout), flush=True)
 
-def errors_vs_oracle(s, mos, n=256, seed=0):
-    """Per-form error on sampled points: for each (point, component) row
-    max_i |d| / max_i |ref|, maximised over rows, for value / gradient /
-    Laplacian."""
-    from oracle import oracle
+
+def sample_points(s, n=256, seed=0):
     rng = np.random.default_rng(seed)
-    idx = np.sort(rng.choice(s.point_num, size=min(n, s.point_num), replace=False))
+    return np.sort(rng.choice(s.point_num, size=min(n, s.point_num), replace=False))
+
+
+def errors_vs_oracle(s, mos, idx):
+    """Per-form error on the sampled points idx (mos[form] holds mo_vgl at
+    those points): for each (point, component) row max_i |d| / max_i |ref|,
+    maximised over rows, for value / gradient / Laplacian."""
+    from oracle import oracle
     coord = np.ascontiguousarray(s.coord_T()[:, idx])
     ref = oracle.mo_vgl_from_ao(s.mo_coef, oracle.ao_vgl(s, coord))
     rm = np.abs(ref).max(axis=2, keepdims=True)
@@ -431,16 +546,43 @@ def errors_vs_oracle(s, mos, n=256, seed=0):
     out = {"points_checked": int(idx.size),
            "measure": "max over (point, component) rows of max_i|d|/max_i|ref|"}
     for form, mo in mos.items():
-        e = (np.abs(mo[idx] - ref) / rm).max(axis=(0, 2))
+        e = (np.abs(mo - ref) / rm).max(axis=(0, 2))
         out[form] = {"value": float(e[0]), "gradient": float(e[1:4].max()),
                      "laplacian": float(e[4])}
     return out
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(s, target_s=10.0):
-    """FP64 oracle (OpenMP over points) on a walker subset of the workload."""
+    """FP64 oracle (OpenMP over points) on a walker subset of the workload,
+    built for this host (-O3 -march=native) into a temporary directory; the
+    portable x86-64-v3 build if that compile fails."""
+    import subprocess
+    import tempfile
     from oracle import oracle
-    cores = os.cpu_count() or 1
+    build = "gcc -O3 -march=native -fopenmp"
+    native = os.path.join(tempfile.mkdtemp(prefix="qmckl_oracle_"), "libqmckl_oracle_native.so")
+    try:
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native", f"NATIVE_OUT={native}"],
+                       check=True, timeout=180, stdout=sys.stderr, stderr=sys.stderr)
+        oracle.use_library(native)
+    except (OSError, subprocess.SubprocessError) as e:
+        log("native oracle build failed, timing the x86-64-v3 build:", e)
+        build = "gcc -O3 -march=x86-64-v3 -fopenmp"
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        cores = os.cpu_count() or 1
     oracle.set_num_threads(cores)
     E = s.elec_num
 
@@ -453,12 +595,20 @@ def cpu_baseline(s, target_s=10.0):
 
     run(1)                       # warm-up
     t1 = run(1)
-    w = int(max(1, min(s.walk_num, 32, round(target_s / max(t1, 1e-6)))))
-    t = run(w) if w > 1 else t1
+    # a walker subset of about a tenth of target_s per evaluation (at most
+    # all walkers), evaluated repeatedly until target_s has passed; the rate
+    # is the median evaluation's
+    w = int(max(1, min(s.walk_num, round(0.1 * target_s / max(t1, 1e-6)))))
+    times = []
+    while not times or (sum(times) < target_s and len(times) < 50):
+        times.append(run(w))
+    t = float(np.median(times))
     pts = w * E
     return {"value": pts / t, "unit": "electron-points/s", "cores": cores, "kind": "port",
+            "cpu_model": cpu_model(), "build": build, "threads": oracle.num_threads(),
             "sample": f"{w} of {s.walk_num} walkers ({pts} points), FP64 oracle "
-                      f"(ao_vgl then per-point mo_vgl loop), {t:.2f} s"}
+                      f"(ao_vgl then per-point mo_vgl loop), median of {len(times)} evaluations "
+                      f"({t:.2f} s each, {sum(times):.1f} s in all)"}
 
 
 def copy_calibration(mib):
@@ -482,12 +632,16 @@ def main():
     ap.add_argument("--walkers", type=int, default=0)
     ap.add_argument("--form", default="fp16x2", choices=list(FORM_IDS))
     ap.add_argument("--no-plain", action="store_true", help="skip the FP64 plain comparison run")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the other mixed precision forms at the same size")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the 48-molecule ragged batch measurement")
     ap.add_argument("--batch-only", action="store_true",
                     help="only run the 48-molecule ragged batch measurement")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only time the adjoint (no forward pass) at the differentiation workload")
     ap.add_argument("--traffic", action="store_true",
                     help="measure HBM bytes per launch of each kernel with rocprofv3 FETCH_SIZE / "
                          "WRITE_SIZE runs (calibrated on a plain copy) and write them under profiles/")
@@ -505,6 +659,9 @@ def main():
     if args.traffic:
         traffic_profile(args)
         return
+    if args.adjoint_only:
+        print(json.dumps({"adjoint_c5": bench_adjoint(args.steps, args.warmup, forward=False)}), flush=True)
+        return
     if args.batch_only:
         print(json.dumps({"batch_48_ragged": bench_batch(["plain", args.form], args.steps, args.warmup)}),
               flush=True)
@@ -516,13 +673,19 @@ def main():
     ctx = Context(0)
     ctx.set_system(s)
     results, mos = {}, {}
+    idx = sample_points(s)
     forms = [args.form] if (args.no_plain or args.form == "plain") else ["plain", args.form]
-    for form in forms:
-        sec, kernels, mo = run_form(ctx, s, form, args.steps, args.warmup)
-        results[form] = {"ms_per_step": sec * 1e3, "value": s.point_num / sec, "kernels": kernels}
-        mos[form] = mo
+    extra = [] if args.no_forms else [f for f in FORM_IDS if f not in forms]
+    for form in forms + extra:
+        steps = args.steps if form in forms else max(3, args.steps // 2)
+        sec, kernels, mo = run_form(ctx, s, form, steps, args.warmup)
+        results[form] = {"ms_per_step": sec * 1e3, "value": s.point_num / sec, "kernels": kernels,
+                         "steps": steps}
+        mos[form] = mo[idx]
+        del mo
         log(form, json.dumps(results[form]))
     ctx.destroy()
+    errs = errors_vs_oracle(s, mos, idx)
     head = results[args.form]
     out = {
         "metric": METRIC, "value": head["value"], "unit": "electron-points/s", "n_gpus": 1,
@@ -535,14 +698,24 @@ def main():
                    "points": s.point_num, "form": args.form},
         "roofline": roofline(head["kernels"], s),
         "kernels": head["kernels"],
-        "mo_error_vs_fp64_oracle": errors_vs_oracle(s, mos),
+        "mo_error_vs_fp64_oracle": {k: v for k, v in errs.items() if k not in extra},
     }
     if "plain" in results and args.form != "plain":
         out["plain_fp64"] = {"value": results["plain"]["value"],
                              "ms_per_step": results["plain"]["ms_per_step"],
                              "kernels": results["plain"]["kernels"],
-                             "roofline": roofline(results["plain"]["kernels"], s)}
+                             "roofline": roofline(results["plain"]["kernels"], s, basis="issued")}
         out["speedup_vs_plain_fp64"] = head["value"] / results["plain"]["value"]
+    # every precision form at the same size, each checked on the same points
+    out["forms"] = {}
+    for form in [f for f in forms + extra if f != "plain"]:
+        r = results[form]
+        rec = {"value": r["value"], "ms_per_step": r["ms_per_step"], "steps": r["steps"],
+               "mo_error_vs_fp64_oracle": errs[form], "roofline": roofline(r["kernels"], s),
+               "kernels": r["kernels"]}
+        if "plain" in results:
+            rec["speedup_vs_plain_fp64"] = r["value"] / results["plain"]["value"]
+        out["forms"][form] = rec
     if not args.no_batch:
         out["batch_48_ragged"] = bench_batch(["plain", args.form], max(2, args.steps // 3), 1)
         out["adjoint_c5"] = bench_adjoint(max(2, args.steps // 3), 1)

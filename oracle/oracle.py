@@ -25,6 +25,15 @@ def build(force=False):
         subprocess.run(["make", "-s", "-B" if force else "all", "-C", _HERE], check=True)
 
 
+def use_library(path):
+    """Loads the oracle from another build of the same source (the
+    -march=native one the CPU baseline times); later calls use it."""
+    global _lib, LIB_PATH
+    LIB_PATH = path
+    _lib = None
+    return lib()
+
+
 def lib():
     global _lib
     if _lib is None:

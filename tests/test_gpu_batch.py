@@ -59,6 +59,43 @@ def test_batch_equals_single_fp64(molecules, oracle_lib):
             c.destroy()
 
 
+@pytest.mark.parametrize("form", [0, 2])
+def test_batch_full_size_against_oracle(oracle_lib, form):
+    """The batch at its full size (48 molecules, 16-512 walkers, 1.52 M
+    points) in the FP64 form and in FP16X2: the two largest molecules by
+    P A M and six spread over the batch checked against the oracle on 64
+    sampled points each (FP64: RTOL_VGL relative to the magnitude sums;
+    FP16X2: 1e-6 of each (point, component) row's maximum), and the largest
+    also bitwise against its own inplace getter."""
+    mols = S.ragged_batch(48)
+    say("full batch points:", sum(m.point_num for m in mols))
+    ctxs, outs = setup(mols, forms=[form])
+    try:
+        batch_mo_vgl(ctxs, outs)
+        by_work = sorted(range(len(mols)), key=lambda i: mols[i].point_num * mols[i].ao_num * mols[i].mo_num)
+        picks = sorted(set(by_work[-2:]) | set(np.linspace(0, len(mols) - 1, 6).astype(int).tolist()))
+        for i in picks:
+            m = mols[i]
+            got = outs[i].numpy()
+            if i == by_work[-1]:
+                assert np.array_equal(got, single(ctxs[i], m)), m.name
+            idx = np.sort(np.random.default_rng(i).choice(m.point_num, size=min(64, m.point_num), replace=False))
+            coord = np.ascontiguousarray(m.coord_T()[:, idx])
+            what = f"full batch form {form} {m.name} P={m.point_num} A={m.ao_num} M={m.mo_num}"
+            if form == 0:
+                ref, mag = oracle_lib.mo_vgl(m, coord_T=coord, with_mag=True)
+                assert_vgl_close(got[idx], ref, mag, what=what)
+            else:
+                ref = oracle_lib.mo_vgl(m, coord_T=coord)
+                rm = np.abs(ref).max(axis=2, keepdims=True)
+                e = float((np.abs(got[idx] - ref) / np.where(rm > 0, rm, 1.0)).max())
+                say(f"{what}: max row-relative error {e:.2e}")
+                assert e <= 1e-6, (what, e)
+    finally:
+        for c in ctxs:
+            c.destroy()
+
+
 def test_batch_mixed_forms_and_moving_points(molecules):
     # the split forms' contractions run as persistent launches over the
     # batch; every molecule must still equal its own single-context result

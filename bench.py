@@ -30,15 +30,15 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "AO+MO VGL electron-points/s: mixed/batched vs FP64 plain, MO rel err ≤1e-6"
-FORM_IDS = {"plain": 0, "fp32": 1, "fp16x2": 2, "bf16x2": 3, "bf16x3": 4}
+FORM_IDS = {"plain": 0, "fp32": 1, "fp16x2": 2, "bf16x2": 3, "bf16x3": 4, "int8x5": 5}
 GEMM_KERNEL = {"plain": "mo_gemm_f64", "fp32": "mo_gemm_fp32", "fp16x2": "mo_gemm_fp16x2",
-               "bf16x2": "mo_gemm_bf16x2", "bf16x3": "mo_gemm_bf16x3"}
+               "bf16x2": "mo_gemm_bf16x2", "bf16x3": "mo_gemm_bf16x3", "int8x5": "mo_gemm_int8x5"}
 # published dense peaks (TFLOP/s) of the matrix-core type each GEMM runs on
 GEMM_PEAK = {"mo_gemm_f64": ("f64", 78.6), "mo_gemm_fp32": ("f32", 157.3),
              "mo_gemm_fp16x2": ("f16", 2500.0), "mo_gemm_bf16x2": ("bf16", 2500.0),
-             "mo_gemm_bf16x3": ("bf16", 2500.0)}
+             "mo_gemm_bf16x3": ("bf16", 2500.0), "mo_gemm_int8x5": ("i8", 5000.0)}
 PRODUCTS = {"mo_gemm_f64": 1, "mo_gemm_fp32": 1, "mo_gemm_fp16x2": 3, "mo_gemm_bf16x2": 3,
-            "mo_gemm_bf16x3": 6}
+            "mo_gemm_bf16x3": 6, "mo_gemm_int8x5": 15}
 
 
 def log(*a):
@@ -66,6 +66,8 @@ def kernel_work(name, s):
         return "hbm", P * (24.0 + 40.0 * A)
     if name == "ao_vgl_f32":
         return "hbm", P * (24.0 + 20.0 * A)
+    if name == "ao_vgl_f32_lo":  # value and residual, the INT8X5 form's operand
+        return "hbm", P * (24.0 + 40.0 * A)
     if name == "en_distance":
         return "hbm", P * (24.0 + 8.0 * N)
     return None, 0.0
@@ -95,7 +97,7 @@ def run_form(ctx, s, form, steps, warmup):
     t1 = time.perf_counter()
     ctx.check(L.qmckl_set_kernel_timing_device(h, False))
     kernels = {}
-    for name in ("en_distance", "point_order", "ao_vgl", "ao_vgl_f32", GEMM_KERNEL[form]):
+    for name in ("en_distance", "point_order", "ao_vgl", "ao_vgl_f32", "ao_vgl_f32_lo", GEMM_KERNEL[form]):
         n = ctypes.c_int64(0)
         ms = ctypes.c_double(0.0)
         L.qmckl_get_kernel_timing_device(h, name.encode(), ctypes.byref(n), ctypes.byref(ms))
@@ -207,11 +209,13 @@ def batch_errors(mols, outs, form, n_pts=64):
 # (the FP64 ao_vgl kernel depends on the basis: lane-per-point for 1024 AOs
 # or more in whole 64-byte rows, lane-per-AO otherwise)
 KERNEL_SYMBOL = {"en_distance": "en_distance_kernel", "ao_vgl": ("ao_vgl_f64_kernel", "ao_kernel<5, false, double>"),
-                 "ao_vgl_f32": "ao_rows_kernel", "mo_gemm_f64": "gemm_f64_frag_kernel<",
+                 "ao_vgl_f32": "ao_rows_kernel<false>", "ao_vgl_f32_lo": "ao_rows_kernel<true>",
+                 "mo_gemm_f64": "gemm_f64_frag_kernel<",
                  "mo_gemm_fp32": "gemm_f32_kernel<",
                  "mo_gemm_fp16x2": ("gemm_split_kernel<2,", "gemm_split_live_kernel<2,"),
                  "mo_gemm_bf16x2": ("gemm_split_kernel<3,", "gemm_split_live_kernel<3,"),
-                 "mo_gemm_bf16x3": ("gemm_split_kernel<4,", "gemm_split_live_kernel<4,")}
+                 "mo_gemm_bf16x3": ("gemm_split_kernel<4,", "gemm_split_live_kernel<4,"),
+                 "mo_gemm_int8x5": "gemm_i8_kernel<"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_{workload}.json")
 
 
@@ -460,7 +464,7 @@ def traffic_profile(args):
         factors[counter] = copy_bytes / (copy_kib * 1024.0)
         per.update(_pmc_per_kernel(counter, run))
     mops = ("SQ_INSTS_VALU_MFMA_MOPS_F64", "SQ_INSTS_VALU_MFMA_MOPS_F16", "SQ_INSTS_VALU_MFMA_MOPS_BF16",
-            "SQ_INSTS_VALU_MFMA_MOPS_F32")
+            "SQ_INSTS_VALU_MFMA_MOPS_F32", "SQ_INSTS_VALU_MFMA_MOPS_I8")
     per.update(_pmc_per_kernel(" ".join(mops), run))
 
     def mean_of(counter, sym):

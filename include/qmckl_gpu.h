@@ -379,7 +379,15 @@ qmckl_exit_code_device qmckl_compute_mo_basis_mo_value_device(qmckl_context_devi
  *           hi*hi + hi*lo + lo*hi;
  *   BF16X2  bf16 hi+lo split, the same three products;
  *   BF16X3  bf16 hi+mid+lo split, six products;
- * all accumulating in FP32 over short K blocks and in FP64 across them.
+ * all accumulating in FP32 over short K blocks and in FP64 across them;
+ *   INT8X5  the accurate form: the AO operand kept as FP32 value plus FP32
+ *           residual, both operands cut into five int8 fixed-point slices
+ *           (per (AO row, 64-AO block) and per MO power-of-two scales), the
+ *           15 slice products of order <= 4 on the int8 matrix cores with
+ *           exact int32 sums, folded into FP64 per 64-AO block: MO error
+ *           about 1e-9 of each row's maximum on well-conditioned bases and
+ *           below 1e-7 on the badly conditioned ChBrClF basis, where the
+ *           FP32-accumulating forms stop at a few 1e-6.
  * Results stay FP64 arrays [P][5][M]; changing the form invalidates mo_vgl. */
 typedef int32_t qmckl_precision_device;
 #define QMCKL_PRECISION_FP64_DEVICE ((qmckl_precision_device)0)
@@ -387,6 +395,7 @@ typedef int32_t qmckl_precision_device;
 #define QMCKL_PRECISION_FP16X2_DEVICE ((qmckl_precision_device)2)
 #define QMCKL_PRECISION_BF16X2_DEVICE ((qmckl_precision_device)3)
 #define QMCKL_PRECISION_BF16X3_DEVICE ((qmckl_precision_device)4)
+#define QMCKL_PRECISION_INT8X5_DEVICE ((qmckl_precision_device)5)
 qmckl_exit_code_device qmckl_set_mo_basis_precision_device(qmckl_context_device context,
                                                            qmckl_precision_device form);
 qmckl_exit_code_device qmckl_get_mo_basis_precision_device(qmckl_context_device context,

@@ -268,8 +268,13 @@ qmckl_exit_code_device qmckl_finalize_ao_basis_device(qmckl_context_device conte
   dfree(c, ao.ao_f32);
   ao.ao_f32 = nullptr;
   ao.ao_f32_cap = 0;
+  dfree(c, ao.ao_f32_lo);
+  ao.ao_f32_lo = nullptr;
+  ao.ao_f32_lo_cap = 0;
   dfree(c, c->mo.coef_split);
   c->mo.coef_split = nullptr;
+  dfree(c, c->mo.coef_i8);
+  c->mo.coef_i8 = nullptr;
   c->mo.coef_split_form = QMCKL_PRECISION_FP64_DEVICE;
   return QMCKL_SUCCESS_DEVICE;
 }

@@ -17,6 +17,7 @@
 #define QMCKL_AMD_SPLIT_FP16X2 QMCKL_PRECISION_FP16X2_DEVICE
 #define QMCKL_AMD_SPLIT_BF16X2 QMCKL_PRECISION_BF16X2_DEVICE
 #define QMCKL_AMD_SPLIT_BF16X3 QMCKL_PRECISION_BF16X3_DEVICE
+#define QMCKL_AMD_INT8X5 QMCKL_PRECISION_INT8X5_DEVICE
 
 namespace qmckl_amd {
 
@@ -130,6 +131,9 @@ struct AoBasis {
   // ld = A rounded up to 64), its row maxima and nonzero flags
   float *ao_f32 = nullptr;
   int64_t ao_f32_cap = 0;
+  // the INT8X5 form's residuals (float)(v - (double)(float)v), same layout
+  float *ao_f32_lo = nullptr;
+  int64_t ao_f32_lo_cap = 0;
   float *ao_f32_rowmax = nullptr;
   int64_t ao_f32_rowmax_cap = 0;
   uint8_t *ao_f32_flags = nullptr;
@@ -213,6 +217,9 @@ struct MoBasis {
   // split terms (or [M][ld] FP32), inverse column scales [M]
   uint16_t *coef_split = nullptr;
   float *coef_colinv = nullptr;
+  // the INT8X5 form's coefficient slices (k_gemm_i8.hip fragment order);
+  // coef_colinv then holds the inverse MO scales
+  int8_t *coef_i8 = nullptr;
   qmckl_precision_device coef_split_form = QMCKL_PRECISION_FP64_DEVICE;
   // tile choices of the mixed forms (results do not depend on them):
   // spatial point order -1 auto / 0 off / 1 on; AO-operand wavefronts per
@@ -397,11 +404,13 @@ hipError_t tile_point_order(Context *c, const int32_t **perm);
 // per tile, ao_rows_groups) and, if flags is not null, nonzero flags
 // [5 * tiles][nkb] with nkb = ld / 32.  With flags, a 64-AO stage that the
 // screen zeroes for all 64 points of a tile is not written (its two flags
-// are zero, so the contraction never reads it).
+// are zero, so the contraction never reads it).  With buf_lo (same size as
+// buf) each value's FP32 residual v - (double)(float)v goes there as well.
 int ao_rows_groups(int64_t P, int64_t ld, int group_waves = 0);
 hipError_t launch_ao_rows_f32(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
                               const int32_t *perm, const double *nucl, const double *ao_factor, int64_t ld,
-                              int groups, float *buf, float *rowmax, uint8_t *flags, int nkb);
+                              int groups, float *buf, float *rowmax, uint8_t *flags, int nkb,
+                              float *buf_lo = nullptr);
 hipError_t launch_ao_from_shell_table(hipStream_t s, const DeviceBasis &b, int64_t P,
                                       const double *coord, const double *nucl,
                                       const double *ao_factor, const double *shell_vgl,
@@ -502,6 +511,17 @@ hipError_t launch_mo_gemm_mixed_persistent(hipStream_t s, int form, int cbw, int
                                            int njobs, int64_t total, int *counter);
 hipError_t launch_strided_copy(hipStream_t s, int64_t rows, int64_t cols, const double *in,
                                int64_t ld_in, double *out, int64_t ld_out);
+// INT8X5 form (k_gemm_i8.hip): the coefficient slices of M MOs (prepared
+// column j = MO col_mo[j]) with ld = ldb, slice_coef_i8_bytes(M, ldb) bytes,
+// and colinv [M] their inverse MO scales; the contraction of the blocked AO
+// operand A and its residuals Alo (launch_ao_rows_f32 with buf_lo) into
+// mo_vgl C, with the flags, point order and column plan of the split forms.
+int64_t slice_coef_i8_bytes(int64_t M, int64_t ldb);
+hipError_t launch_slice_coef_i8(hipStream_t s, int64_t M, int64_t K, const double *coef, int64_t ldb, int8_t *Bq,
+                                float *colinv, const int32_t *col_mo);
+hipError_t launch_mo_gemm_i8(hipStream_t s, int64_t P, int64_t K, int64_t Ncol, const float *A, const float *Alo,
+                             int64_t lda, const int8_t *Bq, int64_t ldb, const float *colinv, const uint8_t *flags,
+                             int nkb, const int32_t *perm, double *C, int64_t ldc, ColPlan plan);
 
 // Allocation-free radial launcher (noalloc.hip): the shell table is built on
 // the device into a fixed file-scope device array of kNoAllocMaxShells

@@ -54,7 +54,11 @@ constexpr int kThreads = 256;
 // wavefront's LDS ring [component][AO mod 4][lane] (the slot is a runtime
 // index, which registers cannot take without spilling); after every fourth
 // AO each lane reads its four values back and stores them as one 16-byte
-// piece, so each store instruction writes one contiguous kilobyte.
+// piece, so each store instruction writes one contiguous kilobyte.  With LO
+// (the INT8X5 form) the residual (float)(v - (double)(float)v) of every
+// value goes the same way into a second array of the same layout, so that
+// hi + lo carries the FP64 value to 2^-48 relative.
+template <bool LO>
 struct RowWriter {
   float mx[5];
   uint32_t nz;       // bit c: some AO of the current 32-block is nonzero in row c
@@ -64,6 +68,8 @@ struct RowWriter {
   uint8_t *flags;    // flags + pt * nkb, component stride fstride
   int64_t fstride;
   float *ring;       // this wavefront's [5][4][64] LDS ring
+  float *ring_lo;    // the residuals' ring (LO)
+  float *base_lo;    // &buf_lo[0][pt][0][lane][0] (LO)
   int lane;
 
   int k_lo, k_hi;    // AOs of the current stage
@@ -78,6 +84,7 @@ struct RowWriter {
     for (int c = 0; c < 5; ++c) {
       const float w = (float)v[c];
       ring[(c * 4 + slot) * 64 + lane] = w;
+      if constexpr (LO) ring_lo[(c * 4 + slot) * 64 + lane] = (float)(v[c] - (double)w);
       mx[c] = fmaxf(mx[c], fabsf(w));
       nz |= (w != 0.0f ? 1u : 0u) << c;
     }
@@ -88,6 +95,14 @@ struct RowWriter {
         *reinterpret_cast<float4 *>(o + c * cstride) =
             make_float4(ring[(c * 4 + 0) * 64 + lane], ring[(c * 4 + 1) * 64 + lane],
                         ring[(c * 4 + 2) * 64 + lane], ring[(c * 4 + 3) * 64 + lane]);
+      if constexpr (LO) {
+        float *ol = base_lo + (int64_t)(k >> 2) * 256;
+#pragma unroll
+        for (int c = 0; c < 5; ++c)
+          *reinterpret_cast<float4 *>(ol + c * cstride) =
+              make_float4(ring_lo[(c * 4 + 0) * 64 + lane], ring_lo[(c * 4 + 1) * 64 + lane],
+                          ring_lo[(c * 4 + 2) * 64 + lane], ring_lo[(c * 4 + 3) * 64 + lane]);
+      }
     }
     if ((k & 31) == 31) {
       if (flags != nullptr) {
@@ -264,14 +279,17 @@ __device__ __attribute__((always_inline)) inline void eval_stage(W &w, int64_t s
   }
 }
 
+template <bool LO>
 __global__ void __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads), amdgpu_waves_per_eu(4)))
 ao_rows_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord, const int32_t *__restrict__ perm,
                const double *__restrict__ nucl, const ShellDesc *__restrict__ shells,
                const int32_t *__restrict__ stage_shell, const double *__restrict__ ao_factor,
                const double *__restrict__ expo, const double *__restrict__ coef,
                const double *__restrict__ nucl_range, int64_t A, int64_t ld, int64_t npt, int groups,
-               float *__restrict__ buf, float *__restrict__ rowmax, uint8_t *__restrict__ flags, int nkb) {
+               float *__restrict__ buf, float *__restrict__ buf_lo, float *__restrict__ rowmax,
+               uint8_t *__restrict__ flags, int nkb) {
   __shared__ float s_ring[kThreads / 64][5 * 4 * 64];
+  __shared__ float s_ring_lo[LO ? kThreads / 64 : 1][LO ? 5 * 4 * 64 : 1];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t wi = (int64_t)blockIdx.x * (kThreads / 64) + wave;
@@ -288,7 +306,7 @@ ao_rows_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord, co
   // this wavefront's AOs: 64-AO stages [g*nst/groups, (g+1)*nst/groups)
   const int64_t nst = ld / 64, nst_a = (A + 63) / 64;
   const int64_t st_lo = g * nst / groups, st_hi = (g + 1) * nst / groups;
-  RowWriter w;
+  RowWriter<LO> w;
 #pragma unroll
   for (int c = 0; c < 5; ++c) w.mx[c] = 0.0f;
   w.nz = 0;
@@ -298,6 +316,8 @@ ao_rows_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord, co
   w.fstride = npt * nkb;
   w.lane = lane;
   w.ring = s_ring[wave];
+  w.ring_lo = LO ? s_ring_lo[wave] : nullptr;
+  w.base_lo = LO ? buf_lo + pt * (ld / 4) * 256 + lane * 4 : nullptr;
 
   // Stage by stage; a shell that straddles a stage boundary is evaluated
   // for each stage it reaches (each writes its own AOs).  The screen and the
@@ -613,7 +633,7 @@ int ao_rows_groups(int64_t P, int64_t ld, int group_waves) {
 
 hipError_t launch_ao_rows_f32(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
                               const int32_t *perm, const double *nucl, const double *ao_factor, int64_t ld,
-                              int groups, float *buf, float *rowmax, uint8_t *flags, int nkb) {
+                              int groups, float *buf, float *rowmax, uint8_t *flags, int nkb, float *buf_lo) {
   if (P <= 0) return hipSuccess;
   if (b.max_l > kMaxL || ld % 64 != 0 || ld < b.ao_num || groups < 1 || groups > ld / 64 ||
       b.stage_shell == nullptr)
@@ -626,9 +646,14 @@ hipError_t launch_ao_rows_f32(hipStream_t s, const DeviceBasis &b, int64_t P, co
   const int64_t nmax = 5 * npt * 64;
   hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)std::min<int64_t>((nmax + kThreads - 1) / kThreads, 4096)),
                      dim3(kThreads), 0, s, nmax, reinterpret_cast<uint32_t *>(rowmax));
-  hipLaunchKernelGGL(ao_rows_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, P, (int)b.nucl_num,
-                     b.shell_num, coord, perm, nucl, b.shells, b.stage_shell, ao_factor, b.expo, b.coef_norm,
-                     b.nucl_range, b.ao_num, ld, npt, groups, buf, rowmax, flags, nkb);
+  if (buf_lo != nullptr)
+    hipLaunchKernelGGL(ao_rows_kernel<true>, dim3((unsigned)blocks), dim3(kThreads), 0, s, P, (int)b.nucl_num,
+                       b.shell_num, coord, perm, nucl, b.shells, b.stage_shell, ao_factor, b.expo, b.coef_norm,
+                       b.nucl_range, b.ao_num, ld, npt, groups, buf, buf_lo, rowmax, flags, nkb);
+  else
+    hipLaunchKernelGGL(ao_rows_kernel<false>, dim3((unsigned)blocks), dim3(kThreads), 0, s, P, (int)b.nucl_num,
+                       b.shell_num, coord, perm, nucl, b.shells, b.stage_shell, ao_factor, b.expo, b.coef_norm,
+                       b.nucl_range, b.ao_num, ld, npt, groups, buf, nullptr, rowmax, flags, nkb);
   return hipGetLastError();
 }
 

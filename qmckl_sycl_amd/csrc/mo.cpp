@@ -148,6 +148,7 @@ const char *gemm_timer_name(qmckl_precision_device form) {
     case QMCKL_PRECISION_FP16X2_DEVICE: return "mo_gemm_fp16x2";
     case QMCKL_PRECISION_BF16X2_DEVICE: return "mo_gemm_bf16x2";
     case QMCKL_PRECISION_BF16X3_DEVICE: return "mo_gemm_bf16x3";
+    case QMCKL_PRECISION_INT8X5_DEVICE: return "mo_gemm_int8x5";
     default: return "mo_gemm_f64";
   }
 }
@@ -167,9 +168,12 @@ qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Contex
   AoBasis &ao = c->ao;
   MoBasis &mo = c->mo;
   const int groups = ao_rows_groups(P, ld, mo.tile_group_waves);
+  const bool i8 = form == QMCKL_PRECISION_INT8X5_DEVICE;
   if (!grow_bytes(c, reinterpret_cast<void **>(&ao.ao_f32), &ao.ao_f32_cap, (int64_t)sizeof(float) * rows * ld) ||
       !grow_bytes(c, reinterpret_cast<void **>(&ao.ao_f32_rowmax), &ao.ao_f32_rowmax_cap,
-                  (int64_t)sizeof(float) * rows))
+                  (int64_t)sizeof(float) * rows) ||
+      (i8 && !grow_bytes(c, reinterpret_cast<void **>(&ao.ao_f32_lo), &ao.ao_f32_lo_cap,
+                         (int64_t)sizeof(float) * rows * ld)))
     return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
   uint8_t *flags = nullptr;
   if (nkb <= kMaxFlagBlocks) {
@@ -187,7 +191,16 @@ qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Contex
   hipError_t e = hipSuccess;
   if (form != QMCKL_PRECISION_FP32_DEVICE) e = col_plan_for(c, split_wg_cols(form, M), &plan);
   if (e != hipSuccess) return hip_check(context, e, fn);
-  if (mo.coef_split == nullptr || mo.coef_split_form != form) {
+  if (i8 && (mo.coef_i8 == nullptr || mo.coef_split_form != form)) {
+    // the coefficients cut into int8 slices, rebuilt whenever the prepared
+    // form or the MO set changed
+    dfree(c, mo.coef_i8);
+    mo.coef_i8 = static_cast<int8_t *>(dmalloc(c, slice_coef_i8_bytes(M, ld)));
+    if (!drealloc(c, reinterpret_cast<void **>(&mo.coef_colinv), sizeof(float) * M) || !mo.coef_i8)
+      return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
+    e = launch_slice_coef_i8(c->stream, M, A, mo.coefficient, ld, mo.coef_i8, mo.coef_colinv, plan.col_mo);
+    if (e == hipSuccess) mo.coef_split_form = form;
+  } else if (!i8 && (mo.coef_split == nullptr || mo.coef_split_form != form)) {
     dfree(c, mo.coef_split);
     mo.coef_split = static_cast<uint16_t *>(
         dmalloc(c, sizeof(uint16_t) * split_terms(form) * split_cols_padded(form, M) * ld));
@@ -201,9 +214,10 @@ qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Contex
     if (e == hipErrorOutOfMemory) return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
   }
   if (e == hipSuccess) {
-    KernelTimer t(c, "ao_vgl_f32");
+    KernelTimer t(c, i8 ? "ao_vgl_f32_lo" : "ao_vgl_f32");
     e = launch_ao_rows_f32(c->stream, ao.dev, P, c->point.coord, perm, c->nucleus.coord, ao.ao_factor, ld,
-                           groups, ao.ao_f32, ao.ao_f32_rowmax, flags, flags ? nkb : 0);
+                           groups, ao.ao_f32, ao.ao_f32_rowmax, flags, flags ? nkb : 0,
+                           i8 ? ao.ao_f32_lo : nullptr);
     t.stop();
   }
   // The contraction kernel: by default the live-block kernel where blocks
@@ -218,9 +232,14 @@ qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Contex
   // In an eager batch, split forms of tile width <= 3 leave their
   // contraction to the batch's persistent launches (wider tiles need more
   // registers than the persistent kernel has to spare).
-  const bool collect = c->collect_jobs != nullptr && form != QMCKL_PRECISION_FP32_DEVICE &&
+  const bool collect = c->collect_jobs != nullptr && form != QMCKL_PRECISION_FP32_DEVICE && !i8 &&
                        split_wg_cols(form, M) <= 192;
-  if (e == hipSuccess && collect) {
+  if (e == hipSuccess && i8) {
+    KernelTimer t(c, gemm_timer_name(form));
+    e = launch_mo_gemm_i8(c->stream, P, A, M, ao.ao_f32, ao.ao_f32_lo, ld, mo.coef_i8, ld, mo.coef_colinv, flags,
+                          flags ? nkb : 0, perm, dst, M, plan);
+    t.stop();
+  } else if (e == hipSuccess && collect) {
     MixedGemmJob job;
     e = make_mixed_gemm_job(form, P, A, M, ao.ao_f32, ld, ao.ao_f32_rowmax, mo.coef_split, ld, mo.coef_colinv,
                             flags, flags ? nkb : 0, perm, dst, M, plan, &job, gemm_tile);
@@ -511,7 +530,7 @@ qmckl_exit_code_device qmckl_set_mo_basis_precision_device(qmckl_context_device 
                                                            qmckl_precision_device form) {
   Context *c = ctx_of(context);
   if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
-  if (form < QMCKL_PRECISION_FP64_DEVICE || form > QMCKL_PRECISION_BF16X3_DEVICE)
+  if (form < QMCKL_PRECISION_FP64_DEVICE || form > QMCKL_PRECISION_INT8X5_DEVICE)
     return fail(context, QMCKL_INVALID_ARG_2_DEVICE, "qmckl_set_mo_basis_precision_device",
                 "unknown precision form");
   if (form != c->mo.precision) {
@@ -892,7 +911,7 @@ qmckl_exit_code_device qmckl_mo_basis_autotune_device(qmckl_context_device conte
         best_waves = waves;
       }
     }
-  if (ok && mo.precision != QMCKL_PRECISION_FP32_DEVICE) {
+  if (ok && mo.precision != QMCKL_PRECISION_FP32_DEVICE && mo.precision != QMCKL_PRECISION_INT8X5_DEVICE) {
     mo.tile_point_order = best_order;
     mo.tile_group_waves = best_waves;
     for (int32_t g : {QMCKL_GEMM_TILE_LIVE_DEVICE, QMCKL_GEMM_TILE_LIVE_CACHED_DEVICE, QMCKL_GEMM_TILE_DENSE_DEVICE,
@@ -960,7 +979,7 @@ std::vector<uint64_t> batch_signature(const Context *c, const double *out) {
           u(mo.coef_split), u(mo.coef_colinv), u(mo.col_plans[0].col_mo), u(mo.col_plans[0].blk),
           u(mo.col_plans[1].col_mo), u(mo.col_plans[1].blk), (uint64_t)ao.shell_num, (uint64_t)b.nchunks,
           c->basis_generation, (uint64_t)(uint32_t)mo.tile_point_order, (uint64_t)(uint32_t)mo.tile_group_waves,
-          (uint64_t)(uint32_t)mo.tile_gemm};
+          (uint64_t)(uint32_t)mo.tile_gemm, u(ao.ao_f32_lo), u(mo.coef_i8)};
 }
 
 }  // namespace

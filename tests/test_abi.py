@@ -262,9 +262,9 @@ def test_precision_form_setting(ctx):
     f = ctypes.c_int32(-1)
     ctx.call("qmckl_get_mo_basis_precision_device", ctypes.byref(f))
     assert f.value == 0                       # FP64 plain by default
-    assert ctx.rc("qmckl_set_mo_basis_precision_device", 5) == 2
+    assert ctx.rc("qmckl_set_mo_basis_precision_device", 6) == 2
     assert ctx.rc("qmckl_set_mo_basis_precision_device", -1) == 2
-    for form in (1, 2, 3, 4, 0):
+    for form in (1, 2, 3, 4, 5, 0):
         ctx.call("qmckl_set_mo_basis_precision_device", form)
         ctx.call("qmckl_get_mo_basis_precision_device", ctypes.byref(f))
         assert f.value == form

@@ -100,7 +100,7 @@ def test_batch_mixed_forms_and_moving_points(molecules):
     # the split forms' contractions run as persistent launches over the
     # batch; every molecule must still equal its own single-context result
     mols = molecules[:25]
-    ctxs, outs = setup(mols, forms=[0, 2, 3, 1, 4])
+    ctxs, outs = setup(mols, forms=[0, 2, 3, 1, 4, 5])
     try:
         rng = np.random.default_rng(5)
         for step in range(2):
@@ -138,9 +138,9 @@ def _move(ctxs, mols, rng):
 def test_batch_graph_replays_match_eager(molecules):
     """The batch recorded into a HIP graph: every replay, with the points
     moved and eager batches run in between, equals the eager batch on the
-    same points bitwise (FP64 and FP16X2 contexts mixed)."""
+    same points bitwise (FP64, FP16X2 and INT8X5 contexts mixed)."""
     mols = molecules[:12]
-    ctxs, outs = setup(mols, forms=[0, 2])
+    ctxs, outs = setup(mols, forms=[0, 2, 5])
     ref_outs = [DeviceArray(c, (m.point_num, 5, m.mo_num)) for c, m in zip(ctxs, mols)]
     g = None
     try:

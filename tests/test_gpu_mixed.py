@@ -6,7 +6,9 @@ max_i |mo - mo_ref| / max_i |mo_ref|, maximised over rows, reported
 separately for value (c=0), gradient (c=1..3) and Laplacian (c=4).
 Bars: 1e-6 for FP32, FP16X2 and BF16X3; the two-term bf16 split (BF16X2)
 carries ~2^-17 relative representation error per operand and is held to
-1e-4 (its measured level is reported)."""
+1e-4 (its measured level is reported); the int8-slice form INT8X5, whose
+sums are exact, is held to 1e-7, and to 1e-6 on the badly conditioned
+ChBrClF basis where the FP32-accumulating forms stop at a few 1e-6."""
 import ctypes
 
 import numpy as np
@@ -19,8 +21,8 @@ from helpers import say
 
 pytestmark = pytest.mark.gpu
 
-FORMS = {"fp32": 1, "fp16x2": 2, "bf16x2": 3, "bf16x3": 4}
-BAR = {"fp32": 1e-6, "fp16x2": 1e-6, "bf16x2": 1e-4, "bf16x3": 1e-6}
+FORMS = {"fp32": 1, "fp16x2": 2, "bf16x2": 3, "bf16x3": 4, "int8x5": 5}
+BAR = {"fp32": 1e-6, "fp16x2": 1e-6, "bf16x2": 1e-4, "bf16x3": 1e-6, "int8x5": 1e-7}
 
 
 def row_err(got, ref):
@@ -47,7 +49,8 @@ def chb():
 # ChBrClF is worse conditioned: Br core exponents up to 1e7 make single
 # terms of an MO row up to ~1e2 times the row's largest value, so every
 # FP32-class form lands at a few 1e-6 (measured 4-7e-6).  Bars 10x looser.
-BAR_CHBRCLF = {"fp32": 1e-5, "fp16x2": 1e-5, "bf16x2": 1e-3, "bf16x3": 1e-5}
+# INT8X5 sums exactly and is held to the 1e-6 of the well-conditioned bases.
+BAR_CHBRCLF = {"fp32": 1e-5, "fp16x2": 1e-5, "bf16x2": 1e-3, "bf16x3": 1e-5, "int8x5": 1e-6}
 
 
 @pytest.mark.parametrize("form", list(FORMS))

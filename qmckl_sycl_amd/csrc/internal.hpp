@@ -404,13 +404,16 @@ hipError_t tile_point_order(Context *c, const int32_t **perm);
 // per tile, ao_rows_groups) and, if flags is not null, nonzero flags
 // [5 * tiles][nkb] with nkb = ld / 32.  With flags, a 64-AO stage that the
 // screen zeroes for all 64 points of a tile is not written (its two flags
-// are zero, so the contraction never reads it).  With buf_lo (same size as
-// buf) each value's FP32 residual v - (double)(float)v goes there as well.
+// are zero, so the contraction never reads it); with skip_halves, neither
+// is a 32-AO half that the screen zeroes (its flag is zero): only for
+// contractions that read no half whose flag is zero (not the dense split
+// kernel, which reads whole stages).  With buf_lo (same size as buf) each
+// value's FP32 residual v - (double)(float)v goes there as well.
 int ao_rows_groups(int64_t P, int64_t ld, int group_waves = 0);
 hipError_t launch_ao_rows_f32(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
                               const int32_t *perm, const double *nucl, const double *ao_factor, int64_t ld,
                               int groups, float *buf, float *rowmax, uint8_t *flags, int nkb,
-                              float *buf_lo = nullptr);
+                              float *buf_lo = nullptr, bool skip_halves = false);
 hipError_t launch_ao_from_shell_table(hipStream_t s, const DeviceBasis &b, int64_t P,
                                       const double *coord, const double *nucl,
                                       const double *ao_factor, const double *shell_vgl,

@@ -71,6 +71,7 @@ struct RowWriter {
   float *ring_lo;    // the residuals' ring (LO)
   float *base_lo;    // &buf_lo[0][pt][0][lane][0] (LO)
   int lane;
+  uint32_t hlive;    // bit h: 32-AO half h of the current stage is stored
 
   int k_lo, k_hi;    // AOs of the current stage
 
@@ -88,7 +89,7 @@ struct RowWriter {
       mx[c] = fmaxf(mx[c], fabsf(w));
       nz |= (w != 0.0f ? 1u : 0u) << c;
     }
-    if (slot == 3) {
+    if (slot == 3 && ((hlive >> ((k >> 5) & 1)) & 1u)) {
       float *o = base + (int64_t)(k >> 2) * 256;
 #pragma unroll
       for (int c = 0; c < 5; ++c)
@@ -211,6 +212,16 @@ __device__ __attribute__((always_inline)) inline void shell_aos_any(W &w, const 
   }
 }
 
+// e^-x for 0 <= x <= cutoff with FP32 accuracy, for the FP32 operand of
+// the FP32-class forms (which round every value to FP32 anyway): -x log2 e
+// = n + f in FP64 (n integer, |f| <= 1/2), 2^f by v_exp_f32 (about 1 ulp),
+// times 2^n exactly.  A third of the FP64 exp's instructions.
+__device__ __attribute__((always_inline)) inline double exp_neg_f32(double x) {
+  const double t = x * -1.4426950408889634074;
+  const double n = rint(t);
+  return ldexp((double)__builtin_amdgcn_exp2f((float)(t - n)), (int)n);
+}
+
 // Is any AO of the shells from s_first that start below k_end nonzero for
 // some point of the wavefront?  Same screen as eval_stage: a shell is zero
 // at r^2 when a * r^2 > cutoff for all of its exponents a (a_min * r^2
@@ -280,14 +291,14 @@ __device__ __attribute__((always_inline)) inline void eval_stage(W &w, int64_t s
 }
 
 template <bool LO>
-__global__ void __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads), amdgpu_waves_per_eu(4)))
+__global__ void __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads), amdgpu_waves_per_eu(LO ? 4 : 5)))
 ao_rows_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord, const int32_t *__restrict__ perm,
                const double *__restrict__ nucl, const ShellDesc *__restrict__ shells,
                const int32_t *__restrict__ stage_shell, const double *__restrict__ ao_factor,
                const double *__restrict__ expo, const double *__restrict__ coef,
                const double *__restrict__ nucl_range, int64_t A, int64_t ld, int64_t npt, int groups,
                float *__restrict__ buf, float *__restrict__ buf_lo, float *__restrict__ rowmax,
-               uint8_t *__restrict__ flags, int nkb) {
+               uint8_t *__restrict__ flags, int nkb, int skip_halves) {
   __shared__ float s_ring[kThreads / 64][5 * 4 * 64];
   __shared__ float s_ring_lo[LO ? kThreads / 64 : 1][LO ? 5 * 4 * 64 : 1];
   const int lane = threadIdx.x & 63;
@@ -318,6 +329,7 @@ ao_rows_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord, co
   w.ring = s_ring[wave];
   w.ring_lo = LO ? s_ring_lo[wave] : nullptr;
   w.base_lo = LO ? buf_lo + pt * (ld / 4) * 256 + lane * 4 : nullptr;
+  w.hlive = 3u;
 
   // Stage by stage; a shell that straddles a stage boundary is evaluated
   // for each stage it reaches (each writes its own AOs).  The screen and the
@@ -329,7 +341,12 @@ ao_rows_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord, co
     const int k_end = (int)(64 * (st + 1));
     const int64_t s_first = st < nst_a ? stage_shell[st] : S;
     if (flags != nullptr) {
-      bool any = false;
+      // the same screen per 32-AO half (a shell counts for each half it
+      // overlaps): with skip_halves a half zero for all 64 points is not
+      // stored either (its flags come out zero, and the contractions that
+      // take skip_halves never read a half whose flag is zero)
+      bool any0 = false, any1 = false;
+      const int k_mid = k_end - 32;
       for (int64_t s = s_first; s < S; ++s) {
         const ShellDesc d = shells[s];
         if (d.ao_start >= k_end) break;
@@ -337,12 +354,17 @@ ao_rows_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord, co
         const double y = py - nucl[N + d.nucl];
         const double z = pz - nucl[2 * N + d.nucl];
         const double r2 = fma(x, x, fma(y, y, z * z));
-        any |= inside && !(r2 > kCutoff * nucl_range[d.nucl]) && !(d.a_min * r2 > kCutoff);
+        const bool live = inside && !(r2 > kCutoff * nucl_range[d.nucl]) && !(d.a_min * r2 > kCutoff);
+        const int a_end = d.ao_start + (d.l + 1) * (d.l + 2) / 2;
+        any0 |= live && d.ao_start < k_mid;
+        any1 |= live && a_end > k_mid;
       }
-      if (__ballot(any) == 0) {
+      const bool h0 = __ballot(any0) != 0, h1 = __ballot(any1) != 0;
+      if (!h0 && !h1) {
         if (lane < 10) flags[pt * nkb + (lane >> 1) * w.fstride + 2 * st + (lane & 1)] = 0;
         continue;
       }
+      w.hlive = skip_halves ? (h0 ? 1u : 0u) | (h1 ? 2u : 0u) : 3u;
     }
     w.k_lo = (int)(64 * st);
     w.k_hi = k_end;
@@ -362,7 +384,7 @@ ao_rows_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord, co
         const double a = ex[j];
         const double ar2 = a * r2;
         if (live && !(ar2 > kCutoff)) {
-          const double v = cf[j] * exp(-ar2);
+          const double v = cf[j] * (LO ? exp(-ar2) : exp_neg_f32(ar2));
           const double two_a = -2.0 * a * v;
           R[0] += v;
           R[1] += two_a * x;
@@ -633,7 +655,8 @@ int ao_rows_groups(int64_t P, int64_t ld, int group_waves) {
 
 hipError_t launch_ao_rows_f32(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
                               const int32_t *perm, const double *nucl, const double *ao_factor, int64_t ld,
-                              int groups, float *buf, float *rowmax, uint8_t *flags, int nkb, float *buf_lo) {
+                              int groups, float *buf, float *rowmax, uint8_t *flags, int nkb, float *buf_lo,
+                              bool skip_halves) {
   if (P <= 0) return hipSuccess;
   if (b.max_l > kMaxL || ld % 64 != 0 || ld < b.ao_num || groups < 1 || groups > ld / 64 ||
       b.stage_shell == nullptr)
@@ -649,11 +672,13 @@ hipError_t launch_ao_rows_f32(hipStream_t s, const DeviceBasis &b, int64_t P, co
   if (buf_lo != nullptr)
     hipLaunchKernelGGL(ao_rows_kernel<true>, dim3((unsigned)blocks), dim3(kThreads), 0, s, P, (int)b.nucl_num,
                        b.shell_num, coord, perm, nucl, b.shells, b.stage_shell, ao_factor, b.expo, b.coef_norm,
-                       b.nucl_range, b.ao_num, ld, npt, groups, buf, buf_lo, rowmax, flags, nkb);
+                       b.nucl_range, b.ao_num, ld, npt, groups, buf, buf_lo, rowmax, flags, nkb,
+                       skip_halves ? 1 : 0);
   else
     hipLaunchKernelGGL(ao_rows_kernel<false>, dim3((unsigned)blocks), dim3(kThreads), 0, s, P, (int)b.nucl_num,
                        b.shell_num, coord, perm, nucl, b.shells, b.stage_shell, ao_factor, b.expo, b.coef_norm,
-                       b.nucl_range, b.ao_num, ld, npt, groups, buf, nullptr, rowmax, flags, nkb);
+                       b.nucl_range, b.ao_num, ld, npt, groups, buf, nullptr, rowmax, flags, nkb,
+                       skip_halves ? 1 : 0);
   return hipGetLastError();
 }
 

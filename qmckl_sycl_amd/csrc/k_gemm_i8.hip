@@ -80,7 +80,7 @@ slice_coef_i8_kernel(int64_t M, int64_t K, const double *__restrict__ coef, int6
     if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
     __syncthreads();
   }
-  const int e = slice_exponent(red[0]);
+  const int e = min(120, max(-120, slice_exponent(red[0])));  // colinv is a float
   if (threadIdx.x == 0 && real) colinv[i] = (float)ldexp(1.0, -e);
   const int64_t nst = ldb / kStageK;
   for (int64_t k = threadIdx.x; k < ldb; k += kThreads) {
@@ -103,8 +103,10 @@ __device__ inline v4i mfma8(const v4i &a, const v4i &b, const v4i &c) {
 // 16-row blocks.  The live-stage list and per-stage live bits are those of
 // the live-block split kernel (nonzero AO flags of the tile, nonzero
 // coefficient blocks of the column plan), one bit per (stage, column
-// block).  Per stage: every thread slices 8 consecutive AOs of one row
-// (the row's 8 threads are neighbouring lanes, so the row's stage maximum
+// block); a 32-AO half whose flag is zero is taken as zeros, since the AO
+// operand kernel may leave it unwritten.  Per stage: every thread slices 8
+// consecutive AOs of one row (the row's 8 threads are neighbouring lanes,
+// so the row's stage maximum
 // is three shuffles) into the double-buffered LDS slab [slice][row][64 K];
 // one barrier; the next stage's AO values are requested; each wavefront
 // reads its A fragments from the slab and, per live column block, its five
@@ -181,7 +183,9 @@ gemm_i8_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *__r
       if (live) {
         const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
         s_list[pos] = st;
-        s_live[pos] = lv;
+        // bits 30, 31: the stage's two 32-AO halves hold values (the AO
+        // operand kernel may leave a half whose flag is zero unwritten)
+        s_live[pos] = lv | (h0 ? 1u << 30 : 0u) | (h1 ? 1u << 31 : 0u);
       }
       base += __popcll(m);
     }
@@ -203,7 +207,11 @@ gemm_i8_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *__r
       rl[h] = __builtin_nontemporal_load(reinterpret_cast<const f4 *>(Lp + o + h * 256));
     }
   };
-  auto write_slab = [&](int buf) {
+  auto write_slab = [&](int buf, uint32_t halves) {
+    if (!((halves >> (q >> 2)) & 1u)) {  // this thread's half is unwritten: zeros
+#pragma unroll
+      for (int h = 0; h < 2; ++h) ra[h] = rl[h] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
     float m = 0.0f;
 #pragma unroll
     for (int h = 0; h < 2; ++h)
@@ -263,7 +271,7 @@ gemm_i8_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *__r
     const uint32_t live = wave_live(s);
     v4i bc[kSl];
     load_b(bc, st, 0, (live & 1u) != 0);  // in flight while the slab is cut
-    write_slab(buf);
+    write_slab(buf, (uint32_t)__builtin_amdgcn_readfirstlane(s_live[s]) >> 30);
     __syncthreads();
     if (s + 1 < count) load_a((int64_t)stage_of(s + 1) * kStageK);
     const unsigned char *slab = s_buf + buf * kSlabBytes;

@@ -213,13 +213,6 @@ qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Contex
     e = tile_point_order(c, &perm);
     if (e == hipErrorOutOfMemory) return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
   }
-  if (e == hipSuccess) {
-    KernelTimer t(c, i8 ? "ao_vgl_f32_lo" : "ao_vgl_f32");
-    e = launch_ao_rows_f32(c->stream, ao.dev, P, c->point.coord, perm, c->nucleus.coord, ao.ao_factor, ld,
-                           groups, ao.ao_f32, ao.ao_f32_rowmax, flags, flags ? nkb : 0,
-                           i8 ? ao.ao_f32_lo : nullptr);
-    t.stop();
-  }
   // The contraction kernel: by default the live-block kernel where blocks
   // can be dead (a column plan, or bases of 1024 AOs or more whose 64-AO
   // stages the point order screens per tile), the dense one otherwise.
@@ -229,6 +222,17 @@ qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Contex
   int gemm_tile = mo.tile_gemm;
   if (gemm_tile == QMCKL_GEMM_TILE_AUTO_DEVICE)
     gemm_tile = plan.col_mo != nullptr || ld >= 1024 ? QMCKL_GEMM_TILE_LIVE_DEVICE : QMCKL_GEMM_TILE_DENSE_DEVICE;
+  // Every contraction but the dense split kernel reads no 32-AO half whose
+  // flag is zero, so the AO kernel need not store such halves.
+  const bool skip_halves =
+      flags != nullptr && (i8 || form == QMCKL_PRECISION_FP32_DEVICE || gemm_tile != QMCKL_GEMM_TILE_DENSE_DEVICE);
+  if (e == hipSuccess) {
+    KernelTimer t(c, i8 ? "ao_vgl_f32_lo" : "ao_vgl_f32");
+    e = launch_ao_rows_f32(c->stream, ao.dev, P, c->point.coord, perm, c->nucleus.coord, ao.ao_factor, ld,
+                           groups, ao.ao_f32, ao.ao_f32_rowmax, flags, flags ? nkb : 0,
+                           i8 ? ao.ao_f32_lo : nullptr, skip_halves);
+    t.stop();
+  }
   // In an eager batch, split forms of tile width <= 3 leave their
   // contraction to the batch's persistent launches (wider tiles need more
   // registers than the persistent kernel has to spare).

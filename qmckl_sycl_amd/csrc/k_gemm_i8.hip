@@ -291,33 +291,31 @@ gemm_i8_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *__r
       v4i bn[kSl];
       if (cb + 1 < CBW) load_b(bn, st, cb + 1, ((live >> (4 * (cb + 1))) & 1u) != 0);
       if ((live >> (4 * cb)) & 1u) {
+        // d[r][L]: the products of order L (slice i of A times slice L - i
+        // of C), issued round by round over both row blocks, so that
+        // consecutive MFMAs feed different sums
+        const v4i z = {0, 0, 0, 0};
+        v4i d[2][kSl];
 #pragma unroll
-        for (int r = 0; r < 2; ++r) {
-          const v4i z = {0, 0, 0, 0};
-          v4i d0 = mfma8(a[r][0], bc[0], z);
-          v4i d1 = mfma8(a[r][0], bc[1], z);
-          v4i d2 = mfma8(a[r][0], bc[2], z);
-          v4i d3 = mfma8(a[r][0], bc[3], z);
-          v4i d4 = mfma8(a[r][0], bc[4], z);
-          d1 = mfma8(a[r][1], bc[0], d1);
-          d2 = mfma8(a[r][1], bc[1], d2);
-          d3 = mfma8(a[r][1], bc[2], d3);
-          d4 = mfma8(a[r][1], bc[3], d4);
-          d2 = mfma8(a[r][2], bc[0], d2);
-          d3 = mfma8(a[r][2], bc[1], d3);
-          d4 = mfma8(a[r][2], bc[2], d4);
-          d3 = mfma8(a[r][3], bc[0], d3);
-          d4 = mfma8(a[r][3], bc[1], d4);
-          d4 = mfma8(a[r][4], bc[0], d4);
+        for (int L = 0; L < kSl; ++L)
+#pragma unroll
+          for (int r = 0; r < 2; ++r) d[r][L] = mfma8(a[r][0], bc[L], z);
+#pragma unroll
+        for (int i = 1; i < kSl; ++i)
+#pragma unroll
+          for (int L = i; L < kSl; ++L)
+#pragma unroll
+            for (int r = 0; r < 2; ++r) d[r][L] = mfma8(a[r][i], bc[L - i], d[r][L]);
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
-            const int e0 = d0[j] * 128 + d1[j];
-            const int e1 = d2[j] * 128 + d3[j];
+            const int e0 = d[r][0][j] * 128 + d[r][1][j];
+            const int e1 = d[r][2][j] * 128 + d[r][3][j];
             double x = fma((double)e1, 0x1p-14, (double)e0);
-            x = fma((double)d4[j], 0x1p-21, x);
+            x = fma((double)d[r][4][j], 0x1p-21, x);
             acc[r][cb][j] = fma(x, rs[r][j], acc[r][cb][j]);
           }
-        }
       }
       if (cb + 1 < CBW) {
 #pragma unroll

@@ -148,6 +148,15 @@ constexpr int kStage = 64;        // K per stage
 constexpr int kLDS = kStage + 8;  // LDS row length in halves (bank spread)
 constexpr int kMaxStages = kMaxFlagBlocks / 2;
 constexpr int kOutRows = 8;  // output rows per pass of the planned-order epilogue
+// Wavefronts per SIMD (= resident workgroups per CU) of the persistent
+// kernel by tile width: narrow tiles hold few accumulators, and more of
+// them in flight hide the AO-slab loads, which a tile of 64 or 128 MOs
+// follows with few MFMAs (the most that fit without spilling).
+constexpr int persistent_waves(int cbw, int terms) {
+  return cbw == 1 ? (terms == 2 ? 4 : 3) : (cbw == 2 && terms == 2) ? 3 : 2;
+}
+// The same for the dense per-context kernel (fewer registers: no job loop).
+constexpr int dense_waves(int cbw, int terms) { return cbw == 1 || (cbw == 2 && terms == 2) ? 4 : 2; }
 
 template <int SP>
 __device__ inline void split8(const float (&v)[8], uint4 (&out)[Terms<SP>::n]) {
@@ -281,12 +290,15 @@ __device__ __forceinline__ void split_tile(int64_t bx, int64_t by, int64_t P, in
       (void *)(Bs + gb0 * nkc * T * 512), (short)0, 4 * CBW * tile_bytes, 0x00020000);
   const int lane_off = 16 * lane;
 
-  float4 ra[KQ / 4];
+  // two register sets of the A slab, stages s and s + 1 in flight while
+  // stage s - 1 computes (the loop is unrolled by two so that each stage's
+  // set is fixed at compile time)
+  float4 ra0[KQ / 4], ra1[KQ / 4];
   uint4 bq[2][CBW][T];
   auto stage_k0 = [&](int s) {
     return (int64_t)__builtin_amdgcn_readfirstlane(listed ? s_list[s] : s) * kStage;
   };
-  auto load_a = [&](int64_t k0) {
+  auto load_a = [&](float4 (&ra)[KQ / 4], int64_t k0) {
     const float *src = Ap + (k0 / 4) * 256;
 #pragma unroll
     for (int q = 0; q < KQ / 4; ++q) ra[q] = *reinterpret_cast<const float4 *>(src + q * 256);
@@ -311,15 +323,7 @@ __device__ __forceinline__ void split_tile(int64_t bx, int64_t by, int64_t P, in
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[r][cb][j] = 0.0;
 
-  if (count > 0) {
-    const int64_t k0 = stage_k0(0);
-    load_a(k0);
-    load_b(k0, 0, wave_live(0, 0));
-    load_b(k0, 1, wave_live(0, 1));
-  } else {
-    __syncthreads();  // s_inv is read by every wavefront in the epilogue
-  }
-  for (int s = 0; s < count; ++s) {
+  auto stage = [&](float4 (&ra)[KQ / 4], int s) {
     __syncthreads();  // every wavefront is done reading the previous A slab
 #pragma unroll
     for (int h = 0; h < KQ / 8; ++h) {
@@ -332,10 +336,10 @@ __device__ __forceinline__ void split_tile(int64_t bx, int64_t by, int64_t P, in
       for (int t = 0; t < T; ++t) *reinterpret_cast<uint4 *>(&sA[t * (WR * kLDS) + ar * kLDS + aq + 8 * h]) = o[t];
     }
     __syncthreads();
+    if (s + 2 < count) load_a(ra, stage_k0(s + 2));  // this set is free again
     const bool more = s + 1 < count;
     const int64_t k1 = more ? stage_k0(s + 1) : 0;
     const uint32_t live1[2] = {more ? wave_live(s + 1, 0) : 0u, more ? wave_live(s + 1, 1) : 0u};
-    if (more) load_a(k1);  // in flight while this stage computes
 
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks) {
@@ -368,6 +372,20 @@ __device__ __forceinline__ void split_tile(int64_t bx, int64_t by, int64_t P, in
       for (int cb = 0; cb < CBW; ++cb)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[r][cb][j] += (double)d[r][cb][j];
+  };
+
+  if (count > 0) {
+    const int64_t k0 = stage_k0(0);
+    load_a(ra0, k0);
+    if (count > 1) load_a(ra1, stage_k0(1));
+    load_b(k0, 0, wave_live(0, 0));
+    load_b(k0, 1, wave_live(0, 1));
+  } else {
+    __syncthreads();  // s_inv is read by every wavefront in the epilogue
+  }
+  for (int s = 0; s < count; s += 2) {
+    stage(ra0, s);
+    if (s + 1 < count) stage(ra1, s + 1);
   }
   if (plan.col_mo == nullptr) {
 #pragma unroll
@@ -427,7 +445,8 @@ __device__ __forceinline__ void split_tile(int64_t bx, int64_t by, int64_t P, in
 
 
 template <int SP, int CBW, int WR>
-__global__ void __attribute__((amdgpu_flat_work_group_size(WR * 8, WR * 8), amdgpu_waves_per_eu(2, 2)))
+__global__ void __attribute__((amdgpu_flat_work_group_size(WR * 8, WR * 8),
+                               amdgpu_waves_per_eu(dense_waves(CBW, Terms<SP>::n), dense_waves(CBW, Terms<SP>::n))))
 gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *__restrict__ A,
                   int64_t lda, const float *__restrict__ rowmax, const uint16_t *__restrict__ Bs,
                   int64_t ldb, const float *__restrict__ colinv, const uint8_t *__restrict__ flags, int nkb,
@@ -753,7 +772,9 @@ gemm_split_live_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const fl
 // Every workgroup leaves the loop once the counter passes `total`.  Each
 // item runs split_tile_live (LIVE) or split_tile.
 template <int SP, int CBW, bool LIVE>
-__global__ void __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(2, 2)))
+__global__ void __attribute__((amdgpu_flat_work_group_size(256, 256),
+                               amdgpu_waves_per_eu(persistent_waves(CBW, Terms<SP>::n),
+                                                   persistent_waves(CBW, Terms<SP>::n))))
 gemm_split_persistent_kernel(const MixedGemmJob *__restrict__ jobs, int njobs, int64_t total, int *counter) {
   constexpr int WR = 32;
   using L = LiveTileLds<SP, CBW, 2>;
@@ -1132,7 +1153,8 @@ hipError_t launch_mo_gemm_mixed_persistent(hipStream_t s, int form, int cbw, int
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) return e;
-  const unsigned grid = (unsigned)(2 * (cus > 0 ? cus : 1));  // two workgroups per CU
+  const unsigned grid =
+      (unsigned)(persistent_waves(cbw, split_terms(form)) * (cus > 0 ? cus : 1));  // as many as fit
   hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, s, counter);
 #define QMCKL_PERSISTENT(SPV, W)                                                                                   \
   do {                                                                                                            \

@@ -23,40 +23,15 @@
 
 
 #include "internal.hpp"
+#include "split_ops.hpp"
 
 namespace qmckl_amd {
 
 namespace {
 
-typedef _Float16 h8 __attribute__((ext_vector_type(8)));
-typedef __bf16 b8 __attribute__((ext_vector_type(8)));
-typedef float f4 __attribute__((ext_vector_type(4)));
-
 constexpr int kThreads = 256;
 constexpr int kBM = 64;
 constexpr int kKB = 32;
-
-// power-of-two scale bringing max |x| into [2^13, 2^14); the exponent is
-// clamped so that the scale and its inverse stay normal floats
-__device__ inline float pow2_scale(float m) {
-  if (!(m > 0.0f) || !isfinite(m)) return 1.0f;
-  return ldexpf(1.0f, min(100, max(-100, 13 - ilogbf(m))));
-}
-
-template <int SP>
-struct Terms;  // number of split terms per operand
-template <>
-struct Terms<QMCKL_AMD_SPLIT_FP16X2> {
-  static constexpr int n = 2;
-};
-template <>
-struct Terms<QMCKL_AMD_SPLIT_BF16X2> {
-  static constexpr int n = 2;
-};
-template <>
-struct Terms<QMCKL_AMD_SPLIT_BF16X3> {
-  static constexpr int n = 3;
-};
 
 template <int SP>
 __device__ inline void split_store(float x, uint16_t *dst, int stride) {
@@ -76,15 +51,6 @@ __device__ inline void split_store(float x, uint16_t *dst, int stride) {
       dst[2 * stride] = __builtin_bit_cast(uint16_t, l);
     }
   }
-}
-
-template <int SP>
-__device__ inline f4 mfma(const uint4 &a, const uint4 &b, f4 c) {
-  if constexpr (SP == QMCKL_AMD_SPLIT_FP16X2)
-    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8, a), __builtin_bit_cast(h8, b), c,
-                                                  0, 0, 0);
-  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8, a), __builtin_bit_cast(b8, b), c,
-                                                 0, 0, 0);
 }
 
 // Split GEMM (FP16X2, BF16X2, BF16X3).
@@ -157,35 +123,6 @@ constexpr int persistent_waves(int cbw, int terms) {
 }
 // The same for the dense per-context kernel (fewer registers: no job loop).
 constexpr int dense_waves(int cbw, int terms) { return cbw == 1 || (cbw == 2 && terms == 2) ? 4 : 2; }
-
-template <int SP>
-__device__ inline void split8(const float (&v)[8], uint4 (&out)[Terms<SP>::n]) {
-  if constexpr (SP == QMCKL_AMD_SPLIT_FP16X2) {
-    h8 hi, lo;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const _Float16 h = (_Float16)v[e];
-      hi[e] = h;
-      lo[e] = (_Float16)(v[e] - (float)h);
-    }
-    out[0] = __builtin_bit_cast(uint4, hi);
-    out[1] = __builtin_bit_cast(uint4, lo);
-  } else {
-    b8 hi, mid, lo;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const __bf16 h = (__bf16)v[e];
-      const float r = v[e] - (float)h;
-      const __bf16 m = (__bf16)r;
-      hi[e] = h;
-      mid[e] = m;
-      if constexpr (SP == QMCKL_AMD_SPLIT_BF16X3) lo[e] = (__bf16)(r - (float)m);
-    }
-    out[0] = __builtin_bit_cast(uint4, hi);
-    out[1] = __builtin_bit_cast(uint4, mid);
-    if constexpr (SP == QMCKL_AMD_SPLIT_BF16X3) out[2] = __builtin_bit_cast(uint4, lo);
-  }
-}
 
 // One 32-row x 64*CBW-column tile (bx, by) of a split contraction; the
 // workgroup's LDS comes in as pointers so that one kernel can hold tiles of

@@ -512,6 +512,15 @@ hipError_t make_mixed_gemm_job(int form, int64_t P, int64_t K, int64_t Ncol, con
 // captured).
 hipError_t launch_mo_gemm_mixed_persistent(hipStream_t s, int form, int cbw, int dense, const MixedGemmJob *jobs,
                                            int njobs, int64_t total, int *counter);
+// Fused AO evaluation and split contraction (k_fused.hip) of P points
+// (coordinates [3][P], tile row i = point perm[i], null: point i) with the
+// coefficients prepared without a column plan (launch_split_coef, col_mo
+// null, ld = ldb) into mo_vgl C [P][5][M] (ldc): one workgroup per 64
+// points and 64 MOs, no AO operand in memory.
+constexpr int64_t kFusedMaxLd = 8192;  // AOs (rounded up to 64) the fused kernel's stage mask covers
+hipError_t launch_fused_ao_mo(hipStream_t s, int form, const DeviceBasis &b, int64_t P, const double *coord,
+                              const int32_t *perm, const double *nucl, const double *ao_factor, int64_t ld,
+                              const uint16_t *Bs, const float *colinv, int64_t M, double *C, int64_t ldc);
 hipError_t launch_strided_copy(hipStream_t s, int64_t rows, int64_t cols, const double *in,
                                int64_t ld_in, double *out, int64_t ld_out);
 // INT8X5 form (k_gemm_i8.hip): the coefficient slices of M MOs (prepared

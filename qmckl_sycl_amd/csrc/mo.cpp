@@ -153,9 +153,34 @@ const char *gemm_timer_name(qmckl_precision_device form) {
   }
 }
 
+const char *fused_timer_name(qmckl_precision_device form) {
+  switch (form) {
+    case QMCKL_PRECISION_FP16X2_DEVICE: return "ao_mo_fused_fp16x2";
+    case QMCKL_PRECISION_BF16X2_DEVICE: return "ao_mo_fused_bf16x2";
+    default: return "ao_mo_fused_bf16x3";
+  }
+}
+
+// The contraction kernel that the context's choice (QMCKL_GEMM_TILE_*) gives
+// form: FUSED only for the split forms with MOs in their given order and at
+// most kFusedMaxLd AOs (AUTO's choice elsewhere); AUTO: LIVE where blocks
+// can be dead (a column plan, or 1024 AOs or more), DENSE otherwise.
+// Measured on MI355X: water64 (4,160 AOs, planned columns) 2.38 ms with the
+// live kernel against 2.97 with the dense one; the 48-molecule ragged batch
+// (30-620 AOs, dense coefficients) 144 M points/s with the dense kernel
+// against 123 M with the live one.
+int32_t resolve_gemm_tile(int32_t choice, qmckl_precision_device form, bool planned, int64_t ld) {
+  const bool fusable = !planned && ld <= kFusedMaxLd &&
+                       (form == QMCKL_PRECISION_FP16X2_DEVICE || form == QMCKL_PRECISION_BF16X2_DEVICE ||
+                        form == QMCKL_PRECISION_BF16X3_DEVICE);
+  if (choice == QMCKL_GEMM_TILE_FUSED_DEVICE) choice = fusable ? choice : QMCKL_GEMM_TILE_AUTO_DEVICE;
+  if (choice != QMCKL_GEMM_TILE_AUTO_DEVICE) return choice;
+  return planned || ld >= 1024 ? QMCKL_GEMM_TILE_LIVE_DEVICE : QMCKL_GEMM_TILE_DENSE_DEVICE;
+}
+
 // Mixed forms: the FP64 AO assembly stored as FP32 in the blocked operand
 // layout with row maxima and nonzero flags, then the split MFMA contraction
-// into dst.
+// into dst; or, for the fused kernel, both in one launch.
 qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Context *c, double *dst,
                                             const char *fn) {
   if (!c->ao.provided || c->point.num <= 0 || c->point.coord == nullptr || !c->nucleus.provided)
@@ -167,16 +192,24 @@ qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Contex
   const int64_t rows = 5 * round_up(P, 64);  // (component, point) rows, points padded to 64
   AoBasis &ao = c->ao;
   MoBasis &mo = c->mo;
-  const int groups = ao_rows_groups(P, ld, mo.tile_group_waves);
   const bool i8 = form == QMCKL_PRECISION_INT8X5_DEVICE;
-  if (!grow_bytes(c, reinterpret_cast<void **>(&ao.ao_f32), &ao.ao_f32_cap, (int64_t)sizeof(float) * rows * ld) ||
-      !grow_bytes(c, reinterpret_cast<void **>(&ao.ao_f32_rowmax), &ao.ao_f32_rowmax_cap,
-                  (int64_t)sizeof(float) * rows) ||
-      (i8 && !grow_bytes(c, reinterpret_cast<void **>(&ao.ao_f32_lo), &ao.ao_f32_lo_cap,
-                         (int64_t)sizeof(float) * rows * ld)))
+  // the split forms use the column plan of their workgroup width; the FP32
+  // form keeps the MO order
+  ColPlan plan;
+  hipError_t e = hipSuccess;
+  if (form != QMCKL_PRECISION_FP32_DEVICE) e = col_plan_for(c, split_wg_cols(form, M), &plan);
+  if (e != hipSuccess) return hip_check(context, e, fn);
+  const int32_t gemm_tile = resolve_gemm_tile(mo.tile_gemm, form, plan.col_mo != nullptr, ld);
+  const bool fused = gemm_tile == QMCKL_GEMM_TILE_FUSED_DEVICE;
+  if (!fused && (!grow_bytes(c, reinterpret_cast<void **>(&ao.ao_f32), &ao.ao_f32_cap,
+                             (int64_t)sizeof(float) * rows * ld) ||
+                 !grow_bytes(c, reinterpret_cast<void **>(&ao.ao_f32_rowmax), &ao.ao_f32_rowmax_cap,
+                             (int64_t)sizeof(float) * rows) ||
+                 (i8 && !grow_bytes(c, reinterpret_cast<void **>(&ao.ao_f32_lo), &ao.ao_f32_lo_cap,
+                                    (int64_t)sizeof(float) * rows * ld))))
     return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
   uint8_t *flags = nullptr;
-  if (nkb <= kMaxFlagBlocks) {
+  if (!fused && nkb <= kMaxFlagBlocks) {
     if (!grow_bytes(c, reinterpret_cast<void **>(&ao.ao_f32_flags), &ao.ao_f32_flags_cap, rows / 64 * nkb))
       return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
     flags = ao.ao_f32_flags;
@@ -185,12 +218,6 @@ qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Contex
   // tile; worth its four small launches for large bases and many points
   // (water64 at 128 walkers: 94% -> 74% of the stages live).
   const int32_t *perm = nullptr;
-  // the split forms use the column plan of their workgroup width; the FP32
-  // form keeps the MO order
-  ColPlan plan;
-  hipError_t e = hipSuccess;
-  if (form != QMCKL_PRECISION_FP32_DEVICE) e = col_plan_for(c, split_wg_cols(form, M), &plan);
-  if (e != hipSuccess) return hip_check(context, e, fn);
   if (i8 && (mo.coef_i8 == nullptr || mo.coef_split_form != form)) {
     // the coefficients cut into int8 slices, rebuilt whenever the prepared
     // form or the MO set changed
@@ -209,19 +236,18 @@ qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Contex
     e = launch_split_coef(c->stream, form, M, A, mo.coefficient, ld, mo.coef_split, mo.coef_colinv, plan.col_mo);
     if (e == hipSuccess) mo.coef_split_form = form;
   }
-  if (e == hipSuccess && flags != nullptr) {
+  if (e == hipSuccess && (flags != nullptr || (fused && nkb <= kMaxFlagBlocks))) {
     e = tile_point_order(c, &perm);
     if (e == hipErrorOutOfMemory) return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
   }
-  // The contraction kernel: by default the live-block kernel where blocks
-  // can be dead (a column plan, or bases of 1024 AOs or more whose 64-AO
-  // stages the point order screens per tile), the dense one otherwise.
-  // Measured on MI355X: water64 (4,160 AOs, planned columns) 2.38 against
-  // 2.97 ms; the 48-molecule ragged batch (30-620 AOs, dense coefficients)
-  // 144 M points/s with the dense kernel against 123 M with the live one.
-  int gemm_tile = mo.tile_gemm;
-  if (gemm_tile == QMCKL_GEMM_TILE_AUTO_DEVICE)
-    gemm_tile = plan.col_mo != nullptr || ld >= 1024 ? QMCKL_GEMM_TILE_LIVE_DEVICE : QMCKL_GEMM_TILE_DENSE_DEVICE;
+  if (e == hipSuccess && fused) {
+    KernelTimer t(c, fused_timer_name(form));
+    e = launch_fused_ao_mo(c->stream, form, ao.dev, P, c->point.coord, perm, c->nucleus.coord, ao.ao_factor, ld,
+                           mo.coef_split, mo.coef_colinv, M, dst, M);
+    t.stop();
+    if (e == hipSuccess) e = sync(c);
+    return hip_check(context, e, fn);
+  }
   // Every contraction but the dense split kernel reads no 32-AO half whose
   // flag is zero, so the AO kernel need not store such halves.
   const bool skip_halves =
@@ -229,8 +255,8 @@ qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Contex
   if (e == hipSuccess) {
     KernelTimer t(c, i8 ? "ao_vgl_f32_lo" : "ao_vgl_f32");
     e = launch_ao_rows_f32(c->stream, ao.dev, P, c->point.coord, perm, c->nucleus.coord, ao.ao_factor, ld,
-                           groups, ao.ao_f32, ao.ao_f32_rowmax, flags, flags ? nkb : 0,
-                           i8 ? ao.ao_f32_lo : nullptr, skip_halves);
+                           ao_rows_groups(P, ld, mo.tile_group_waves), ao.ao_f32, ao.ao_f32_rowmax, flags,
+                           flags ? nkb : 0, i8 ? ao.ao_f32_lo : nullptr, skip_halves);
     t.stop();
   }
   // In an eager batch, split forms of tile width <= 3 leave their
@@ -761,13 +787,15 @@ struct TuneKey {
   int device;
   int64_t P, A, M, N;
   int form;
+  bool fused;  // the context keeps the fused contraction kernel
   bool operator==(const TuneKey &o) const {
-    return device == o.device && P == o.P && A == o.A && M == o.M && N == o.N && form == o.form;
+    return device == o.device && P == o.P && A == o.A && M == o.M && N == o.N && form == o.form &&
+           fused == o.fused;
   }
 };
 struct TuneKeyHash {
   size_t operator()(const TuneKey &k) const {
-    size_t h = (size_t)k.device * 1000003u ^ (size_t)k.form;
+    size_t h = (size_t)k.device * 1000003u ^ (size_t)k.form ^ ((size_t)k.fused << 20);
     for (int64_t v : {k.P, k.A, k.M, k.N}) h = h * 1000003u ^ (size_t)v;
     return h;
   }
@@ -801,7 +829,7 @@ qmckl_exit_code_device qmckl_set_mo_basis_tile_config_device(qmckl_context_devic
 qmckl_exit_code_device qmckl_set_mo_basis_gemm_tile_device(qmckl_context_device context, int32_t gemm_tile) {
   Context *c = ctx_of(context);
   if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
-  if (gemm_tile < QMCKL_GEMM_TILE_AUTO_DEVICE || gemm_tile > QMCKL_GEMM_TILE_LIVE64_DEVICE)
+  if (gemm_tile < QMCKL_GEMM_TILE_AUTO_DEVICE || gemm_tile > QMCKL_GEMM_TILE_FUSED_DEVICE)
     return fail(context, QMCKL_INVALID_ARG_2_DEVICE, "qmckl_set_mo_basis_gemm_tile_device",
                 "gemm_tile not one of QMCKL_GEMM_TILE_*");
   c->mo.tile_gemm = gemm_tile;
@@ -866,7 +894,8 @@ qmckl_exit_code_device qmckl_mo_basis_autotune_device(qmckl_context_device conte
     return QMCKL_SUCCESS_DEVICE;
   }
   if (c->point.num <= 0 || c->point.coord == nullptr) return fail(context, QMCKL_NOT_PROVIDED_DEVICE, fn, "no points set");
-  const TuneKey key{c->device_id, c->point.num, c->ao.ao_num, mo.mo_num, c->nucleus.num, (int)mo.precision};
+  const TuneKey key{c->device_id, c->point.num, c->ao.ao_num, mo.mo_num, c->nucleus.num, (int)mo.precision,
+                    mo.tile_gemm == QMCKL_GEMM_TILE_FUSED_DEVICE};
   {
     std::lock_guard<std::mutex> lock(g_tune_mutex);
     auto it = tune_cache().find(key);
@@ -899,9 +928,12 @@ qmckl_exit_code_device qmckl_mo_basis_autotune_device(qmckl_context_device conte
   // point order x AO wavefronts with the default contraction, then the
   // contraction kernels with the best of those (the FP32 form has one)
   float best = -1.0f;
-  int32_t best_order = 0, best_waves = 256, best_gemm = QMCKL_GEMM_TILE_AUTO_DEVICE;
+  // a context set to the fused kernel keeps it (the other kernels round
+  // differently); its point order and AO wavefronts are timed with it
   const int32_t gemm0 = mo.tile_gemm;
-  mo.tile_gemm = QMCKL_GEMM_TILE_AUTO_DEVICE;
+  const bool keep_fused = gemm0 == QMCKL_GEMM_TILE_FUSED_DEVICE;
+  int32_t best_order = 0, best_waves = 256, best_gemm = keep_fused ? gemm0 : QMCKL_GEMM_TILE_AUTO_DEVICE;
+  mo.tile_gemm = best_gemm;
   bool ok = e == hipSuccess;
   for (int32_t order = 0; order <= (can_order ? 1 : 0) && ok; ++order)
     for (int32_t waves : {16, 32, 64, 128, 256}) {
@@ -915,7 +947,8 @@ qmckl_exit_code_device qmckl_mo_basis_autotune_device(qmckl_context_device conte
         best_waves = waves;
       }
     }
-  if (ok && mo.precision != QMCKL_PRECISION_FP32_DEVICE && mo.precision != QMCKL_PRECISION_INT8X5_DEVICE) {
+  if (ok && !keep_fused && mo.precision != QMCKL_PRECISION_FP32_DEVICE &&
+      mo.precision != QMCKL_PRECISION_INT8X5_DEVICE) {
     mo.tile_point_order = best_order;
     mo.tile_group_waves = best_waves;
     for (int32_t g : {QMCKL_GEMM_TILE_LIVE_DEVICE, QMCKL_GEMM_TILE_LIVE_CACHED_DEVICE, QMCKL_GEMM_TILE_DENSE_DEVICE,

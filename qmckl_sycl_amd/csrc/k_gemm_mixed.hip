@@ -124,6 +124,53 @@ constexpr int persistent_waves(int cbw, int terms) {
 // The same for the dense per-context kernel (fewer registers: no job loop).
 constexpr int dense_waves(int cbw, int terms) { return cbw == 1 || (cbw == 2 && terms == 2) ? 4 : 2; }
 
+// Stage list of row tile rt for the column blocks gb0 .. gb0 + 4 CBW - 1,
+// built by wavefront 0 (lane = this thread's lane): the live 64-AO stages
+// in s_list[0, *s_count) and, per listed stage, bit b (first 32-AO half)
+// and 32 + b (second half) of s_live for live column block gb0 + b.  A
+// stage or half is live when the AO flags of the tile (null: all) mark it
+// and, with a column plan, some block has a nonzero coefficient there.
+template <int CBW>
+__device__ __forceinline__ void build_stage_list(int lane, int nst, int64_t rt, const uint8_t *__restrict__ flags,
+                                                 int nkb, const ColPlan &plan, int64_t gb0, int *s_list,
+                                                 uint64_t *s_live, int *s_count) {
+  const uint8_t *f = flags != nullptr ? flags + rt * nkb : nullptr;
+  int base = 0;
+  for (int s0 = 0; s0 < nst; s0 += 64) {
+    const int st = s0 + lane;
+    const bool h0 = st < nst && (f == nullptr || f[2 * st] != 0);
+    const bool h1 = st < nst && (f == nullptr || (2 * st + 1 < nkb && f[2 * st + 1] != 0));
+    uint64_t lv = (h0 ? 0xffffffffull : 0ull) | (h1 ? 0xffffffff00000000ull : 0ull);
+    if ((h0 || h1) && plan.blk != nullptr) {
+      // all loads first (independent, no branches), then the bits
+      const int64_t g = 4 * (int64_t)st;  // first 16-AO block of the stage
+      uint64_t w[4 * CBW];
+#pragma unroll
+      for (int b = 0; b < 4 * CBW; ++b) {
+        const int64_t blk = gb0 + b < plan.nblk ? gb0 + b : plan.nblk - 1;
+        w[b] = plan.blk[blk * plan.nw + (g >> 6)];
+      }
+      uint64_t bits = 0;
+#pragma unroll
+      for (int b = 0; b < 4 * CBW; ++b) {
+        const uint64_t nib = gb0 + b < plan.nblk ? (w[b] >> (g & 63)) & 15ull : 0ull;
+        bits |= ((nib & 3ull) != 0 ? 1ull : 0ull) << b;
+        bits |= ((nib & 12ull) != 0 ? 1ull : 0ull) << (32 + b);
+      }
+      lv &= bits;
+    }
+    const bool live = lv != 0;
+    const uint64_t m = __ballot(live);
+    if (live) {
+      const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
+      s_list[pos] = st;
+      s_live[pos] = lv;
+    }
+    base += __popcll(m);
+  }
+  if (lane == 0) *s_count = base;
+}
+
 // One 32-row x 64*CBW-column tile (bx, by) of a split contraction; the
 // workgroup's LDS comes in as pointers so that one kernel can hold tiles of
 // several widths (the persistent batch kernel below).
@@ -154,43 +201,7 @@ __device__ __forceinline__ void split_tile(int64_t bx, int64_t by, int64_t P, in
   const bool listed = flags != nullptr || plan.blk != nullptr;
   int count = nst;
   if (listed) {
-    if (wave == 0) {
-      const uint8_t *f = flags != nullptr ? flags + rt * nkb : nullptr;
-      int base = 0;
-      for (int s0 = 0; s0 < nst; s0 += 64) {
-        const int st = s0 + lane;
-        const bool h0 = st < nst && (f == nullptr || f[2 * st] != 0);
-        const bool h1 = st < nst && (f == nullptr || (2 * st + 1 < nkb && f[2 * st + 1] != 0));
-        uint64_t lv = (h0 ? 0xffffffffull : 0ull) | (h1 ? 0xffffffff00000000ull : 0ull);
-        if ((h0 || h1) && plan.blk != nullptr) {
-          // all loads first (independent, no branches), then the bits
-          const int64_t g = 4 * (int64_t)st;  // first 16-AO block of the stage
-          uint64_t w[4 * CBW];
-#pragma unroll
-          for (int b = 0; b < 4 * CBW; ++b) {
-            const int64_t blk = gb0 + b < plan.nblk ? gb0 + b : plan.nblk - 1;
-            w[b] = plan.blk[blk * plan.nw + (g >> 6)];
-          }
-          uint64_t bits = 0;
-#pragma unroll
-          for (int b = 0; b < 4 * CBW; ++b) {
-            const uint64_t nib = gb0 + b < plan.nblk ? (w[b] >> (g & 63)) & 15ull : 0ull;
-            bits |= ((nib & 3ull) != 0 ? 1ull : 0ull) << b;
-            bits |= ((nib & 12ull) != 0 ? 1ull : 0ull) << (32 + b);
-          }
-          lv &= bits;
-        }
-        const bool live = lv != 0;
-        const uint64_t m = __ballot(live);
-        if (live) {
-          const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
-          s_list[pos] = st;
-          s_live[pos] = lv;
-        }
-        base += __popcll(m);
-      }
-      if (lane == 0) *s_count = base;
-    }
+    if (wave == 0) build_stage_list<CBW>(lane, nst, rt, flags, nkb, plan, gb0, s_list, s_live, s_count);
     __syncthreads();
     count = *s_count;
   }
@@ -423,7 +434,12 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
 // against 2.97 ms for split_tile; 52% of its MFMAs are issued.  RB = 4
 // (the whole 64-row tile per wavefront, one wavefront per SIMD) halves the
 // fragment bytes per MFMA but runs 3.1 ms: one wavefront per SIMD hides
-// too little latency.
+// too little latency.  Counters (same workload, 2.38 ms): texture-data
+// unit 85% busy, 32 GB through L1 per launch (nine tenths fragments, L2
+// latency 242 cycles), matrix pipe 23% busy.  Measured slower: eight-
+// wavefront workgroups of a whole 64-row tile fetching each live fragment
+// once into LDS for both row halves (3.40 ms: the fragment reads, 160 KB
+// per stage and workgroup, then bind on LDS bandwidth).
 template <int SP, int CBW, int RB, bool NT>
 __device__ __forceinline__ void split_tile_live(int64_t bx, int64_t by, int64_t P, int64_t npt, int64_t K,
                                                 int64_t Ncol, const float *__restrict__ A, int64_t lda,
@@ -464,42 +480,7 @@ __device__ __forceinline__ void split_tile_live(int64_t bx, int64_t by, int64_t 
   const bool listed = flags != nullptr || plan.blk != nullptr;
   int count = nst;
   if (listed) {
-    if (wave == 0) {
-      const uint8_t *f = flags != nullptr ? flags + rt * nkb : nullptr;
-      int base = 0;
-      for (int s0 = 0; s0 < nst; s0 += 64) {
-        const int st = s0 + lane;
-        const bool h0 = st < nst && (f == nullptr || f[2 * st] != 0);
-        const bool h1 = st < nst && (f == nullptr || (2 * st + 1 < nkb && f[2 * st + 1] != 0));
-        uint64_t lv = (h0 ? 0xffffffffull : 0ull) | (h1 ? 0xffffffff00000000ull : 0ull);
-        if ((h0 || h1) && plan.blk != nullptr) {
-          const int64_t g = 4 * (int64_t)st;
-          uint64_t w[4 * CBW];
-#pragma unroll
-          for (int b = 0; b < 4 * CBW; ++b) {
-            const int64_t blk = gb0 + b < plan.nblk ? gb0 + b : plan.nblk - 1;
-            w[b] = plan.blk[blk * plan.nw + (g >> 6)];
-          }
-          uint64_t bits = 0;
-#pragma unroll
-          for (int b = 0; b < 4 * CBW; ++b) {
-            const uint64_t nib = gb0 + b < plan.nblk ? (w[b] >> (g & 63)) & 15ull : 0ull;
-            bits |= ((nib & 3ull) != 0 ? 1ull : 0ull) << b;
-            bits |= ((nib & 12ull) != 0 ? 1ull : 0ull) << (32 + b);
-          }
-          lv &= bits;
-        }
-        const bool live = lv != 0;
-        const uint64_t m = __ballot(live);
-        if (live) {
-          const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
-          s_list[pos] = st;
-          s_live[pos] = lv;
-        }
-        base += __popcll(m);
-      }
-      if (lane == 0) *s_count = base;
-    }
+    if (wave == 0) build_stage_list<CBW>(lane, nst, rt, flags, nkb, plan, gb0, s_list, s_live, s_count);
     __syncthreads();
     count = *s_count;
   }

@@ -439,7 +439,10 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
 // latency 242 cycles), matrix pipe 23% busy.  Measured slower: eight-
 // wavefront workgroups of a whole 64-row tile fetching each live fragment
 // once into LDS for both row halves (3.40 ms: the fragment reads, 160 KB
-// per stage and workgroup, then bind on LDS bandwidth).
+// per stage and workgroup, then bind on LDS bandwidth).  Not faster: the
+// whole 64-row tile per wavefront (half the fragment bytes per MFMA) in
+// eight-wavefront workgroups of 2-3 column blocks each, two wavefronts per
+// SIMD (2.39 ms).
 template <int SP, int CBW, int RB, bool NT>
 __device__ __forceinline__ void split_tile_live(int64_t bx, int64_t by, int64_t P, int64_t npt, int64_t K,
                                                 int64_t Ncol, const float *__restrict__ A, int64_t lda,

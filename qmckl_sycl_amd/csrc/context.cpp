@@ -262,6 +262,8 @@ qmckl_exit_code_device qmckl_context_destroy_device(qmckl_context_device context
   c->host_allocs.clear();
   if (c->batch_event) (void)hipEventDestroy(c->batch_event);
   c->batch_event = nullptr;
+  if (c->batch_jobs_pinned) (void)hipHostFree(c->batch_jobs_pinned);
+  c->batch_jobs_pinned = nullptr;
   if (c->stream) (void)hipStreamDestroy(c->stream);
   c->stream = nullptr;
   c->tag = kInvalidTag;

@@ -288,6 +288,8 @@ struct Context {
   MixedGemmJob *batch_jobs = nullptr;
   int64_t batch_jobs_cap = 0;
   std::vector<MixedGemmJob> batch_jobs_host;  // the last eager batch's table, as uploaded
+  void *batch_jobs_pinned = nullptr;  // pinned host staging of the table's upload
+  size_t batch_jobs_pinned_cap = 0;
   hipEvent_t batch_event = nullptr;
   struct TimerAcc {
     int64_t launches = 0;

@@ -703,8 +703,20 @@ hipError_t run_persistent_contractions(Context *c0, const std::vector<MixedGemmJ
   if (!grow_bytes(c0, reinterpret_cast<void **>(&c0->batch_jobs), &c0->batch_jobs_cap,
                   (int64_t)pp.table_bytes()))
     return hipErrorOutOfMemory;
-  hipError_t e = hipMemcpyAsync(c0->batch_jobs, pp.sorted.data(), sizeof(MixedGemmJob) * pp.sorted.size(),
-                                hipMemcpyHostToDevice, c0->stream);
+  // staged through pinned host memory: a copy from pageable memory is
+  // staged by the runtime (about 0.3 ms for a 48-molecule table on MI355X)
+  const size_t bytes = sizeof(MixedGemmJob) * pp.sorted.size();
+  hipError_t e = hipSuccess;
+  if (c0->batch_jobs_pinned_cap < bytes) {
+    if (c0->batch_jobs_pinned) (void)hipHostFree(c0->batch_jobs_pinned);
+    c0->batch_jobs_pinned = nullptr;
+    c0->batch_jobs_pinned_cap = 0;
+    e = hipHostMalloc(&c0->batch_jobs_pinned, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) return e;
+    c0->batch_jobs_pinned_cap = bytes;
+  }
+  std::memcpy(c0->batch_jobs_pinned, pp.sorted.data(), bytes);
+  e = hipMemcpyAsync(c0->batch_jobs, c0->batch_jobs_pinned, bytes, hipMemcpyHostToDevice, c0->stream);
   if (e == hipSuccess) e = launch_persistent(c0, pp, c0->batch_jobs, job_ctx);
   // the job table is read by the launches: wait before it can be reused
   if (e == hipSuccess) e = hipStreamSynchronize(c0->stream);

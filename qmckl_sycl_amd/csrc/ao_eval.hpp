@@ -13,10 +13,8 @@ namespace qmckl_amd {
 
 // Evaluates the l = L Cartesian AOs of one shell for this lane's point and
 // pushes them in the reference order (a descending, then b descending).
-template <int L, typename W>
-__device__ __attribute__((always_inline)) inline void shell_aos(W &w,
-                                                                const double *__restrict__ ao_factor,
-                                                                double x, double y, double z,
+template <int L, typename W, typename FP>
+__device__ __attribute__((always_inline)) inline void shell_aos(W &w, FP ao_factor, double x, double y, double z,
                                                                 const double (&R)[5]) {
   double px[L + 1], py[L + 1], pz[L + 1];
   px[0] = py[0] = pz[0] = 1.0;
@@ -75,10 +73,9 @@ __device__ inline void pow3(double x, int e, double &f0, double &f1, double &f2)
 // Shells of l > 3 (rare): the same AOs with the exponents as (wavefront-
 // uniform) loop counts instead of unrolled code, which keeps the register
 // footprint of the kernel at that of l <= 3.
-template <typename W>
-__device__ __attribute__((always_inline)) inline void shell_aos_any(W &w, const double *__restrict__ ao_factor,
-                                                        double x, double y, double z, const double (&R)[5],
-                                                        int l) {
+template <typename W, typename FP>
+__device__ __attribute__((always_inline)) inline void shell_aos_any(W &w, FP ao_factor, double x, double y,
+                                                                    double z, const double (&R)[5], int l) {
   const int k0 = w.k;
   int i = 0;
   for (int a = l; a >= 0; --a) {

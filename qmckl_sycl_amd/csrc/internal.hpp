@@ -173,6 +173,23 @@ struct ColPlanSlot {
   std::vector<uint64_t> host_blk;  // host copy of blk (when kept)
 };
 
+// One operand of launch_ao_rows_f32 (no residual array) as an item of a
+// batched launch: the arguments and the job's wavefronts [wave0, wave0 +
+// waves) of the launch (waves = point tiles x groups).
+struct AoRowsJob {
+  int64_t P, S, A, ld, npt;
+  int N, groups, nkb, skip_halves;
+  const double *coord;
+  const int32_t *perm;
+  const double *nucl;
+  const ShellDesc *shells;
+  const int32_t *stage_shell;
+  const double *ao_factor, *expo, *coef, *nucl_range;
+  float *buf, *rowmax;
+  uint8_t *flags;
+  int64_t wave0, waves;
+};
+
 // One molecule's split contraction in a persistent batch launch (the
 // arguments of launch_mo_gemm_mixed, the tile width and its work items).
 struct MixedGemmJob {
@@ -283,11 +300,17 @@ struct Context {
   // contraction of tile width <= 3 is appended here instead of launched (the
   // batch runs all of them in persistent launches)
   std::vector<MixedGemmJob> *collect_jobs = nullptr;
+  // ... and, with it, its AO operand here (the batch writes all of them in
+  // one launch)
+  std::vector<AoRowsJob> *collect_ao = nullptr;
   // the batch's job table and work counters (device), and an event marking
   // the end of this context's part of a batch
   MixedGemmJob *batch_jobs = nullptr;
   int64_t batch_jobs_cap = 0;
   std::vector<MixedGemmJob> batch_jobs_host;  // the last eager batch's table, as uploaded
+  AoRowsJob *batch_ao_jobs = nullptr;         // the batch's AO job table (device)
+  int64_t batch_ao_jobs_cap = 0;
+  std::vector<AoRowsJob> batch_ao_host;       // the last eager batch's AO table, as uploaded
   void *batch_jobs_pinned = nullptr;  // pinned host staging of the table's upload
   size_t batch_jobs_pinned_cap = 0;
   hipEvent_t batch_event = nullptr;
@@ -412,6 +435,14 @@ hipError_t tile_point_order(Context *c, const int32_t **perm);
 // kernel, which reads whole stages).  With buf_lo (same size as buf) each
 // value's FP32 residual v - (double)(float)v goes there as well.
 int ao_rows_groups(int64_t P, int64_t ld, int group_waves = 0);
+// Fills *job with the checks of launch_ao_rows_f32; wave0 is left to the
+// caller.
+hipError_t make_ao_rows_job(const DeviceBasis &b, int64_t P, const double *coord, const int32_t *perm,
+                            const double *nucl, const double *ao_factor, int64_t ld, int groups, float *buf,
+                            float *rowmax, uint8_t *flags, int nkb, bool skip_halves, AoRowsJob *job);
+// Every job's row maxima zeroed, then all jobs' operands in one launch of
+// `total` wavefronts; jobs (device memory) in order of wave0.
+hipError_t launch_ao_rows_batch(hipStream_t s, const AoRowsJob *jobs, int njobs, int64_t total);
 hipError_t launch_ao_rows_f32(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
                               const int32_t *perm, const double *nucl, const double *ao_factor, int64_t ld,
                               int groups, float *buf, float *rowmax, uint8_t *flags, int nkb,

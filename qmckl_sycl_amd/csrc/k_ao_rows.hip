@@ -188,20 +188,36 @@ __device__ __attribute__((always_inline)) inline void eval_stage(W &w, int64_t s
   }
 }
 
-template <bool LO>
-__global__ void __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads), amdgpu_waves_per_eu(LO ? 4 : 5)))
-ao_rows_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord, const int32_t *__restrict__ perm,
-               const double *__restrict__ nucl, const ShellDesc *__restrict__ shells,
-               const int32_t *__restrict__ stage_shell, const double *__restrict__ ao_factor,
-               const double *__restrict__ expo, const double *__restrict__ coef,
-               const double *__restrict__ nucl_range, int64_t A, int64_t ld, int64_t npt, int groups,
-               float *__restrict__ buf, float *__restrict__ buf_lo, float *__restrict__ rowmax,
-               uint8_t *__restrict__ flags, int nkb, int skip_halves) {
-  __shared__ float s_ring[kThreads / 64][5 * 4 * 64];
-  __shared__ float s_ring_lo[LO ? kThreads / 64 : 1][LO ? 5 * 4 * 64 : 1];
+// Pointer types of the read-only inputs: the per-context kernel's
+// arguments, or, in the batched kernel (whose pointers come from a table in
+// memory), the constant address space, so that the wavefront-uniform reads
+// of the basis tables stay scalar loads.
+struct ArgPtrs {
+  template <class T>
+  using p = const T *__restrict__;
+};
+struct TablePtrs {
+#if defined(__HIP_DEVICE_COMPILE__)
+  template <class T>
+  using p = const __attribute__((address_space(4))) T *;
+#else  // the host pass compiles the kernel bodies without address spaces
+  template <class T>
+  using p = const T *;
+#endif
+};
+
+// The work of wavefront wi (< npt * groups) of one operand: point tile
+// wi / groups, AO group wi % groups; ring / ring_lo are its LDS rings.
+template <bool LO, class PT>
+__device__ __attribute__((always_inline)) inline void ao_rows_wave(
+    int64_t wi, int64_t P, int N, int64_t S, typename PT::template p<double> coord,
+    typename PT::template p<int32_t> perm, typename PT::template p<double> nucl,
+    typename PT::template p<ShellDesc> shells, typename PT::template p<int32_t> stage_shell,
+    typename PT::template p<double> ao_factor, typename PT::template p<double> expo,
+    typename PT::template p<double> coef, typename PT::template p<double> nucl_range, int64_t A, int64_t ld,
+    int64_t npt, int groups, float *__restrict__ buf, float *__restrict__ buf_lo, float *__restrict__ rowmax,
+    uint8_t *__restrict__ flags, int nkb, int skip_halves, float *ring, float *ring_lo) {
   const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int64_t wi = (int64_t)blockIdx.x * (kThreads / 64) + wave;
   const int64_t pt = wi / groups;
   const int g = (int)(wi - pt * groups);
   if (pt >= npt) return;  // wavefront-uniform
@@ -224,8 +240,8 @@ ao_rows_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord, co
   w.flags = flags != nullptr ? flags + pt * nkb : nullptr;
   w.fstride = npt * nkb;
   w.lane = lane;
-  w.ring = s_ring[wave];
-  w.ring_lo = LO ? s_ring_lo[wave] : nullptr;
+  w.ring = ring;
+  w.ring_lo = ring_lo;
   w.base_lo = LO ? buf_lo + pt * (ld / 4) * 256 + lane * 4 : nullptr;
   w.hlive = 3u;
 
@@ -276,8 +292,8 @@ ao_rows_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord, co
       const double r2 = fma(x, x, fma(y, y, z * z));
       const bool live = inside && !(r2 > kCutoff * nucl_range[d.nucl]);
       double R[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-      const double *ex = expo + d.prim_start;
-      const double *cf = coef + d.prim_start;
+      const auto ex = expo + d.prim_start;
+      const auto cf = coef + d.prim_start;
       for (int j = 0; j < d.prim_num; ++j) {
         const double a = ex[j];
         const double ar2 = a * r2;
@@ -312,6 +328,58 @@ ao_rows_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord, co
 #pragma unroll
   for (int c = 0; c < 5; ++c)
     if (inside && w.mx[c] > 0.0f) atomicMax(rm + c * npt * 64, __float_as_uint(w.mx[c]));
+}
+
+template <bool LO>
+__global__ void __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads), amdgpu_waves_per_eu(LO ? 4 : 5)))
+ao_rows_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord, const int32_t *__restrict__ perm,
+               const double *__restrict__ nucl, const ShellDesc *__restrict__ shells,
+               const int32_t *__restrict__ stage_shell, const double *__restrict__ ao_factor,
+               const double *__restrict__ expo, const double *__restrict__ coef,
+               const double *__restrict__ nucl_range, int64_t A, int64_t ld, int64_t npt, int groups,
+               float *__restrict__ buf, float *__restrict__ buf_lo, float *__restrict__ rowmax,
+               uint8_t *__restrict__ flags, int nkb, int skip_halves) {
+  __shared__ float s_ring[kThreads / 64][5 * 4 * 64];
+  __shared__ float s_ring_lo[LO ? kThreads / 64 : 1][LO ? 5 * 4 * 64 : 1];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t wi = (int64_t)blockIdx.x * (kThreads / 64) + wave;
+  ao_rows_wave<LO, ArgPtrs>(wi, P, N, S, coord, perm, nucl, shells, stage_shell, ao_factor, expo, coef, nucl_range,
+                            A, ld, npt, groups, buf, buf_lo, rowmax, flags, nkb, skip_halves, s_ring[wave],
+                            LO ? s_ring_lo[wave] : nullptr);
+}
+
+// The operands of a batch of contexts in one launch: wavefront wi belongs to
+// the last job whose first wavefront is <= wi (jobs in order of wave0).
+// One launch instead of one per molecule: a small molecule's kernel runs a
+// few hundred wavefronts, each walking all shells of its stages, so the
+// per-molecule launches left the GPU mostly idle behind four hardware queues.
+__global__ void __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads), amdgpu_waves_per_eu(5)))
+ao_rows_batch_kernel(const AoRowsJob *__restrict__ jobs, int njobs, int64_t total) {
+  __shared__ float s_ring[kThreads / 64][5 * 4 * 64];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t wi = (int64_t)blockIdx.x * (kThreads / 64) + wave;
+  if (wi >= total) return;
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (jobs[mid].wave0 <= wi) lo = mid;
+    else hi = mid - 1;
+  }
+  const AoRowsJob &J = jobs[lo];
+  using K = TablePtrs;
+  ao_rows_wave<false, K>(wi - J.wave0, J.P, J.N, J.S, (K::p<double>)J.coord, (K::p<int32_t>)J.perm,
+                         (K::p<double>)J.nucl, (K::p<ShellDesc>)J.shells, (K::p<int32_t>)J.stage_shell,
+                         (K::p<double>)J.ao_factor, (K::p<double>)J.expo, (K::p<double>)J.coef,
+                         (K::p<double>)J.nucl_range, J.A, J.ld, J.npt, J.groups, J.buf, nullptr, J.rowmax, J.flags,
+                         J.nkb, J.skip_halves, s_ring[wave], nullptr);
+}
+
+// Zeroes the row maxima of every job (blockIdx.y = job).
+__global__ void __launch_bounds__(kThreads) fill_rowmax_batch_kernel(const AoRowsJob *__restrict__ jobs) {
+  const AoRowsJob &J = jobs[blockIdx.y];
+  const int64_t n = 5 * J.npt * 64;
+  uint32_t *p = reinterpret_cast<uint32_t *>(J.rowmax);
+  for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) p[i] = 0u;
 }
 
 __global__ void __launch_bounds__(kThreads) fill_zero_kernel(int64_t n, uint32_t *__restrict__ p) {
@@ -577,6 +645,50 @@ hipError_t launch_ao_rows_f32(hipStream_t s, const DeviceBasis &b, int64_t P, co
                        b.shell_num, coord, perm, nucl, b.shells, b.stage_shell, ao_factor, b.expo, b.coef_norm,
                        b.nucl_range, b.ao_num, ld, npt, groups, buf, nullptr, rowmax, flags, nkb,
                        skip_halves ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t make_ao_rows_job(const DeviceBasis &b, int64_t P, const double *coord, const int32_t *perm,
+                            const double *nucl, const double *ao_factor, int64_t ld, int groups, float *buf,
+                            float *rowmax, uint8_t *flags, int nkb, bool skip_halves, AoRowsJob *job) {
+  if (P <= 0) return hipErrorInvalidValue;
+  if (b.max_l > kMaxL || ld % 64 != 0 || ld < b.ao_num || groups < 1 || groups > ld / 64 ||
+      b.stage_shell == nullptr || ld > 0x7fffffff)
+    return hipErrorInvalidValue;
+  if (flags != nullptr && (int64_t)nkb * 32 != ld) return hipErrorInvalidValue;
+  job->P = P;
+  job->N = (int)b.nucl_num;
+  job->S = b.shell_num;
+  job->coord = coord;
+  job->perm = perm;
+  job->nucl = nucl;
+  job->shells = b.shells;
+  job->stage_shell = b.stage_shell;
+  job->ao_factor = ao_factor;
+  job->expo = b.expo;
+  job->coef = b.coef_norm;
+  job->nucl_range = b.nucl_range;
+  job->A = b.ao_num;
+  job->ld = ld;
+  job->npt = (P + 63) / 64;
+  job->groups = groups;
+  job->buf = buf;
+  job->rowmax = rowmax;
+  job->flags = flags;
+  job->nkb = flags != nullptr ? nkb : 0;
+  job->skip_halves = skip_halves ? 1 : 0;
+  job->wave0 = 0;
+  job->waves = job->npt * groups;
+  return hipSuccess;
+}
+
+hipError_t launch_ao_rows_batch(hipStream_t s, const AoRowsJob *jobs, int njobs, int64_t total) {
+  if (njobs <= 0 || total <= 0) return hipSuccess;
+  if (njobs > 65535) return hipErrorInvalidValue;
+  const int64_t blocks = (total + kThreads / 64 - 1) / (kThreads / 64);
+  if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(fill_rowmax_batch_kernel, dim3(64, (unsigned)njobs), dim3(kThreads), 0, s, jobs);
+  hipLaunchKernelGGL(ao_rows_batch_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, jobs, njobs, total);
   return hipGetLastError();
 }
 

@@ -252,18 +252,25 @@ qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Contex
   // flag is zero, so the AO kernel need not store such halves.
   const bool skip_halves =
       flags != nullptr && (i8 || form == QMCKL_PRECISION_FP32_DEVICE || gemm_tile != QMCKL_GEMM_TILE_DENSE_DEVICE);
-  if (e == hipSuccess) {
+  // In an eager batch, split forms of tile width <= 3 leave their AO
+  // operand to the batch's single AO launch and their contraction to its
+  // persistent launches (wider tiles need more registers than the
+  // persistent kernel has to spare).
+  const bool collect = c->collect_jobs != nullptr && form != QMCKL_PRECISION_FP32_DEVICE && !i8 &&
+                       split_wg_cols(form, M) <= 192;
+  if (e == hipSuccess && collect && c->collect_ao != nullptr) {
+    AoRowsJob aj{};
+    e = make_ao_rows_job(ao.dev, P, c->point.coord, perm, c->nucleus.coord, ao.ao_factor, ld,
+                         ao_rows_groups(P, ld, mo.tile_group_waves), ao.ao_f32, ao.ao_f32_rowmax, flags,
+                         flags ? nkb : 0, skip_halves, &aj);
+    if (e == hipSuccess) c->collect_ao->push_back(aj);
+  } else if (e == hipSuccess) {
     KernelTimer t(c, i8 ? "ao_vgl_f32_lo" : "ao_vgl_f32");
     e = launch_ao_rows_f32(c->stream, ao.dev, P, c->point.coord, perm, c->nucleus.coord, ao.ao_factor, ld,
                            ao_rows_groups(P, ld, mo.tile_group_waves), ao.ao_f32, ao.ao_f32_rowmax, flags,
                            flags ? nkb : 0, i8 ? ao.ao_f32_lo : nullptr, skip_halves);
     t.stop();
   }
-  // In an eager batch, split forms of tile width <= 3 leave their
-  // contraction to the batch's persistent launches (wider tiles need more
-  // registers than the persistent kernel has to spare).
-  const bool collect = c->collect_jobs != nullptr && form != QMCKL_PRECISION_FP32_DEVICE && !i8 &&
-                       split_wg_cols(form, M) <= 192;
   if (e == hipSuccess && i8) {
     KernelTimer t(c, gemm_timer_name(form));
     e = launch_mo_gemm_i8(c->stream, P, A, M, ao.ao_f32, ao.ao_f32_lo, ld, mo.coef_i8, ld, mo.coef_colinv, flags,
@@ -663,6 +670,28 @@ PersistentPlan plan_persistent(const std::vector<MixedGemmJob> &jobs, const std:
   return pp;
 }
 
+// The AO operands a batch collected, largest first (so that the small ones
+// fill the end of the launch), with their first wavefronts; returns the
+// launch's wavefronts.
+int64_t plan_ao_jobs(std::vector<AoRowsJob> &ao) {
+  std::stable_sort(ao.begin(), ao.end(), [](const AoRowsJob &x, const AoRowsJob &y) {
+    return x.npt * x.ld > y.npt * y.ld;
+  });
+  int64_t total = 0;
+  for (AoRowsJob &j : ao) {
+    j.wave0 = total;
+    total += j.waves;
+  }
+  return total;
+}
+
+bool same_ao_jobs(const std::vector<AoRowsJob> &x, const std::vector<AoRowsJob> &y) {
+  if (x.size() != y.size()) return false;
+  for (size_t k = 0; k < x.size(); ++k)
+    if (std::memcmp(&x[k], &y[k], sizeof(AoRowsJob)) != 0) return false;
+  return true;
+}
+
 bool same_job(const MixedGemmJob &x, const MixedGemmJob &y) {
   return x.P == y.P && x.npt == y.npt && x.K == y.K && x.Ncol == y.Ncol && x.A == y.A && x.lda == y.lda &&
          x.rowmax == y.rowmax && x.Bs == y.Bs && x.ldb == y.ldb && x.colinv == y.colinv && x.flags == y.flags &&
@@ -672,18 +701,37 @@ bool same_job(const MixedGemmJob &x, const MixedGemmJob &y) {
          x.item0 == y.item0 && x.gx == y.gx && x.items == y.items;
 }
 
-// Enqueues the plan's launches on c0's stream after every collecting
-// context's work; table holds the sorted jobs followed by one int counter per
-// group (already on the device).
+// Enqueues on c0's stream, after every collecting context's own work (its
+// point order and coefficient preparation), the batch's AO operands in one
+// launch (ao_table: njobs_ao jobs, ao_waves wavefronts) and then the plan's
+// persistent contractions; table holds the sorted jobs followed by one int
+// counter per group.  Both tables are already on the device.
+// Eager (not captured): a context whose stream has nothing pending is not
+// waited for (the event record and wait of 47 contexts took about 0.6 ms of
+// host time, during which the GPU idled).
 hipError_t launch_persistent(Context *c0, const PersistentPlan &pp, MixedGemmJob *table,
-                             const std::vector<Context *> &job_ctx) {
+                             const std::vector<Context *> &job_ctx, const AoRowsJob *ao_table, int njobs_ao,
+                             int64_t ao_waves, bool eager) {
   int *counters = reinterpret_cast<int *>(reinterpret_cast<char *>(table) + sizeof(MixedGemmJob) * pp.sorted.size());
   hipError_t e = hipSuccess;
   for (Context *c : job_ctx) {
     if (e != hipSuccess || c == c0) continue;
+    if (eager) {
+      const hipError_t q = hipStreamQuery(c->stream);
+      if (q == hipSuccess) continue;
+      if (q != hipErrorNotReady) {
+        e = q;
+        continue;
+      }
+    }
     if (c->batch_event == nullptr) e = hipEventCreateWithFlags(&c->batch_event, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventRecord(c->batch_event, c->stream);
     if (e == hipSuccess) e = hipStreamWaitEvent(c0->stream, c->batch_event, 0);
+  }
+  if (e == hipSuccess && njobs_ao > 0) {
+    KernelTimer t(c0, "ao_rows_batch");
+    e = launch_ao_rows_batch(c0->stream, ao_table, njobs_ao, ao_waves);
+    t.stop();
   }
   for (size_t g = 0; g < pp.groups.size() && e == hipSuccess; ++g) {
     KernelTimer t(c0, "mo_gemm_batch_persistent");
@@ -695,17 +743,23 @@ hipError_t launch_persistent(Context *c0, const PersistentPlan &pp, MixedGemmJob
   return e;
 }
 
-// Eager batch: the table goes to c0's memory (its host copy stays in c0 for
-// a graph built from this batch), the launches complete before returning.
+// Eager batch: the tables go to c0's memory (their host copies stay in c0
+// for a graph built from this batch), the launches complete before
+// returning.
 hipError_t run_persistent_contractions(Context *c0, const std::vector<MixedGemmJob> &jobs,
-                                       const std::vector<Context *> &job_ctx) {
+                                       const std::vector<Context *> &job_ctx, std::vector<AoRowsJob> ao_jobs) {
   const PersistentPlan pp = plan_persistent(jobs, job_ctx);
+  const int64_t ao_waves = plan_ao_jobs(ao_jobs);
+  const size_t gemm_bytes = sizeof(MixedGemmJob) * pp.sorted.size();
+  const size_t ao_bytes = sizeof(AoRowsJob) * ao_jobs.size();
   if (!grow_bytes(c0, reinterpret_cast<void **>(&c0->batch_jobs), &c0->batch_jobs_cap,
-                  (int64_t)pp.table_bytes()))
+                  (int64_t)pp.table_bytes()) ||
+      !grow_bytes(c0, reinterpret_cast<void **>(&c0->batch_ao_jobs), &c0->batch_ao_jobs_cap,
+                  (int64_t)std::max<size_t>(ao_bytes, 1)))
     return hipErrorOutOfMemory;
   // staged through pinned host memory: a copy from pageable memory is
   // staged by the runtime (about 0.3 ms for a 48-molecule table on MI355X)
-  const size_t bytes = sizeof(MixedGemmJob) * pp.sorted.size();
+  const size_t bytes = gemm_bytes + ao_bytes;
   hipError_t e = hipSuccess;
   if (c0->batch_jobs_pinned_cap < bytes) {
     if (c0->batch_jobs_pinned) (void)hipHostFree(c0->batch_jobs_pinned);
@@ -715,12 +769,21 @@ hipError_t run_persistent_contractions(Context *c0, const std::vector<MixedGemmJ
     if (e != hipSuccess) return e;
     c0->batch_jobs_pinned_cap = bytes;
   }
-  std::memcpy(c0->batch_jobs_pinned, pp.sorted.data(), bytes);
-  e = hipMemcpyAsync(c0->batch_jobs, c0->batch_jobs_pinned, bytes, hipMemcpyHostToDevice, c0->stream);
-  if (e == hipSuccess) e = launch_persistent(c0, pp, c0->batch_jobs, job_ctx);
-  // the job table is read by the launches: wait before it can be reused
+  char *staging = static_cast<char *>(c0->batch_jobs_pinned);
+  std::memcpy(staging, pp.sorted.data(), gemm_bytes);
+  if (ao_bytes) std::memcpy(staging + gemm_bytes, ao_jobs.data(), ao_bytes);
+  e = hipMemcpyAsync(c0->batch_jobs, staging, gemm_bytes, hipMemcpyHostToDevice, c0->stream);
+  if (e == hipSuccess && ao_bytes)
+    e = hipMemcpyAsync(c0->batch_ao_jobs, staging + gemm_bytes, ao_bytes, hipMemcpyHostToDevice, c0->stream);
+  if (e == hipSuccess)
+    e = launch_persistent(c0, pp, c0->batch_jobs, job_ctx, c0->batch_ao_jobs, (int)ao_jobs.size(), ao_waves,
+                          true);
+  // the job tables are read by the launches: wait before they can be reused
   if (e == hipSuccess) e = hipStreamSynchronize(c0->stream);
-  if (e == hipSuccess) c0->batch_jobs_host = pp.sorted;
+  if (e == hipSuccess) {
+    c0->batch_jobs_host = pp.sorted;
+    c0->batch_ao_host = ao_jobs;
+  }
   return e;
 }
 
@@ -759,16 +822,20 @@ qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_batch_device(int64_t count,
   qmckl_exit_code_device first = QMCKL_SUCCESS_DEVICE;
   int64_t enqueued = 0;
   std::vector<MixedGemmJob> jobs;
+  std::vector<AoRowsJob> ao_jobs;
   std::vector<Context *> job_ctx;
   c0->batch_jobs_host.clear();
+  c0->batch_ao_host.clear();
   for (; enqueued < count; ++enqueued) {
     Context *c = ctx_of(contexts[enqueued]);
     c->defer_sync = true;
     const size_t before = jobs.size();
     c->collect_jobs = &jobs;
+    c->collect_ao = &ao_jobs;
     qmckl_exit_code_device rc = qmckl_context_touch_device(contexts[enqueued]);
     if (rc == QMCKL_SUCCESS_DEVICE) rc = compute_mo_vgl(contexts[enqueued], c, mo_vgl[enqueued], fn);
     c->collect_jobs = nullptr;
+    c->collect_ao = nullptr;
     c->defer_sync = false;
     if (jobs.size() > before) job_ctx.push_back(c);
     if (rc != QMCKL_SUCCESS_DEVICE) {
@@ -778,7 +845,7 @@ qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_batch_device(int64_t count,
     }
   }
   if (first == QMCKL_SUCCESS_DEVICE && !jobs.empty()) {
-    const hipError_t e = run_persistent_contractions(c0, jobs, job_ctx);
+    const hipError_t e = run_persistent_contractions(c0, jobs, job_ctx, ao_jobs);
     if (e != hipSuccess) first = hip_check(contexts[0], e, fn);
   }
   for (int64_t i = 0; i < enqueued; ++i) {
@@ -1004,6 +1071,7 @@ struct BatchGraph {
   std::vector<qmckl_context_device> contexts;
   std::vector<std::vector<uint64_t>> signature;
   MixedGemmJob *table = nullptr;  // the persistent launches' job table and counters (owned)
+  AoRowsJob *ao_table = nullptr;  // the AO launch's job table (owned)
 };
 
 std::mutex g_graph_mutex;
@@ -1056,11 +1124,17 @@ qmckl_exit_code_device qmckl_mo_basis_mo_vgl_batch_graph_create_device(int64_t c
   // the eager run's persistent job table, copied to memory the graph owns
   // (a later batch may reuse the context's table)
   const std::vector<MixedGemmJob> eager_jobs = c0->batch_jobs_host;
+  const std::vector<AoRowsJob> eager_ao = c0->batch_ao_host;
   if (e == hipSuccess && !eager_jobs.empty()) {
     const size_t bytes = sizeof(MixedGemmJob) * eager_jobs.size() + 64 * sizeof(int);
     e = hipMalloc(reinterpret_cast<void **>(&g->table), bytes);
     if (e == hipSuccess)
       e = hipMemcpy(g->table, eager_jobs.data(), sizeof(MixedGemmJob) * eager_jobs.size(), hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess && !eager_ao.empty()) {
+    e = hipMalloc(reinterpret_cast<void **>(&g->ao_table), sizeof(AoRowsJob) * eager_ao.size());
+    if (e == hipSuccess)
+      e = hipMemcpy(g->ao_table, eager_ao.data(), sizeof(AoRowsJob) * eager_ao.size(), hipMemcpyHostToDevice);
   }
   hipEvent_t fork = nullptr;
   std::vector<hipEvent_t> joins((size_t)count, nullptr);
@@ -1076,6 +1150,7 @@ qmckl_exit_code_device qmckl_mo_basis_mo_vgl_batch_graph_create_device(int64_t c
   // joined back; split-form contractions are collected as in the eager run
   // and recorded as its persistent launches on the first context's stream
   std::vector<MixedGemmJob> jobs;
+  std::vector<AoRowsJob> ao_jobs;
   std::vector<Context *> job_ctx;
   for (int64_t i = 0; i < count && e == hipSuccess && rc == QMCKL_SUCCESS_DEVICE; ++i) {
     Context *c = ctx_of(contexts[i]);
@@ -1085,9 +1160,11 @@ qmckl_exit_code_device qmckl_mo_basis_mo_vgl_batch_graph_create_device(int64_t c
     c->capturing = true;
     const size_t before = jobs.size();
     c->collect_jobs = &jobs;
+    c->collect_ao = &ao_jobs;
     rc = qmckl_context_touch_device(contexts[i]);
     if (rc == QMCKL_SUCCESS_DEVICE) rc = compute_mo_vgl(contexts[i], c, mo_vgl[i], fn);
     c->collect_jobs = nullptr;
+    c->collect_ao = nullptr;
     c->defer_sync = false;
     c->capturing = false;
     if (jobs.size() > before) job_ctx.push_back(c);
@@ -1097,12 +1174,15 @@ qmckl_exit_code_device qmckl_mo_basis_mo_vgl_batch_graph_create_device(int64_t c
   hipEvent_t pjoin = nullptr;
   if (e == hipSuccess && rc == QMCKL_SUCCESS_DEVICE && !jobs.empty()) {
     const PersistentPlan pp = plan_persistent(jobs, job_ctx);
-    // the capture must record exactly the table the eager run uploaded
-    bool same = g->table != nullptr && pp.sorted.size() == eager_jobs.size() && pp.groups.size() <= 64;
+    const int64_t ao_waves = plan_ao_jobs(ao_jobs);
+    // the capture must record exactly the tables the eager run uploaded
+    bool same = g->table != nullptr && pp.sorted.size() == eager_jobs.size() && pp.groups.size() <= 64 &&
+                same_ao_jobs(ao_jobs, eager_ao) && (ao_jobs.empty() || g->ao_table != nullptr);
     for (size_t k = 0; same && k < pp.sorted.size(); ++k) same = same_job(pp.sorted[k], eager_jobs[k]);
     if (!same) e = hipErrorInvalidValue;
     c0->capturing = true;  // no kernel timers inside the capture
-    if (e == hipSuccess) e = launch_persistent(c0, pp, g->table, job_ctx);
+    if (e == hipSuccess)
+      e = launch_persistent(c0, pp, g->table, job_ctx, g->ao_table, (int)ao_jobs.size(), ao_waves, false);
     c0->capturing = false;
     if (e == hipSuccess) e = hipEventCreateWithFlags(&pjoin, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventRecord(pjoin, c0->stream);
@@ -1128,6 +1208,7 @@ qmckl_exit_code_device qmckl_mo_basis_mo_vgl_batch_graph_create_device(int64_t c
   if (g->graph) (void)hipGraphDestroy(g->graph);
   if (g->stream) (void)hipStreamDestroy(g->stream);
   if (g->table) (void)hipFree(g->table);
+  if (g->ao_table) (void)hipFree(g->ao_table);
   delete g;
   (void)hipGetLastError();
   if (rc != QMCKL_SUCCESS_DEVICE) return rc;
@@ -1173,6 +1254,7 @@ qmckl_exit_code_device qmckl_mo_basis_mo_vgl_batch_graph_destroy_device(int64_t 
   (void)hipGraphDestroy(g->graph);
   (void)hipStreamDestroy(g->stream);
   if (g->table) (void)hipFree(g->table);
+  if (g->ao_table) (void)hipFree(g->ao_table);
   delete g;
   return QMCKL_SUCCESS_DEVICE;
 }

@@ -18,7 +18,9 @@
 // Also written: rowmax[c][tile row] = max |entry| of the row (each lane owns
 // its five rows, so a wavefront needs no reduction; the AO groups of a tile
 // combine theirs with atomic integer maxima of the bit patterns), and
-// nonzero flags per (row tile, 32-AO block) gathered with wavefront ballots.
+// flags per (row tile, 32-AO block): 1 where some shell overlapping the
+// block passes the screen for some point of the tile (the same flag for the
+// five components), 0 where every value of the block is zero by the screen.
 //
 // The 64 points of tile t are perm[64t .. 64t+63] (the caller orders the
 // points so that a tile is spatially compact).  With flags, a 64-AO stage
@@ -51,7 +53,7 @@ namespace {
 constexpr int kThreads = 256;
 
 // Running state of one lane: the AO counter of the wavefront, row maxima
-// and nonzero bits of the current 32-AO block.  Values are staged in the
+// and the screen of the current stage's 32-AO halves.  Values are staged in the
 // wavefront's LDS ring [component][AO mod 4][lane] (the slot is a runtime
 // index, which registers cannot take without spilling); after every fourth
 // AO each lane reads its four values back and stores them as one 16-byte
@@ -62,7 +64,7 @@ constexpr int kThreads = 256;
 template <bool LO>
 struct RowWriter {
   float mx[5];
-  uint32_t nz;       // bit c: some AO of the current 32-block is nonzero in row c
+  uint32_t hscreen;  // bit h: half h of the current stage passes the screen (its flag)
   int k;             // next AO index (wavefront-uniform)
   float *base;       // &buf[0][pt][0][lane][0]
   int64_t cstride;   // floats between components
@@ -88,7 +90,6 @@ struct RowWriter {
       ring[(c * 4 + slot) * 64 + lane] = w;
       if constexpr (LO) ring_lo[(c * 4 + slot) * 64 + lane] = (float)(v[c] - (double)w);
       mx[c] = fmaxf(mx[c], fabsf(w));
-      nz |= (w != 0.0f ? 1u : 0u) << c;
     }
     if (slot == 3 && ((hlive >> ((k >> 5) & 1)) & 1u)) {
       float *o = base + (int64_t)(k >> 2) * 256;
@@ -106,16 +107,10 @@ struct RowWriter {
                           ring_lo[(c * 4 + 2) * 64 + lane], ring_lo[(c * 4 + 3) * 64 + lane]);
       }
     }
-    if ((k & 31) == 31) {
-      if (flags != nullptr) {
-#pragma unroll
-        for (int c = 0; c < 5; ++c) {
-          const bool any = __ballot((nz >> c) & 1u) != 0;
-          if (lane == 0) flags[c * fstride + (k >> 5)] = any ? 1 : 0;
-        }
-      }
-      nz = 0;
-    }
+    // the flags of a finished 32-AO block, from the screen (per-value
+    // nonzero tests cost three VALU instructions per value)
+    if ((k & 31) == 31 && flags != nullptr && lane < 5)
+      flags[lane * fstride + (k >> 5)] = (uint8_t)((hscreen >> ((k >> 5) & 1)) & 1u);
     ++k;
   }
 };
@@ -234,7 +229,7 @@ __device__ __attribute__((always_inline)) inline void ao_rows_wave(
   RowWriter<LO> w;
 #pragma unroll
   for (int c = 0; c < 5; ++c) w.mx[c] = 0.0f;
-  w.nz = 0;
+  w.hscreen = 3u;
   w.cstride = npt * (ld / 4) * 256;
   w.base = buf + pt * (ld / 4) * 256 + lane * 4;
   w.flags = flags != nullptr ? flags + pt * nkb : nullptr;
@@ -278,7 +273,8 @@ __device__ __attribute__((always_inline)) inline void ao_rows_wave(
         if (lane < 10) flags[pt * nkb + (lane >> 1) * w.fstride + 2 * st + (lane & 1)] = 0;
         continue;
       }
-      w.hlive = skip_halves ? (h0 ? 1u : 0u) | (h1 ? 2u : 0u) : 3u;
+      w.hscreen = (h0 ? 1u : 0u) | (h1 ? 2u : 0u);
+      w.hlive = skip_halves ? w.hscreen : 3u;
     }
     w.k_lo = (int)(64 * st);
     w.k_hi = k_end;

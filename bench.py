@@ -113,7 +113,7 @@ def run_form(ctx, s, form, steps, warmup):
     return (t1 - t0) / steps, kernels, mo
 
 
-def bench_batch(forms, steps, warmup, scale=1.0):
+def bench_batch(forms, steps, warmup, scale=1.0, eager_only=False):
     """Batching form: the 48-molecule ragged batch (2-64 nuclei, 30-1,000 AOs,
     16-512 walkers) through qmckl_get_mo_basis_mo_vgl_batch_device, one
     context per molecule, timed per form with the default tile choices;
@@ -153,6 +153,9 @@ def bench_batch(forms, steps, warmup, scale=1.0):
                 c.check(L.qmckl_set_mo_basis_precision_device(c.handle, FORM_IDS[form]))
             sec = timed(lambda: batch_mo_vgl(ctxs, outs))
             res[form] = {"ms_per_step": sec * 1e3, "value": points / sec}
+            if eager_only:
+                log("batch", form, json.dumps(res[form]))
+                continue
 
             def one_by_one():
                 for c, o in zip(ctxs, outs):
@@ -644,6 +647,9 @@ def main():
                     help="skip the 48-molecule ragged batch measurement")
     ap.add_argument("--batch-only", action="store_true",
                     help="only run the 48-molecule ragged batch measurement")
+    ap.add_argument("--batch-eager", action="store_true",
+                    help="with --batch-only: only the eager batch calls (no per-molecule calls, graph or "
+                         "autotuner), for counter runs")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only time the adjoint (no forward pass) at the differentiation workload")
     ap.add_argument("--traffic", action="store_true",
@@ -667,8 +673,9 @@ def main():
         print(json.dumps({"adjoint_c5": bench_adjoint(args.steps, args.warmup, forward=False)}), flush=True)
         return
     if args.batch_only:
-        print(json.dumps({"batch_48_ragged": bench_batch(["plain", args.form], args.steps, args.warmup)}),
-              flush=True)
+        forms = [args.form] if args.no_plain else ["plain", args.form]
+        print(json.dumps({"batch_48_ragged": bench_batch(forms, args.steps, args.warmup,
+                                                         eager_only=args.batch_eager)}), flush=True)
         return
 
     from qmckl_sycl_amd.context import Context

@@ -464,9 +464,10 @@ qmckl_exit_code_device qmckl_mo_basis_autotune_device(qmckl_context_device conte
  * AOs are neither loaded nor multiplied, the AO operand is streamed with
  * nontemporal loads; LIVE_CACHED (2): the same with cached AO loads; DENSE
  * (3): every block of a live 64-AO stage is multiplied, dead ones on zeros;
- * LIVE64 (4): LIVE with 64 rows per wavefront (one wavefront per SIMD).
- * These four contract the stored FP32 AO operand and give the same results
- * bit for bit.  FUSED (5): one kernel evaluates the AOs of 64 points into
+ * LIVE64 (4): LIVE with 64 rows per wavefront (one wavefront per SIMD);
+ * DENSE64 (6): DENSE with workgroups of a whole 64-row tile (eight
+ * wavefronts).  These five contract the stored FP32 AO operand and give the
+ * same results bit for bit.  FUSED (5): one kernel evaluates the AOs of 64 points into
  * local memory and contracts them with 64 MOs (each group of 64 MOs
  * evaluates the AOs again); it allocates no AO operand (about 20 bytes per
  * point and AO) and its results round
@@ -479,14 +480,15 @@ qmckl_exit_code_device qmckl_mo_basis_autotune_device(qmckl_context_device conte
  * 1024 AOs or more) and DENSE otherwise.  The FP32 and INT8X5 forms have
  * one kernel each and ignore the choice.  set: ARG_2 for an unknown value;
  * get: ARG_2 for NULL.  The autotuner above also times LIVE, LIVE_CACHED,
- * DENSE and LIVE64 (after the point order and AO wavefronts) and keeps the
- * fastest; a context set to FUSED keeps FUSED. */
+ * DENSE, LIVE64 and DENSE64 (after the point order and AO wavefronts) and
+ * keeps the fastest; a context set to FUSED keeps FUSED. */
 #define QMCKL_GEMM_TILE_AUTO_DEVICE ((int32_t)0)
 #define QMCKL_GEMM_TILE_LIVE_DEVICE ((int32_t)1)
 #define QMCKL_GEMM_TILE_LIVE_CACHED_DEVICE ((int32_t)2)
 #define QMCKL_GEMM_TILE_DENSE_DEVICE ((int32_t)3)
 #define QMCKL_GEMM_TILE_LIVE64_DEVICE ((int32_t)4)
 #define QMCKL_GEMM_TILE_FUSED_DEVICE ((int32_t)5)
+#define QMCKL_GEMM_TILE_DENSE64_DEVICE ((int32_t)6)
 qmckl_exit_code_device qmckl_set_mo_basis_gemm_tile_device(qmckl_context_device context, int32_t gemm_tile);
 qmckl_exit_code_device qmckl_get_mo_basis_gemm_tile_device(qmckl_context_device context, int32_t *gemm_tile);
 

@@ -897,12 +897,11 @@ hipError_t launch_f32(hipStream_t s, int64_t P, int64_t npt, int64_t K, int64_t 
   return hipGetLastError();
 }
 
-template <int SP, int CBW>
+template <int SP, int CBW, int WR = 32>
 hipError_t launch_split(hipStream_t s, int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *A,
                         int64_t lda, const float *rowmax, const void *B, int64_t ldb,
                         const float *colinv, const uint8_t *flags, int nkb, const int32_t *perm, double *C,
                         int64_t ldc, ColPlan plan) {
-  constexpr int WR = 32;
   const int64_t gx = 5 * npt * (64 / WR), gy = (Ncol + 64 * CBW - 1) / (64 * CBW);
   if (gx > 0x7fffffff || gy > 65535 || (lda % kStage) != 0 || (ldb % kStage) != 0)
     return hipErrorInvalidValue;
@@ -951,6 +950,16 @@ hipError_t launch_form(hipStream_t s, int64_t P, int64_t npt, int64_t K, int64_t
     return launch_f32<5>(s, P, npt, K, Ncol, A, lda, B, ldb, flags, nkb, perm, C, ldc);
   } else {
     const int cbw = split_wg_cols(SP, Ncol) / 64;
+    if (gemm_tile == QMCKL_GEMM_TILE_DENSE64_DEVICE) {
+      if (cbw == 1) return launch_split<SP, 1, 64>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc, plan);
+      if (cbw == 2) return launch_split<SP, 2, 64>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc, plan);
+      if (cbw == 3) return launch_split<SP, 3, 64>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc, plan);
+      if constexpr (Terms<SP>::n == 2) {
+        if (cbw == 4) return launch_split<SP, 4, 64>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc, plan);
+        if (cbw == 5) return launch_split<SP, 5, 64>(s, P, npt, K, Ncol, A, lda, rowmax, B, ldb, colinv, flags, nkb, perm, C, ldc, plan);
+      }
+      return hipErrorInvalidValue;
+    }
     if (gemm_tile != QMCKL_GEMM_TILE_DENSE_DEVICE) {
 #define QMCKL_LIVE(W)                                                                                              \
   switch (gemm_tile) {                                                                                            \
@@ -1050,7 +1059,7 @@ hipError_t make_mixed_gemm_job(int form, int64_t P, int64_t K, int64_t Ncol, con
   job->ldc = ldc;
   job->plan = plan;
   job->cbw = cbw;
-  job->dense = gemm_tile == QMCKL_GEMM_TILE_DENSE_DEVICE ? 1 : 0;
+  job->dense = gemm_tile == QMCKL_GEMM_TILE_DENSE_DEVICE || gemm_tile == QMCKL_GEMM_TILE_DENSE64_DEVICE ? 1 : 0;
   job->item0 = 0;
   job->gx = 5 * npt * 2;  // 32-row halves of the 64-row tiles
   job->items = job->gx * ((Ncol + 64 * cbw - 1) / (64 * cbw));

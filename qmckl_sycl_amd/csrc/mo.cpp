@@ -251,7 +251,8 @@ qmckl_exit_code_device compute_mo_vgl_mixed(qmckl_context_device context, Contex
   // Every contraction but the dense split kernel reads no 32-AO half whose
   // flag is zero, so the AO kernel need not store such halves.
   const bool skip_halves =
-      flags != nullptr && (i8 || form == QMCKL_PRECISION_FP32_DEVICE || gemm_tile != QMCKL_GEMM_TILE_DENSE_DEVICE);
+      flags != nullptr && (i8 || form == QMCKL_PRECISION_FP32_DEVICE ||
+                           (gemm_tile != QMCKL_GEMM_TILE_DENSE_DEVICE && gemm_tile != QMCKL_GEMM_TILE_DENSE64_DEVICE));
   // In an eager batch, split forms of tile width <= 3 leave their AO
   // operand to the batch's single AO launch and their contraction to its
   // persistent launches (wider tiles need more registers than the
@@ -908,7 +909,7 @@ qmckl_exit_code_device qmckl_set_mo_basis_tile_config_device(qmckl_context_devic
 qmckl_exit_code_device qmckl_set_mo_basis_gemm_tile_device(qmckl_context_device context, int32_t gemm_tile) {
   Context *c = ctx_of(context);
   if (!c) return QMCKL_INVALID_CONTEXT_DEVICE;
-  if (gemm_tile < QMCKL_GEMM_TILE_AUTO_DEVICE || gemm_tile > QMCKL_GEMM_TILE_FUSED_DEVICE)
+  if (gemm_tile < QMCKL_GEMM_TILE_AUTO_DEVICE || gemm_tile > QMCKL_GEMM_TILE_DENSE64_DEVICE)
     return fail(context, QMCKL_INVALID_ARG_2_DEVICE, "qmckl_set_mo_basis_gemm_tile_device",
                 "gemm_tile not one of QMCKL_GEMM_TILE_*");
   c->mo.tile_gemm = gemm_tile;
@@ -1031,7 +1032,7 @@ qmckl_exit_code_device qmckl_mo_basis_autotune_device(qmckl_context_device conte
     mo.tile_point_order = best_order;
     mo.tile_group_waves = best_waves;
     for (int32_t g : {QMCKL_GEMM_TILE_LIVE_DEVICE, QMCKL_GEMM_TILE_LIVE_CACHED_DEVICE, QMCKL_GEMM_TILE_DENSE_DEVICE,
-                      QMCKL_GEMM_TILE_LIVE64_DEVICE}) {
+                      QMCKL_GEMM_TILE_LIVE64_DEVICE, QMCKL_GEMM_TILE_DENSE64_DEVICE}) {
       mo.tile_gemm = g;
       float ms = 0.0f;
       if (!(ok = time_chain(&ms))) break;

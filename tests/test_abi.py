@@ -347,7 +347,8 @@ def test_tile_config_and_autotune_validation(ctx):
     assert L.qmckl_set_mo_basis_tile_config_device(0, 0, 0) == INVALID_CONTEXT
     gt = ctypes.c_int32(9)
     assert L.qmckl_get_mo_basis_gemm_tile_device(ctx.handle, ctypes.byref(gt)) == 0 and gt.value == 0
-    assert L.qmckl_set_mo_basis_gemm_tile_device(ctx.handle, 6) == 2
+    assert L.qmckl_set_mo_basis_gemm_tile_device(ctx.handle, 7) == 2
+    assert L.qmckl_set_mo_basis_gemm_tile_device(ctx.handle, 6) == 0
     assert L.qmckl_set_mo_basis_gemm_tile_device(ctx.handle, 5) == 0
     assert L.qmckl_set_mo_basis_gemm_tile_device(ctx.handle, -1) == 2
     assert L.qmckl_set_mo_basis_gemm_tile_device(ctx.handle, 3) == 0

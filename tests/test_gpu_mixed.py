@@ -134,7 +134,7 @@ def test_tile_choices_and_autotune_keep_results(form):
                 c.touch()
                 assert np.array_equal(c.mo_vgl(P), ref), (order, waves)
         c.call("qmckl_set_mo_basis_tile_config_device", -1, 0)
-        for gemm in (1, 2, 3, 4):  # LIVE, LIVE_CACHED, DENSE, LIVE64
+        for gemm in (1, 2, 3, 4, 6):  # LIVE, LIVE_CACHED, DENSE, LIVE64, DENSE64
             c.call("qmckl_set_mo_basis_gemm_tile_device", gemm)
             c.touch()
             assert np.array_equal(c.mo_vgl(P), ref), gemm
@@ -144,7 +144,7 @@ def test_tile_choices_and_autotune_keep_results(form):
         c.call("qmckl_get_mo_basis_tile_config_device", ctypes.byref(po), ctypes.byref(gw))
         c.call("qmckl_get_mo_basis_gemm_tile_device", ctypes.byref(gt))
         say(f"autotuned {form}: point_order={po.value} group_waves={gw.value} gemm_tile={gt.value}")
-        assert po.value in (0, 1) and gw.value in (16, 32, 64, 128, 256) and gt.value in (0, 1, 2, 3, 4)
+        assert po.value in (0, 1) and gw.value in (16, 32, 64, 128, 256) and gt.value in (0, 1, 2, 3, 4, 6)
         assert np.array_equal(c.mo_vgl(P), ref)
 
 

@@ -23,11 +23,31 @@ def rel_err_vs_mag(got, ref, mag):
     return r.reshape(r.shape[0], r.shape[1], -1).max(axis=(0, 2))
 
 
+# Entries whose terms cancel by less than this factor (|ref| >= COND * mag)
+# are also held to the bound element by element, |got - ref| <= rtol |ref|.
+COND = 1e-3
+
+
+def elementwise_rel_err(got, ref, mag, cond=COND):
+    """(max |got - ref| / |ref| per VGL component over the entries with
+    |ref| >= cond * mag, fraction of entries that qualify)."""
+    d = np.abs(got - ref)
+    a = np.abs(ref)
+    ok = (a >= cond * mag) & (a > 0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r = np.where(ok, d / np.where(ok, a, 1.0), 0.0)
+    return r.reshape(r.shape[0], r.shape[1], -1).max(axis=(0, 2)), float(ok.mean())
+
+
 def assert_vgl_close(got, ref, mag, rtol=RTOL_VGL, what=""):
     assert got.shape == ref.shape, (got.shape, ref.shape)
     err = rel_err_vs_mag(got, ref, mag)
-    say(f"{what}: max |d|/mag per component = {np.array2string(err, precision=2)}")
+    el, frac = elementwise_rel_err(got, ref, mag)
+    say(f"{what}: max |d|/mag per component = {np.array2string(err, precision=2)}; "
+        f"max |d|/|ref| over the {frac:.1%} of entries with |ref| >= {COND:g} mag = "
+        f"{np.array2string(el, precision=2)}")
     assert np.all(err <= rtol), (what, err)
+    assert np.all(el <= rtol), (what, "elementwise", el)
     return err
 
 

@@ -137,6 +137,33 @@ __device__ __attribute__((always_inline)) inline bool stage_live(int64_t s_first
   return __ballot(any) != 0;
 }
 
+// This lane's screen of the 64-AO stage [k_lo, k_end) from shell s_first
+// on, per 32-AO half: bit 0 (1) when some shell overlapping the first
+// (second) half passes it.  A shell is zero at r^2 when a * r^2 > cutoff for
+// all of its exponents a (a_min * r^2 rounds no higher than a * r^2), or
+// past the nucleus range.
+__device__ __attribute__((always_inline)) inline uint32_t lane_screen(int64_t s_first, int64_t S, int k_lo,
+                                                                      int k_end, int N,
+                                                                      bool inside, double px, double py, double pz,
+                                                                      const double *__restrict__ nucl,
+                                                                      const ShellDesc *__restrict__ shells,
+                                                                      const double *__restrict__ nucl_range) {
+  uint32_t h = 0;
+  const int k_mid = k_lo + 32;
+  for (int64_t s = s_first; s < S; ++s) {
+    const ShellDesc d = shells[s];
+    if (d.ao_start >= k_end) break;
+    const double x = px - nucl[d.nucl];
+    const double y = py - nucl[N + d.nucl];
+    const double z = pz - nucl[2 * N + d.nucl];
+    const double r2 = fma(x, x, fma(y, y, z * z));
+    const bool live = inside && !(r2 > kCutoff * nucl_range[d.nucl]) && !(d.a_min * r2 > kCutoff);
+    const int a_end = d.ao_start + (d.l + 1) * (d.l + 2) / 2;
+    h |= (live && d.ao_start < k_mid ? 1u : 0u) | (live && a_end > k_mid ? 2u : 0u);
+  }
+  return h;
+}
+
 // Walks the shells from s_first that start below k_end in AO order for
 // this lane's point and pushes their AOs into w (the writer keeps those of
 // its current stage).
@@ -434,7 +461,7 @@ static_assert(64 % kRing64 == 0 && kRing64 >= 2 && kRing64 <= 16, "ring length")
 
 template <int NC>
 struct F64Writer {
-  uint32_t nz;       // bit t: an entry of this lane's rows in row tile t0 + t is nonzero (current 32-AO block)
+  uint32_t hs;       // bit h: this lane's point passes the screen in 32-AO half h of the current stage (NC = 5)
   int64_t t0;        // row tile of this lane's first row (point * 5)
   int cs;            // first component whose row is in tile t0 + 1 (5: none)
   int k;             // next AO index (wavefront-uniform)
@@ -447,35 +474,44 @@ struct F64Writer {
   uint8_t *flags;    // zeroed [tiles][nkb], or null
   int nkb, lane;
 
-  // the ring's first n slots hold AOs kb .. kb + n - 1: read them all out
-  // (lane l: slot l % kRing64 of kRing64 points), then store
+  // the ring's first n slots hold AOs kb .. kb + n - 1: read them out (lane
+  // l: slot l % kRing64 of kRing64 points) and store them; the five
+  // components half the points at a time (half the registers: 141 instead
+  // of 195 with the screen flags below)
   __device__ __attribute__((always_inline)) void flush(int kb, int n) {
     __builtin_amdgcn_wave_barrier();
     const int s = lane % kRing64;
-    double v[kRing64][NC];
+    constexpr int NH = NC == 5 ? 2 : 1;
+    constexpr int H = kRing64 / NH;
 #pragma unroll
-    for (int j = 0; j < kRing64; ++j) {
-      const int q = j * (64 / kRing64) + lane / kRing64;
+    for (int half = 0; half < NH; ++half) {
+      double v[H][NC];
 #pragma unroll
-      for (int c = 0; c < NC; ++c) v[j][c] = ring[(c * kRing64 + s) * kRingLd + ring_col(s, q)];
-    }
-    __builtin_amdgcn_wave_barrier();
+      for (int j = 0; j < H; ++j) {
+        const int q = (half * H + j) * (64 / kRing64) + lane / kRing64;
 #pragma unroll
-    for (int j = 0; j < kRing64; ++j) {
-      if (pq[j] < 0) continue;
-      double *row = out + (int64_t)pq[j] * NC * A + s;
+        for (int c = 0; c < NC; ++c) v[j][c] = ring[(c * kRing64 + s) * kRingLd + ring_col(s, q)];
+      }
+      if (half == NH - 1) __builtin_amdgcn_wave_barrier();  // every read of the ring is issued
 #pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        if (s < n) __builtin_nontemporal_store(v[j][c], row + c * A + kb);
+      for (int j = 0; j < H; ++j) {
+        if (pq[half * H + j] < 0) continue;
+        double *row = out + (int64_t)pq[half * H + j] * NC * A + s;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          if (s < n) __builtin_nontemporal_store(v[j][c], row + c * A + kb);
+        }
       }
     }
   }
+  // the flags of 32-AO block b, from the screen: a point whose shells there
+  // pass it marks the (one or two) row tiles of its rows
   __device__ __attribute__((always_inline)) void flag_block(int b) {
-    if (flags != nullptr) {
-      if (nz & 1u) flags[t0 * nkb + b] = 1;
-      if (nz & 2u) flags[(t0 + 1) * nkb + b] = 1;
+    if constexpr (NC != 5) return;  // the flags serve the MO contraction of ao_vgl
+    if (flags != nullptr && ((hs >> (b & 1)) & 1u)) {
+      flags[t0 * nkb + b] = 1;
+      if (cs < 5) flags[(t0 + 1) * nkb + b] = 1;
     }
-    nz = 0;
   }
   __device__ __attribute__((always_inline)) void push(const double (&v)[5]) {
     if (k < k_lo || k >= k_hi) {  // part of a shell outside the current stage
@@ -484,10 +520,7 @@ struct F64Writer {
     }
     const int slot = k % kRing64;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      ring[(c * kRing64 + slot) * kRingLd + ring_col(slot, lane)] = v[c];
-      if constexpr (NC == 5) nz |= (v[c] != 0.0 ? 1u : 0u) << (c >= cs ? 1 : 0);
-    }
+    for (int c = 0; c < NC; ++c) ring[(c * kRing64 + slot) * kRingLd + ring_col(slot, lane)] = v[c];
     if (slot == kRing64 - 1) flush(k - (kRing64 - 1), kRing64);
     if ((k & 31) == 31) flag_block(k >> 5);
     ++k;
@@ -535,7 +568,7 @@ ao_vgl_f64_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord,
   const int64_t nst = (A + 63) / 64;
   const int64_t st_lo = g * nst / groups, st_hi = (g + 1) * nst / groups;
   F64Writer<NC> w;
-  w.nz = 0;
+  w.hs = 0;
   w.t0 = (5 * p) >> 6;
   w.cs = inside && 64 - (int)((5 * p) & 63) < 5 ? 64 - (int)((5 * p) & 63) : 5;
   w.ring = s_ring;
@@ -553,7 +586,15 @@ ao_vgl_f64_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord,
     const int64_t s_first = stage_shell[st];
     w.k_lo = (int)(64 * st);
     w.k_hi = k_end;
-    if (!stage_live(s_first, S, k_end, N, inside, px, py, pz, nucl, shells, nucl_range)) {
+    // per-lane halves for the flags (NC = 5); whole-stage liveness otherwise
+    bool live;
+    if constexpr (NC == 5) {
+      w.hs = lane_screen(s_first, S, w.k_lo, k_end, N, inside, px, py, pz, nucl, shells, nucl_range);
+      live = __ballot(w.hs != 0) != 0;
+    } else {
+      live = stage_live(s_first, S, k_end, N, inside, px, py, pz, nucl, shells, nucl_range);
+    }
+    if (!live) {
       w.zeros(k_end);
       continue;
     }

@@ -728,7 +728,7 @@ def main():
             rec["speedup_vs_plain_fp64"] = r["value"] / results["plain"]["value"]
         out["forms"][form] = rec
     if not args.no_batch:
-        out["batch_48_ragged"] = bench_batch(["plain", args.form], max(2, args.steps // 3), 1)
+        out["batch_48_ragged"] = bench_batch(["plain", args.form], max(5, args.steps), 2)
         out["adjoint_c5"] = bench_adjoint(max(2, args.steps // 3), 1)
         out["value_only_fp64"] = bench_value_only(args.steps, 1)
     if not args.no_cpu:

@@ -132,6 +132,63 @@ def test_single_s_primitive_closed_form(oracle_lib):
     assert ao[2, 4, 0] == pytest.approx(-6 * a * c, rel=1e-15)
 
 
+def test_shell_vgl_layout_closed_form(oracle_lib):
+    """qo_shell_vgl [P][5][S] against the radial sums in closed form on two
+    nuclei with shells of several exponents, normalisation factors and
+    angular momenta: R = sum_i c_i f_i g_s exp(-a_i r^2), grad R = -2 sum a_i
+    (...) r, lap R = sum (4 a_i^2 r^2 - 6 a_i)(...), zero past either screen
+    (src/qmckl_ao_sycl.cpp:53-55, 72-84)."""
+    nuc = np.array([[0.0, 0.0, 0.0], [1.4, -0.3, 0.8]])
+    shells = [  # (nucleus, l, exponents, coefficients, prim factors, shell factor)
+        (0, 0, [5.0, 0.9], [0.4, 0.7], [1.1, 0.9], 1.3),
+        (0, 1, [1.7], [1.0], [2.0], 0.5),
+        (1, 2, [0.35, 3.1, 12.0], [0.2, -0.5, 0.8], [1.0, 1.2, 0.7], 2.0),
+        (1, 0, [40.0], [1.5], [1.0], 1.0),
+    ]
+    expo = np.concatenate([np.array(sh[2]) for sh in shells])
+    coef = np.concatenate([np.array(sh[3]) for sh in shells])
+    pf = np.concatenate([np.array(sh[4]) for sh in shells])
+    nprim = np.array([len(sh[2]) for sh in shells], dtype=np.int64)
+    pidx = np.concatenate([[0], np.cumsum(nprim)[:-1]]).astype(np.int64)
+    lvals = np.array([sh[1] for sh in shells], dtype=np.int32)
+    A = int(sum(S.ncart(int(l)) for l in lvals))
+    rng = np.random.default_rng(3)
+    pts = np.concatenate([rng.normal(0.4, 1.2, size=(9, 3)), [[0.0, 0.0, 0.0], [1.4, -0.3, 0.8], [20.0, 0.0, 0.0]]])
+    s = S.System(
+        name="two_nuclei", charge=np.array([1.0, 1.0]), nucl_coord=np.ascontiguousarray(nuc.T),
+        nucleus_index=np.array([0, 2], dtype=np.int64), nucleus_shell_num=np.array([2, 2], dtype=np.int64),
+        shell_ang_mom=lvals, shell_prim_num=nprim, shell_prim_index=pidx,
+        shell_factor=np.array([sh[5] for sh in shells]), exponent=expo, coefficient=coef, prim_factor=pf,
+        ao_factor=np.ones(A), mo_coef=np.eye(A), elec_up_num=len(pts), elec_dn_num=0,
+        elec_coord=pts.reshape(1, -1, 3))
+    got = oracle_lib.shell_vgl(s)
+    assert got.shape == (len(pts), 5, len(shells))
+    cut = -math.log(1e-12)
+    # nucleus range: the largest 1/alpha of its primitives
+    rng_n = [max(1.0 / a for sh in shells if sh[0] == n for a in sh[2]) for n in range(2)]
+    want = np.zeros_like(got)
+    for p, r in enumerate(pts):
+        for k, (n, l, al, c, f, g) in enumerate(shells):
+            d = r - nuc[n]
+            r2 = float(d @ d)
+            if r2 > cut * rng_n[n]:
+                continue
+            for a, ci, fi in zip(al, c, f):
+                if a * r2 > cut:
+                    continue
+                v = ci * fi * g * math.exp(-a * r2)
+                want[p, 0, k] += v
+                want[p, 1:4, k] += -2.0 * a * v * d
+                want[p, 4, k] += (4.0 * a * a * r2 - 6.0 * a) * v
+    scale = np.maximum(np.abs(want), 1e-300)
+    assert np.all((want == 0.0) == (got == 0.0)), "screened entries differ"
+    assert np.all(np.abs(got - want) <= 1e-13 * scale + 1e-300), np.max(np.abs(got - want) / scale)
+    # the far point is past both nuclei's range: all zero; the nuclei
+    # themselves have zero gradients
+    assert np.all(got[-1] == 0.0)
+    assert np.all(got[-3, 1:4, :2] == 0.0) and np.all(got[-2, 1:4, 2:] == 0.0)
+
+
 def test_primitive_screen_gives_exact_zero(oracle_lib):
     a = 3.0
     r_cut = math.sqrt(27.631021115928547 / a)

@@ -559,6 +559,22 @@ def errors_vs_oracle(s, mos, idx):
     return out
 
 
+def errors_all_points(s, mo, chunk=8192):
+    """The same measure as errors_vs_oracle over every point of the
+    workload (mo: mo_vgl of all points), the oracle run in chunks."""
+    from oracle import oracle
+    coord = s.coord_T()
+    worst = np.zeros(5)
+    for p0 in range(0, s.point_num, chunk):
+        sl = slice(p0, min(s.point_num, p0 + chunk))
+        ref = oracle.mo_vgl_from_ao(s.mo_coef, oracle.ao_vgl(s, np.ascontiguousarray(coord[:, sl])))
+        rm = np.abs(ref).max(axis=2, keepdims=True)
+        e = (np.abs(mo[sl] - ref) / np.where(rm > 0, rm, 1.0)).max(axis=(0, 2))
+        worst = np.maximum(worst, e)
+    return {"points_checked": int(s.point_num), "value": float(worst[0]), "gradient": float(worst[1:4].max()),
+            "laplacian": float(worst[4])}
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -684,6 +700,7 @@ def main():
     ctx = Context(0)
     ctx.set_system(s)
     results, mos = {}, {}
+    head_mo = None  # the headline form's mo_vgl at every point
     idx = sample_points(s)
     forms = [args.form] if (args.no_plain or args.form == "plain") else ["plain", args.form]
     extra = [] if args.no_forms else [f for f in FORM_IDS if f not in forms]
@@ -693,6 +710,8 @@ def main():
         results[form] = {"ms_per_step": sec * 1e3, "value": s.point_num / sec, "kernels": kernels,
                          "steps": steps}
         mos[form] = mo[idx]
+        if form == args.form:
+            head_mo = mo
         del mo
         log(form, json.dumps(results[form]))
     ctx.destroy()
@@ -710,7 +729,9 @@ def main():
         "roofline": roofline(head["kernels"], s),
         "kernels": head["kernels"],
         "mo_error_vs_fp64_oracle": {k: v for k, v in errs.items() if k not in extra},
+        "mo_error_all_points": errors_all_points(s, head_mo),
     }
+    del head_mo
     if "plain" in results and args.form != "plain":
         out["plain_fp64"] = {"value": results["plain"]["value"],
                              "ms_per_step": results["plain"]["ms_per_step"],

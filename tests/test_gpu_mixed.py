@@ -100,6 +100,33 @@ def test_water64_forms_sampled(oracle_lib):
         assert max(e.values()) <= BAR.get(form, 1e-12), (form, e)
 
 
+def test_water64_fp16x2_every_point(oracle_lib):
+    """The headline workload in full: 64-water, 128 walkers, FP16X2 and
+    FP64, every one of the 81,920 points against the oracle (evaluated in
+    chunks of 8,192 points), the row error measure over all 409,600 rows."""
+    s = S.water64(walk_num=128)
+    P = s.point_num
+    coord = s.coord_T()
+    with Context(0) as c:
+        c.set_system(s)
+        got16 = run_form(c, "fp16x2", P)
+        c.call("qmckl_set_mo_basis_precision_device", 0)
+        got64 = c.mo_vgl(P)
+    worst16 = {"value": 0.0, "gradient": 0.0, "laplacian": 0.0}
+    worst64 = dict(worst16)
+    for p0 in range(0, P, 8192):
+        sl = slice(p0, min(P, p0 + 8192))
+        ref = oracle_lib.mo_vgl_from_ao(s.mo_coef, oracle_lib.ao_vgl(s, np.ascontiguousarray(coord[:, sl])))
+        for worst, got in ((worst16, got16), (worst64, got64)):
+            e = row_err(got[sl], ref)
+            for k in worst:
+                worst[k] = max(worst[k], e[k])
+        del ref
+    say(f"water64 every point: fp16x2 {worst16}; fp64 {worst64}")
+    assert max(worst16.values()) <= BAR["fp16x2"], worst16
+    assert max(worst64.values()) <= 1e-12, worst64
+
+
 @pytest.mark.parametrize("form", list(FORMS))
 def test_point_order_invariance(form):
     """Each point's MOs do not depend on which points share its tile (the

@@ -262,6 +262,14 @@ struct MoBasis {
   bool adj_cfrag_ok = false;
   double *adj_wfrag = nullptr;
   int64_t adj_wfrag_cap = 0;
+  int32_t *adj_kmap = nullptr;   // (point, component) rows in point-tile order
+  int64_t adj_kmap_cap = 0;
+  int32_t *adj_perm = nullptr;   // the points in that order (stable, so the sums are reproducible)
+  int64_t adj_perm_cap = 0;
+  void *adj_order_ws = nullptr;
+  int64_t adj_order_ws_cap = 0;
+  uint8_t *adj_tflags = nullptr;  // screen flags per (64-point tile, 32-AO block)
+  int64_t adj_tflags_cap = 0;
   // with a column plan, G = w C runs with K over the MOs in the plan's
   // order: the weights' columns gathered into that order (per call), the
   // coefficient fragments built with their rows in it, and the live bits of
@@ -387,6 +395,11 @@ hipError_t launch_transpose(hipStream_t s, int64_t rows, int64_t cols, const dou
 int64_t point_order_workspace_bytes(int64_t P);
 hipError_t launch_point_order(hipStream_t s, int64_t P, const double *coord, int64_t N, const double *nucl,
                               void *workspace, int32_t *perm);
+// The same cells with the points of a cell in ascending index (a stable
+// radix sort of the cell codes): a function of the coordinates alone.
+int64_t point_order_stable_workspace_bytes(int64_t P);
+hipError_t launch_point_order_stable(hipStream_t s, int64_t P, const double *coord, int64_t N, const double *nucl,
+                                     void *workspace, int32_t *perm);
 hipError_t launch_distance(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, const double *A, int64_t lda,
                            const double *B, int64_t ldb, double *C, int64_t ldc);
 hipError_t launch_ee_distance(hipStream_t s, int64_t E, int64_t W, const double *coord, double *out);
@@ -493,10 +506,23 @@ int64_t plan_mo_columns(const uint64_t *words, int64_t M, int64_t nw, int64_t nk
 // out_t[Ncol][R] = (A^T Bf)^T with A stored K-major [K][lda] (element (r, k)
 // at A[k*lda + r]) and B in the fragment order of launch_frag_b_f64, split
 // over K into `splits` partials [splits][R][Ncol] reduced in a fixed order.
+// With kmap (K-row map, [K]), row k of the product is row kmap[k] of A (B
+// already in the mapped order); with tflags ([K/320 rounded up][tnkb]), the
+// K rows of 320-row tile t contribute nothing to the 32-row output blocks b
+// whose flag (t, b) is zero, and are skipped.
 int gemm_tn_frag_splits(int64_t R, int64_t K);
 hipError_t launch_gemm_tn_f64_frag(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const double *A,
                                    int64_t lda, const double *Bf, int64_t nk4, double *out_t,
-                                   double *partial, int splits);
+                                   double *partial, int splits, const int32_t *kmap = nullptr,
+                                   const uint8_t *tflags = nullptr, int tnkb = 0);
+// kmap[5 i + c] = 5 perm[i] + c for the (point, component) rows of P points.
+hipError_t launch_point_rows_map(hipStream_t s, int64_t P, const int32_t *perm, int32_t *kmap);
+// Screen flags of P points in 64-point tiles (tile t = points perm[64 t ..],
+// perm null: in order): tflags[t][b] = 1 when some shell overlapping 32-AO
+// block b passes the AO screen for some point of the tile, else 0 (every
+// AO of the block is then zero at every point of the tile).
+hipError_t launch_tile_screen(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
+                              const int32_t *perm, const double *nucl, uint8_t *tflags, int nkb);
 hipError_t launch_ao_adjoint(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
                              const double *nucl, const double *ao_factor, const double *G,
                              double *partR, double *partA);

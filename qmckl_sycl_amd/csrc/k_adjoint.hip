@@ -189,24 +189,40 @@ ao_adjoint_kernel(int64_t P, int S, int A, int N, const double *__restrict__ coo
   }
 }
 
-// dR[n][q] = sum_b sum_{s of n} partR[b][s][q]; dA[j] = sum_b partA[b][j]
+// dR[n][q] = sum over blocks b and the shells s of nucleus n of
+// partR[b][s][q], dA[t] = sum over b of partA[b][t], in a fixed order:
+// kRedLanes threads per output each sum the blocks b = j, j + kRedLanes, ...
+// (b ascending, and for dR the shells ascending within each b), then one
+// thread adds the kRedLanes partial sums in j order.  256 threads handle
+// 256 / kRedLanes outputs.
+constexpr int kRedLanes = 16;
 __global__ void __launch_bounds__(kThreads)
 adjoint_reduce_kernel(int64_t nblk, int S, int N, int64_t Np, const int64_t *__restrict__ nucleus_index,
                       const int64_t *__restrict__ nucleus_shell_num, const double *__restrict__ partR,
                       const double *__restrict__ partA, double *__restrict__ dR, double *__restrict__ dA) {
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (dR != nullptr && t < 3 * (int64_t)N) {
+  __shared__ double part[kThreads];
+  constexpr int per = kThreads / kRedLanes;  // outputs per workgroup
+  const int j = threadIdx.x % kRedLanes, o = threadIdx.x / kRedLanes;
+  const int64_t nR = dR != nullptr ? 3 * (int64_t)N : 0;
+  const int64_t t = blockIdx.x * (int64_t)per + o;  // output: dR entries first, then dA
+  double acc = 0.0;
+  if (t < nR) {
     const int64_t n = t / 3, q = t - 3 * n;
-    double acc = 0.0;
-    for (int64_t b = 0; b < nblk; ++b)
-      for (int64_t s = nucleus_index[n]; s < nucleus_index[n] + nucleus_shell_num[n]; ++s)
-        acc += partR[(b * S + s) * 3 + q];
-    dR[t] = acc;
+    const int64_t s0 = nucleus_index[n], s1 = s0 + nucleus_shell_num[n];
+    for (int64_t b = j; b < nblk; b += kRedLanes)
+      for (int64_t s = s0; s < s1; ++s) acc += partR[(b * S + s) * 3 + q];
+  } else if (dA != nullptr && t - nR < Np) {
+    const int64_t a = t - nR;
+    for (int64_t b = j; b < nblk; b += kRedLanes) acc += partA[b * Np + a];
   }
-  if (dA != nullptr && t < Np) {
-    double acc = 0.0;
-    for (int64_t b = 0; b < nblk; ++b) acc += partA[b * Np + t];
-    dA[t] = acc;
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  if (j == 0) {
+    double sum = 0.0;
+#pragma unroll
+    for (int i = 0; i < kRedLanes; ++i) sum += part[o * kRedLanes + i];
+    if (t < nR) dR[t] = sum;
+    else if (dA != nullptr && t - nR < Np) dA[t - nR] = sum;
   }
 }
 
@@ -234,9 +250,12 @@ int64_t ao_adjoint_blocks(int64_t P) { return (P + kTP - 1) / kTP; }
 hipError_t launch_adjoint_reduce(hipStream_t s, const DeviceBasis &b, int64_t nblk,
                                  const int64_t *nucleus_index, const int64_t *nucleus_shell_num,
                                  const double *partR, const double *partA, double *dR, double *dA) {
-  const int64_t n = 3 * b.nucl_num > b.prim_num ? 3 * b.nucl_num : b.prim_num;
-  const int64_t g = (n + kThreads - 1) / kThreads;
-  hipLaunchKernelGGL(adjoint_reduce_kernel, dim3((unsigned)(g > 0 ? g : 1)), dim3(kThreads), 0, s, nblk,
+  const int64_t n = (dR != nullptr ? 3 * b.nucl_num : 0) + (dA != nullptr ? b.prim_num : 0);
+  if (n <= 0) return hipSuccess;
+  constexpr int per = kThreads / kRedLanes;
+  const int64_t g = (n + per - 1) / per;
+  if (g > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(adjoint_reduce_kernel, dim3((unsigned)g), dim3(kThreads), 0, s, nblk,
                      (int)b.shell_num, (int)b.nucl_num, b.prim_num, nucleus_index, nucleus_shell_num,
                      partR, partA, dR, dA);
   return hipGetLastError();

@@ -604,6 +604,48 @@ ao_vgl_f64_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord,
   }
 }
 
+// Screen flags per (64-point tile, 32-AO block): wavefront = (tile, group
+// of 64-AO stages), lane = point; per stage the lane screen of each half,
+// ORed over the tile's points by a ballot.
+__global__ void __launch_bounds__(kThreads)
+tile_screen_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord, const int32_t *__restrict__ perm,
+                   const double *__restrict__ nucl, const ShellDesc *__restrict__ shells,
+                   const int32_t *__restrict__ stage_shell, const double *__restrict__ nucl_range, int64_t A,
+                   int groups, uint8_t *__restrict__ tflags, int nkb) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wi = (int64_t)blockIdx.x * (kThreads / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t npt = (P + 63) / 64;
+  const int64_t pt = wi / groups;
+  const int g = (int)(wi - pt * groups);
+  if (pt >= npt) return;  // wavefront-uniform
+  const int64_t i = pt * 64 + lane;
+  const bool inside = i < P;
+  const int64_t p = inside ? (perm != nullptr ? (int64_t)perm[i] : i) : 0;
+  const double px = inside ? coord[p] : 0.0;
+  const double py = inside ? coord[P + p] : 0.0;
+  const double pz = inside ? coord[2 * P + p] : 0.0;
+  const int64_t nst = (A + 63) / 64;
+  for (int64_t st = g * nst / groups; st < (g + 1) * nst / groups; ++st) {
+    const int k_lo = (int)(64 * st);
+    const int k_end = (int)(64 * (st + 1) < A ? 64 * (st + 1) : A);
+    const uint32_t hs = lane_screen(stage_shell[st], S, k_lo, k_end, N, inside, px, py, pz, nucl, shells,
+                                    nucl_range);
+    const bool h0 = __ballot(hs & 1u) != 0, h1 = __ballot(hs & 2u) != 0;
+    if (lane == 0) {
+      tflags[pt * nkb + 2 * st] = h0 ? 1 : 0;
+      if (2 * st + 1 < nkb) tflags[pt * nkb + 2 * st + 1] = h1 ? 1 : 0;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kThreads)
+point_rows_map_kernel(int64_t P, const int32_t *__restrict__ perm, int32_t *__restrict__ kmap) {
+  for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < 5 * P; k += (int64_t)gridDim.x * kThreads) {
+    const int64_t i = k / 5;
+    kmap[k] = (int32_t)(5 * (int64_t)perm[i] + (k - 5 * i));
+  }
+}
+
 __global__ void __launch_bounds__(kThreads) fill_zero_bytes_kernel(int64_t n, uint8_t *__restrict__ p) {
   for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
     p[i] = 0;
@@ -682,6 +724,33 @@ hipError_t launch_ao_rows_f32(hipStream_t s, const DeviceBasis &b, int64_t P, co
                        b.shell_num, coord, perm, nucl, b.shells, b.stage_shell, ao_factor, b.expo, b.coef_norm,
                        b.nucl_range, b.ao_num, ld, npt, groups, buf, nullptr, rowmax, flags, nkb,
                        skip_halves ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_screen(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
+                              const int32_t *perm, const double *nucl, uint8_t *tflags, int nkb) {
+  if (P <= 0) return hipSuccess;
+  const int64_t A = b.ao_num;
+  if (A <= 0 || b.stage_shell == nullptr || (int64_t)nkb != (A + 31) / 32 || P > 0x7fffffff)
+    return hipErrorInvalidValue;
+  const int64_t npt = (P + 63) / 64, nst = (A + 63) / 64;
+  int64_t groups = (64 * 256 + npt - 1) / npt;  // about 16,384 wavefronts
+  if (groups > nst) groups = nst;
+  if (groups < 1) groups = 1;
+  const int64_t blocks = (npt * groups + kThreads / 64 - 1) / (kThreads / 64);
+  if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(tile_screen_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, P, (int)b.nucl_num,
+                     b.shell_num, coord, perm, nucl, b.shells, b.stage_shell, b.nucl_range, A, (int)groups, tflags,
+                     nkb);
+  return hipGetLastError();
+}
+
+hipError_t launch_point_rows_map(hipStream_t s, int64_t P, const int32_t *perm, int32_t *kmap) {
+  if (P <= 0) return hipSuccess;
+  if (5 * P > 0x7fffffff) return hipErrorInvalidValue;
+  int64_t g = (5 * P + kThreads - 1) / kThreads;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(point_rows_map_kernel, dim3((unsigned)g), dim3(kThreads), 0, s, P, perm, kmap);
   return hipGetLastError();
 }
 

@@ -9,6 +9,9 @@
 // kernel is HBM-bound: 24 B read + 8N B written per point.
 #include <hip/hip_runtime.h>
 
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+
 #include "internal.hpp"
 
 namespace qmckl_amd {
@@ -370,6 +373,45 @@ hipError_t launch_ee_distance(hipStream_t s, int64_t E, int64_t W, const double 
 }
 
 int64_t point_order_workspace_bytes(int64_t P) { return 64 + 4 * (int64_t)kCells + 4 * P; }
+
+// The same cells, points ordered by a stable radix sort of their cell codes:
+// within a cell in ascending point index, so the order is a function of the
+// coordinates alone (the atomic scatter above leaves it open).
+namespace {
+size_t stable_sort_temp_bytes(int64_t P) {
+  size_t bytes = 0;
+  if (rocprim::radix_sort_pairs(nullptr, bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                rocprim::counting_iterator<int32_t>(0), (int32_t *)nullptr, (size_t)P, 0,
+                                3 * kCellBits) != hipSuccess)
+    return 0;
+  return bytes;
+}
+}  // namespace
+
+int64_t point_order_stable_workspace_bytes(int64_t P) {
+  return 64 + 4 * (int64_t)kCells + 8 * P + 256 + (int64_t)stable_sort_temp_bytes(P);
+}
+
+hipError_t launch_point_order_stable(hipStream_t s, int64_t P, const double *coord, int64_t N, const double *nucl,
+                                     void *workspace, int32_t *perm) {
+  if (P <= 0) return hipSuccess;
+  if (P > 0x7fffffff || N <= 0) return hipErrorInvalidValue;
+  char *ws = static_cast<char *>(workspace);
+  double *bbox = reinterpret_cast<double *>(ws);
+  int32_t *counts = reinterpret_cast<int32_t *>(ws + 64);
+  int32_t *cell = counts + kCells;
+  uint32_t *sorted = reinterpret_cast<uint32_t *>(cell + P);
+  const int64_t used = 64 + 4 * (int64_t)kCells + 8 * P;
+  void *temp = ws + (used + 255) / 256 * 256;
+  size_t temp_bytes = stable_sort_temp_bytes(P);
+  const unsigned blocks = (unsigned)((P + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL(order_setup_kernel, dim3(1), dim3(kThreads), 0, s, N, nucl, bbox, counts);
+  hipLaunchKernelGGL(order_cells_kernel, dim3(blocks), dim3(kThreads), 0, s, P, coord, bbox, counts, cell);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return rocprim::radix_sort_pairs(temp, temp_bytes, reinterpret_cast<const uint32_t *>(cell), sorted,
+                                   rocprim::counting_iterator<int32_t>(0), perm, (size_t)P, 0, 3 * kCellBits, s);
+}
 
 hipError_t launch_point_order(hipStream_t s, int64_t P, const double *coord, int64_t N, const double *nucl,
                               void *workspace, int32_t *perm) {

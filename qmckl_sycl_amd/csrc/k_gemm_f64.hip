@@ -210,7 +210,8 @@ template <int CBW, bool TRANSA>
 __global__ void __launch_bounds__(kThreads, 2)
 gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restrict__ A, int64_t lda,
                      const double *__restrict__ Bf, int64_t nk4, double *__restrict__ C, int64_t ldc,
-                     const uint8_t *__restrict__ flags, int nkb, int64_t kchunk, ColPlan plan) {
+                     const uint8_t *__restrict__ flags, int nkb, int64_t kchunk, ColPlan plan,
+                     const int32_t *__restrict__ kmap, const uint8_t *__restrict__ tflags, int tnkb) {
   __shared__ double As[kBK][kBM + 1];
   __shared__ int s_list[kMaxStages16];
   __shared__ uint32_t s_live[kMaxStages16];
@@ -229,7 +230,7 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
   C += (int64_t)blockIdx.z * R * ldc;
 
   // stage list: live stages and, per stage, bit b for live column block gb0 + b
-  const bool listed = flags != nullptr || plan.blk != nullptr;
+  const bool listed = flags != nullptr || plan.blk != nullptr || (TRANSA && tflags != nullptr);
   int count = nst;
   if (listed) {
     if (wave == 0) {
@@ -239,6 +240,13 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
       for (int s0 = 0; s0 < nst; s0 += 64) {
         const int st = s0 + lane;
         bool live = st < nst && (f == nullptr || f[st >> 1] != 0);
+        if (TRANSA && tflags != nullptr && live) {
+          // the K rows of stage st lie in one 320-row tile (20 stages); the
+          // workgroup's 64 output rows are blocks 2 bx and 2 bx + 1
+          const uint8_t *tf = tflags + ((g0 + st) / 20) * tnkb;
+          const int64_t b0 = 2 * (int64_t)blockIdx.x;
+          live = tf[b0] != 0 || (b0 + 1 < tnkb && tf[b0 + 1] != 0);
+        }
         uint32_t lv = 0xffffffffu;
         if (live && plan.blk != nullptr) {
           // all loads first (independent, no branches), then the bits
@@ -287,11 +295,20 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
   auto stage_k0 = [&](int s) {
     return kbeg + (int64_t)__builtin_amdgcn_readfirstlane(listed ? s_list[s] : s) * kBK;
   };
-  auto load_a = [&](int64_t k0) {
+  // with a K-row map, this thread's mapped row of the stage after the next
+  // one is fetched a stage ahead, so the slab load does not wait on it
+  int32_t kr_next = 0;
+  auto map_row = [&](int s) -> int32_t {
+    if (kmap == nullptr || s >= count) return 0;
+    const int64_t k = stage_k0(s) + aq4;
+    return k < kend ? kmap[k] : 0;
+  };
+  auto load_a = [&](int64_t k0, int32_t kr_mapped) {
     if constexpr (TRANSA) {
       const bool kin = k0 + aq4 < kend;
+      const int64_t kr = kmap != nullptr ? (int64_t)kr_mapped : k0 + aq4;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) ra[j] = kin && row0 + ar4 + j < R ? Ap[(k0 + aq4) * lda + j] : 0.0;
+      for (int j = 0; j < 4; ++j) ra[j] = kin && row0 + ar4 + j < R ? Ap[kr * lda + j] : 0.0;
     } else {
 #pragma unroll
       for (int j = 0; j < 4; ++j) ra[j] = k0 + aq + j < kend ? Ap[k0 + j] : 0.0;
@@ -339,7 +356,8 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
   if (count > 0) {
     const int64_t k0 = stage_k0(0);
     live0 = wave_live(0);
-    load_a(k0);
+    load_a(k0, map_row(0));
+    kr_next = map_row(1);
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) load_b(k0, ks, live0);
   }
@@ -350,7 +368,10 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
     const bool more = s + 1 < count;
     const int64_t k1 = more ? stage_k0(s + 1) : 0;
     const uint32_t live1 = more ? wave_live(s + 1) : 0u;
-    if (more) load_a(k1);
+    if (more) {
+      load_a(k1, kr_next);
+      kr_next = map_row(s + 2);
+    }
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
       double a[4];
@@ -417,7 +438,8 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
 template <int CBW, bool TRANSA>
 hipError_t launch_frag(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const double *A, int64_t lda,
                        const double *Bf, int64_t nk4, double *C, int64_t ldc, const uint8_t *flags,
-                       int nkb, int splits, ColPlan plan) {
+                       int nkb, int splits, ColPlan plan, const int32_t *kmap, const uint8_t *tflags,
+                       int tnkb) {
   const int64_t gx = (R + kBM - 1) / kBM, gy = (Ncol + 64 * CBW - 1) / (64 * CBW);
   if (gx > 0x7fffffff || gy > 65535 || splits < 1 || splits > 65535) return hipErrorInvalidValue;
   if (flags != nullptr && ((int64_t)nkb * 32 < K || nkb > kMaxFlagBlocks || splits != 1))
@@ -429,35 +451,51 @@ hipError_t launch_frag(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const 
   // the forward product addresses one workgroup's column blocks with 32-bit offsets
   if (!TRANSA && (int64_t)4 * CBW * nk4 * 512 >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
   if (TRANSA && (plan.col_mo != nullptr || plan.blk != nullptr)) return hipErrorInvalidValue;
+  if (!TRANSA && (kmap != nullptr || tflags != nullptr)) return hipErrorInvalidValue;
+  if (tflags != nullptr && ((int64_t)tnkb * 32 < R || (kchunk + kBK - 1) / kBK > kMaxStages16))
+    return hipErrorInvalidValue;
   hipLaunchKernelGGL((gemm_f64_frag_kernel<CBW, TRANSA>), dim3((unsigned)gx, (unsigned)gy, (unsigned)splits),
-                     dim3(kThreads), 0, s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, kchunk, plan);
+                     dim3(kThreads), 0, s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, kchunk, plan, kmap,
+                     tflags, tnkb);
   return hipGetLastError();
 }
 
 template <bool TRANSA>
 hipError_t dispatch_frag(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const double *A, int64_t lda,
                          const double *Bf, int64_t nk4, double *C, int64_t ldc, const uint8_t *flags, int nkb,
-                         int splits, ColPlan plan) {
+                         int splits, ColPlan plan, const int32_t *kmap = nullptr,
+                         const uint8_t *tflags = nullptr, int tnkb = 0) {
   switch (f64_frag_cbw(Ncol)) {
-    case 1: return launch_frag<1, TRANSA>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, splits, plan);
-    case 2: return launch_frag<2, TRANSA>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, splits, plan);
-    case 3: return launch_frag<3, TRANSA>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, splits, plan);
-    case 4: return launch_frag<4, TRANSA>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, splits, plan);
-    default: return launch_frag<5, TRANSA>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, splits, plan);
+    case 1: return launch_frag<1, TRANSA>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, splits, plan, kmap, tflags, tnkb);
+    case 2: return launch_frag<2, TRANSA>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, splits, plan, kmap, tflags, tnkb);
+    case 3: return launch_frag<3, TRANSA>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, splits, plan, kmap, tflags, tnkb);
+    case 4: return launch_frag<4, TRANSA>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, splits, plan, kmap, tflags, tnkb);
+    default: return launch_frag<5, TRANSA>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, splits, plan, kmap, tflags, tnkb);
   }
 }
 
-// out[m][a] = sum_z part[z][a][m], z ascending (fixed order: deterministic)
+// out[m][a] = sum_z part[z][a][m], z ascending (fixed order: deterministic).
+// A workgroup sums a 64 x 64 tile of (a, m) with lanes along m (reads of
+// part in whole rows), then writes it transposed through LDS with lanes
+// along a (writes of out in whole rows).
 __global__ void __launch_bounds__(kThreads)
 sum_splits_t_kernel(int64_t rows, int64_t cols, int splits, const double *__restrict__ part,
                     double *__restrict__ out) {
+  __shared__ double tile[64][65];
+  const int64_t a0 = (int64_t)blockIdx.y * 64, m0 = (int64_t)blockIdx.x * 64;
+  const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;  // 4 rows of 64 per pass
   const int64_t n = rows * cols;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t m = i / rows, a = i - m * rows;  // out index i = m*rows + a
+  for (int r = ly; r < 64; r += kThreads / 64) {
+    const int64_t a = a0 + r, m = m0 + lx;
     double t = 0.0;
-    for (int z = 0; z < splits; ++z) t += part[(int64_t)z * n + a * cols + m];
-    out[i] = t;
+    if (a < rows && m < cols)
+      for (int z = 0; z < splits; ++z) t += part[(int64_t)z * n + a * cols + m];
+    tile[r][lx] = t;
+  }
+  __syncthreads();
+  for (int r = ly; r < 64; r += kThreads / 64) {
+    const int64_t m = m0 + r, a = a0 + lx;
+    if (m < cols && a < rows) out[m * rows + a] = tile[lx][r];
   }
 }
 
@@ -529,16 +567,17 @@ int gemm_tn_frag_splits(int64_t R, int64_t K) {
 
 hipError_t launch_gemm_tn_f64_frag(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const double *A,
                                    int64_t lda, const double *Bf, int64_t nk4, double *out_t,
-                                   double *partial, int splits) {
+                                   double *partial, int splits, const int32_t *kmap, const uint8_t *tflags,
+                                   int tnkb) {
   if (R <= 0 || Ncol <= 0) return hipSuccess;
   if (K <= 0 || nk4 * 4 < K || splits < 1) return hipErrorInvalidValue;
-  hipError_t e = dispatch_frag<true>(s, R, K, Ncol, A, lda, Bf, nk4, partial, Ncol, nullptr, 0, splits, ColPlan());
+  hipError_t e = dispatch_frag<true>(s, R, K, Ncol, A, lda, Bf, nk4, partial, Ncol, nullptr, 0, splits, ColPlan(),
+                                     kmap, tflags, tnkb);
   if (e != hipSuccess) return e;
-  const int64_t n = R * Ncol;
-  int64_t g = (n + kThreads - 1) / kThreads;
-  if (g > 65536) g = 65536;
-  hipLaunchKernelGGL(sum_splits_t_kernel, dim3((unsigned)g), dim3(kThreads), 0, s, R, Ncol, splits, partial,
-                     out_t);
+  const int64_t gm = (Ncol + 63) / 64, ga = (R + 63) / 64;
+  if (gm > 0x7fffffff || ga > 65535) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(sum_splits_t_kernel, dim3((unsigned)gm, (unsigned)ga), dim3(kThreads), 0, s, R, Ncol, splits,
+                     partial, out_t);
   return hipGetLastError();
 }
 

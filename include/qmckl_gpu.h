@@ -443,7 +443,8 @@ qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_batch_device(int64_t count,
  * The mixed precision forms have two tile choices that change speed, not
  * results: whether points are ordered spatially into 64-point tiles
  * (point_order -1 automatic, 0 off, 1 on; it also applies to the FP64
- * ao_vgl kernel used for bases of 1024 AOs or more) and how many wavefronts
+ * ao_vgl kernel used for bases of 1024 AOs or more and to the adjoint's
+ * point visits) and how many wavefronts
  * per CU the AO operand kernel aims at (group_waves 0 automatic, or 1..1024).
  * set: ARG_2 / ARG_3 for values out of range.  get: ARG_2 / ARG_3 for NULL.
  * autotune: times the context's mixed chain on its current points for

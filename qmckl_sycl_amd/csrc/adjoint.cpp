@@ -49,6 +49,39 @@ qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_adjoint_device(qmckl_context_de
   if (!has_device(c)) return fail(context, QMCKL_FAILURE_DEVICE, fn, "no device bound");
   MoBasis &mo = c->mo;
   hipError_t e = hipSuccess;
+  // Both parts visit the points in spatially compact 64-point tiles where
+  // they are ordered for ao_vgl (bases of 1024 AOs or more, 4,096 points or
+  // more, or as the context's point-order setting says).  The order sets
+  // the sums' rounding, so it comes from a stable sort of the points' cell
+  // codes (within a cell by point index) and the gradients stay bitwise
+  // reproducible from call to call.
+  const int32_t cfg = c->mo.tile_point_order;
+  const bool order = cfg >= 0 ? cfg == 1 : round_up(A, 64) >= 1024 && P >= 4096;
+  const int32_t *perm = nullptr;  // point of position i
+  const int32_t *kmap = nullptr;  // (point, component) row of row 5 i + c
+  if (order) {
+    if (!grow_bytes(c, reinterpret_cast<void **>(&mo.adj_perm), &mo.adj_perm_cap, (int64_t)sizeof(int32_t) * P) ||
+        !grow_bytes(c, &mo.adj_order_ws, &mo.adj_order_ws_cap, point_order_stable_workspace_bytes(P)) ||
+        !grow_bytes(c, reinterpret_cast<void **>(&mo.adj_kmap), &mo.adj_kmap_cap, (int64_t)sizeof(int32_t) * 5 * P))
+      return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
+    e = launch_point_order_stable(c->stream, P, c->point.coord, c->nucleus.num, c->nucleus.coord, mo.adj_order_ws,
+                                  mo.adj_perm);
+    if (e == hipSuccess) e = launch_point_rows_map(c->stream, P, mo.adj_perm, mo.adj_kmap);
+    if (e != hipSuccess) return hip_check(context, e, fn);
+    perm = mo.adj_perm;
+    kmap = mo.adj_kmap;
+  }
+  // the AO screen per (64-point tile, 32-AO block): where it is zero, every
+  // AO of the block vanishes at every point of the tile
+  const int tnkb = (int)((A + 31) / 32);
+  const uint8_t *tflags = nullptr;
+  if (perm != nullptr) {
+    if (!grow_bytes(c, reinterpret_cast<void **>(&mo.adj_tflags), &mo.adj_tflags_cap, (P + 63) / 64 * tnkb))
+      return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
+    e = launch_tile_screen(c->stream, c->ao.dev, P, c->point.coord, perm, c->nucleus.coord, mo.adj_tflags, tnkb);
+    if (e != hipSuccess) return hip_check(context, e, fn);
+    tflags = mo.adj_tflags;
+  }
   if (d_nucl_coord != nullptr || d_exponent != nullptr) {
     const int64_t nblk = ao_adjoint_blocks(P);
     if (!grow_d(c, &mo.adj_G, &mo.adj_G_cap, 5 * P * A) ||
@@ -87,22 +120,30 @@ qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_adjoint_device(qmckl_context_de
       if (e != hipSuccess) return hip_check(context, e, fn);
       mo.adj_cfrag_ok = true;
     }
-    if (mo.adj_planned && !grow_d(c, &mo.adj_wperm, &mo.adj_wperm_cap, 5 * P * M))
+    // w's rows in tile order and its columns in the plan's MO order, then
+    // G = w' C' (rows in tile order), skipping zero coefficient blocks and,
+    // in tile order, the (tile, 16-AO) blocks of G the AO screen zeroes: the
+    // AO adjoint reads G only where a shell passes the same screen for its
+    // point, so those blocks are neither computed nor stored.
+    const bool gather = mo.adj_planned || kmap != nullptr;
+    if (gather && !grow_d(c, &mo.adj_wperm, &mo.adj_wperm_cap, 5 * P * M))
       return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
     {
       KernelTimer t(c, "adjoint_g_gemm_f64");
-      if (mo.adj_planned) {
-        // w's columns in the plan's MO order, then G = w' C' skipping zero blocks
-        e = launch_gather_cols(c->stream, 5 * P, M, weight, M, plan.col_mo, mo.adj_wperm, M);
+      if (gather)
+        e = launch_gather(c->stream, 5 * P, M, weight, M, kmap, mo.adj_planned ? plan.col_mo : nullptr,
+                          mo.adj_wperm, M);
+      const double *wa = gather ? mo.adj_wperm : weight;
+      if (e == hipSuccess && mo.adj_planned) {
         ColPlan gp;
         gp.blk = mo.adj_blk;
         gp.nw = mo.adj_blk_nw;
         gp.nblk = (A + 15) / 16;
-        if (e == hipSuccess)
-          e = launch_mo_gemm_f64_frag(c->stream, 5 * P, M, A, mo.adj_wperm, M, mo.adj_cfrag, mo.adj_cfrag_nk4,
-                                      mo.adj_G, A, nullptr, 0, gp);
-      } else {
-        e = launch_mo_gemm_f64_frag(c->stream, 5 * P, M, A, weight, M, mo.adj_cfrag, mo.adj_cfrag_nk4, mo.adj_G, A);
+        e = launch_mo_gemm_f64_frag(c->stream, 5 * P, M, A, wa, M, mo.adj_cfrag, mo.adj_cfrag_nk4, mo.adj_G, A,
+                                    nullptr, 0, gp, tflags, tnkb);
+      } else if (e == hipSuccess) {
+        e = launch_mo_gemm_f64_frag(c->stream, 5 * P, M, A, wa, M, mo.adj_cfrag, mo.adj_cfrag_nk4, mo.adj_G, A,
+                                    nullptr, 0, ColPlan(), tflags, tnkb);
       }
       t.stop();
     }
@@ -110,7 +151,7 @@ qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_adjoint_device(qmckl_context_de
     if (e == hipSuccess) e = hipMemsetAsync(mo.adj_partA, 0, sizeof(double) * nblk * Np, c->stream);
     if (e == hipSuccess) {
       KernelTimer t(c, "ao_adjoint");
-      e = launch_ao_adjoint(c->stream, c->ao.dev, P, c->point.coord, c->nucleus.coord, c->ao.ao_factor,
+      e = launch_ao_adjoint(c->stream, c->ao.dev, P, c->point.coord, perm, c->nucleus.coord, c->ao.ao_factor,
                             mo.adj_G, mo.adj_partR, mo.adj_partA);
       t.stop();
     }
@@ -136,40 +177,15 @@ qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_adjoint_device(qmckl_context_de
     if (!grow_d(c, &mo.adj_partC, &mo.adj_partC_cap, (int64_t)splits * M * A) ||
         !grow_d(c, &mo.adj_wfrag, &mo.adj_wfrag_cap, wlen))
       return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
-    // The K order must not depend on anything but the points (the sums'
-    // rounding follows it), so the points are ordered by the stable sort.
-    const int32_t *perm = nullptr;
-    const int tnkb = (int)((A + 31) / 32);
     const int64_t kchunk = ((5 * P + splits - 1) / splits + 15) / 16 * 16;
-    const int32_t cfg = c->mo.tile_point_order;
-    const bool order = cfg >= 0 ? cfg == 1 : round_up(A, 64) >= 1024 && P >= 4096;
-    if (order && tnkb <= kMaxFlagBlocks && kchunk / 16 <= 2 * kMaxFlagBlocks) {
-      if (!grow_bytes(c, reinterpret_cast<void **>(&mo.adj_perm), &mo.adj_perm_cap, (int64_t)sizeof(int32_t) * P) ||
-          !grow_bytes(c, &mo.adj_order_ws, &mo.adj_order_ws_cap, point_order_stable_workspace_bytes(P)))
-        return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
-      e = launch_point_order_stable(c->stream, P, c->point.coord, c->nucleus.num, c->nucleus.coord,
-                                    mo.adj_order_ws, mo.adj_perm);
-      if (e != hipSuccess) return hip_check(context, e, fn);
-      perm = mo.adj_perm;
-    }
-    if (perm != nullptr) {
-      if (!grow_bytes(c, reinterpret_cast<void **>(&mo.adj_kmap), &mo.adj_kmap_cap,
-                      (int64_t)sizeof(int32_t) * 5 * P) ||
-          !grow_bytes(c, reinterpret_cast<void **>(&mo.adj_tflags), &mo.adj_tflags_cap, (P + 63) / 64 * tnkb))
-        return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
-      e = launch_point_rows_map(c->stream, P, perm, mo.adj_kmap);
-      if (e == hipSuccess)
-        e = launch_tile_screen(c->stream, c->ao.dev, P, c->point.coord, perm, c->nucleus.coord, mo.adj_tflags, tnkb);
-    }
+    const bool tiles = tflags != nullptr && tnkb <= kMaxFlagBlocks && kchunk / 16 <= 2 * kMaxFlagBlocks;
     int64_t nk4w = 0;
     if (e == hipSuccess)
-      e = launch_frag_b_f64(c->stream, 5 * P, M, weight, M, mo.adj_wfrag, &nk4w, nullptr,
-                            perm != nullptr ? mo.adj_kmap : nullptr);
+      e = launch_frag_b_f64(c->stream, 5 * P, M, weight, M, mo.adj_wfrag, &nk4w, nullptr, tiles ? kmap : nullptr);
     if (e == hipSuccess) {
       KernelTimer t(c, "adjoint_dc_gemm_f64");
       e = launch_gemm_tn_f64_frag(c->stream, A, 5 * P, M, c->ao.ao_vgl, A, mo.adj_wfrag, nk4w, d_mo_coefficient,
-                                  mo.adj_partC, splits, perm != nullptr ? mo.adj_kmap : nullptr,
-                                  perm != nullptr ? mo.adj_tflags : nullptr, tnkb);
+                                  mo.adj_partC, splits, tiles ? kmap : nullptr, tiles ? tflags : nullptr, tnkb);
       t.stop();
     }
     if (e == hipSuccess) e = sync(c);

@@ -476,16 +476,20 @@ int f64_frag_cbw(int64_t Ncol);
 hipError_t launch_frag_b_f64(hipStream_t s, int64_t K, int64_t Ncol, const double *B, int64_t ldb,
                              double *Bf, int64_t *nk4, const int32_t *col_mo = nullptr,
                              const int32_t *row_map = nullptr);
-// out[r][j] = in[r][map[j]] for r < rows, j < cols.
-hipError_t launch_gather_cols(hipStream_t s, int64_t rows, int64_t cols, const double *in, int64_t ld_in,
-                              const int32_t *map, double *out, int64_t ld_out);
+// out[r][j] = in[rmap[r]][cmap[j]] for r < rows, j < cols (a null map: the identity).
+hipError_t launch_gather(hipStream_t s, int64_t rows, int64_t cols, const double *in, int64_t ld_in,
+                         const int32_t *rmap, const int32_t *cmap, double *out, int64_t ld_out);
 // With a column plan, Bf is in the plan's column order (launch_frag_b_f64
 // with plan.col_mo), column j of the product is stored at C column
 // plan.col_mo[j], and the MFMAs and coefficient loads of (column block,
-// 16-AO block) pairs whose coefficients are all zero are skipped.
+// 16-AO block) pairs whose coefficients are all zero are skipped.  ocols
+// (onkb bytes per tile of 320 output rows, one per 32 output columns; not
+// with plan.col_mo): a 16-column block of a tile whose flag is zero is
+// neither computed nor stored.
 hipError_t launch_mo_gemm_f64_frag(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const double *A,
                                    int64_t lda, const double *Bf, int64_t nk4, double *C, int64_t ldc,
-                                   const uint8_t *flags = nullptr, int nkb = 0, ColPlan plan = ColPlan());
+                                   const uint8_t *flags = nullptr, int nkb = 0, ColPlan plan = ColPlan(),
+                                   const uint8_t *ocols = nullptr, int onkb = 0);
 // Nonzero map of coef [M][K] at 16-AO granularity into words [M][nw],
 // nw = ceil(ceil(K / 16) / 64): bit s of words[m * nw + s / 64] is set when
 // MO m has a nonzero (or NaN) coefficient among the AOs [16s, 16s + 16).
@@ -523,8 +527,11 @@ hipError_t launch_point_rows_map(hipStream_t s, int64_t P, const int32_t *perm, 
 // AO of the block is then zero at every point of the tile).
 hipError_t launch_tile_screen(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
                               const int32_t *perm, const double *nucl, uint8_t *tflags, int nkb);
+// Per-block partials of dL/dR and dL/dalpha from G: block j holds the
+// points perm[64 j ..] (perm null: 64 j ..), whose G rows are 5 i + c for
+// i = 64 j .. (G's rows in the order of perm).
 hipError_t launch_ao_adjoint(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
-                             const double *nucl, const double *ao_factor, const double *G,
+                             const int32_t *perm, const double *nucl, const double *ao_factor, const double *G,
                              double *partR, double *partA);
 int64_t ao_adjoint_blocks(int64_t P);
 hipError_t launch_adjoint_reduce(hipStream_t s, const DeviceBasis &b, int64_t nblk,

@@ -20,7 +20,12 @@
 // has enough wavefronts), wavefront = one shell at a time, lane =
 // point; per (point, shell) the lane evaluates g..g''', loops over the
 // shell's AOs (monomials to third order), and reduces over the wavefront in
-// a fixed shuffle order.  Each (workgroup, shell) writes one partial dR and
+// a fixed order.  The 64 points of a block are those of a spatially compact
+// tile when the caller orders them (perm), so that most shells are
+// screened for every point of the block and skipped whole: on 64 waters at
+// 32 walkers 26% of the (block, shell) pairs stay live in tile order, 47% in
+// walker order, against the 87% the nucleus screen alone leaves.  Each
+// (workgroup, shell) writes one partial dR and
 // each (workgroup, primitive) one partial dalpha; a second kernel sums the
 // partials in a fixed order, so the result is bitwise reproducible.
 #include <hip/hip_runtime.h>
@@ -53,23 +58,57 @@ __device__ inline void pow4(double x, int a, double *p) {
   p[3] = f3;
 }
 
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+// Sum over the wavefront, the same value in every lane, without LDS:
+// neighbours by DPP quad permutes (xor 1, xor 2), the mirrors of 8 and 16
+// lanes, then v_permlane16_swap / v_permlane32_swap for the 16-lane rows
+// and the 32-lane halves.  At every step the two partners add the same two
+// values, so all lanes hold the same bits; the order is fixed.  (The
+// shuffles through ds_bpermute it replaces cost six dependent LDS round
+// trips per sum, and the kernel takes three sums per shell and one per
+// primitive.)
+template <int CTRL>
+__device__ __attribute__((always_inline)) inline double dpp_mov(double x) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, x);
+  const uint32_t lo = __builtin_amdgcn_update_dpp(0u, (uint32_t)u, CTRL, 0xf, 0xf, false);
+  const uint32_t hi = __builtin_amdgcn_update_dpp(0u, (uint32_t)(u >> 32), CTRL, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+// x of this lane plus x of the lane in the partner row (16-lane rows 0<->1,
+// 2<->3) or half (lanes 0-31 <-> 32-63)
+template <bool HALVES>
+__device__ __attribute__((always_inline)) inline double swap_add(double x) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, x);
+  const uint32_t l = (uint32_t)u, h = (uint32_t)(u >> 32);
+  const auto lo = HALVES ? __builtin_amdgcn_permlane32_swap(l, l, false, false)
+                         : __builtin_amdgcn_permlane16_swap(l, l, false, false);
+  const auto hi = HALVES ? __builtin_amdgcn_permlane32_swap(h, h, false, false)
+                         : __builtin_amdgcn_permlane16_swap(h, h, false, false);
+  // [0]: the lower row (half) everywhere, [1]: the upper one
+  return __builtin_bit_cast(double, ((uint64_t)hi[0] << 32) | lo[0]) +
+         __builtin_bit_cast(double, ((uint64_t)hi[1] << 32) | lo[1]);
+}
+__device__ __attribute__((always_inline)) inline double wave_sum(double v) {
+  v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_mov<0x141>(v);  // row_half_mirror
+  v += dpp_mov<0x140>(v);  // row_mirror
+  v = swap_add<false>(v);
+  return swap_add<true>(v);
 }
 
 __global__ void __launch_bounds__(kThreads)
 ao_adjoint_kernel(int64_t P, int S, int A, int N, const double *__restrict__ coord,
-                  const double *__restrict__ nucl, const ShellDesc *__restrict__ shells,
+                  const int32_t *__restrict__ perm, const double *__restrict__ nucl,
+                  const ShellDesc *__restrict__ shells,
                   const int32_t *__restrict__ ao_abc, const double *__restrict__ ao_factor,
                   const double *__restrict__ expo, const double *__restrict__ coef,
                   const double *__restrict__ nucl_range, const double *__restrict__ G,
                   double *__restrict__ partR, double *__restrict__ partA, int64_t Np, int sgroups) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t blk = blockIdx.x;
-  const int64_t p = blk * kTP + lane;
-  const bool inb = p < P;
+  const int64_t i = blk * kTP + lane;  // this lane's G rows are 5 i + c
+  const bool inb = i < P;
+  const int64_t p = inb && perm != nullptr ? (int64_t)perm[i] : i;
   const double rx = inb ? coord[p] : 0.0;
   const double ry = inb ? coord[P + p] : 0.0;
   const double rz = inb ? coord[2 * P + p] : 0.0;
@@ -89,18 +128,32 @@ ao_adjoint_kernel(int64_t P, int S, int A, int N, const double *__restrict__ coo
     const double dy = ry - nucl[N + d.nucl];
     const double dz = rz - nucl[2 * N + d.nucl];
     const double s = fma(dx, dx, fma(dy, dy, dz * dz));
-    const bool live = inb && !(s > kCutoff * nucl_range[d.nucl]);
-    const bool any_live = __any(live);
+    // A lane is live when the nucleus screen passes and some primitive is
+    // not screened: a_min s > cutoff means a s > cutoff for every exponent
+    // of the shell (a >= a_min, and rounding is monotonic), so every term of
+    // a lane that is not live is zero.
+    const bool live = inb && !(s > kCutoff * nucl_range[d.nucl]) && !(d.a_min * s > kCutoff);
+    if (!__any(live)) {
+      // what the sums below give for a shell that no lane reaches: -(+0)
+      // for the coordinates, +0 for the exponents
+      if (lane == 0) {
+        myR[(int64_t)d.shell * 3 + 0] = -0.0;
+        myR[(int64_t)d.shell * 3 + 1] = -0.0;
+        myR[(int64_t)d.shell * 3 + 2] = -0.0;
+        for (int j = 0; j < d.prim_num; ++j) myA[d.prim_start + j] = 0.0;
+      }
+      continue;
+    }
     // G of the shell's AOs is read one AO ahead of its use, the first AO's
     // before the primitive sums, so that the loads (64 rows 5 A doubles
     // apart per instruction) are in flight while the lane computes
-    const double *Gp = G + (p * 5) * (int64_t)A;
+    const double *Gp = G + (i * 5) * (int64_t)A;
     double wn[5];
     auto load_g = [&](int k) {
 #pragma unroll
       for (int c = 0; c < 5; ++c) wn[c] = live ? Gp[(int64_t)c * A + k] : 0.0;
     };
-    if (any_live) load_g(d.ao_start);
+    load_g(d.ao_start);
     double g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
     for (int j = 0; j < d.prim_num; ++j) {
       const double a = expo[d.prim_start + j];
@@ -115,7 +168,7 @@ ao_adjoint_kernel(int64_t P, int S, int A, int N, const double *__restrict__ coo
     }
     double gR0 = 0.0, gR1 = 0.0, gR2 = 0.0, U0 = 0.0, U1 = 0.0, U2 = 0.0;
     const int ncart = (d.l + 1) * (d.l + 2) / 2;
-    if (any_live) {
+    {
       const double dd[3] = {dx, dy, dz};
       const double gi[3] = {2.0 * dx * g1, 2.0 * dy * g1, 2.0 * dz * g1};
       const double lapg = 4.0 * s * g2 + 6.0 * g1;
@@ -229,7 +282,7 @@ adjoint_reduce_kernel(int64_t nblk, int S, int N, int64_t Np, const int64_t *__r
 }  // namespace
 
 hipError_t launch_ao_adjoint(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
-                             const double *nucl, const double *ao_factor, const double *G,
+                             const int32_t *perm, const double *nucl, const double *ao_factor, const double *G,
                              double *partR, double *partA) {
   if (P <= 0) return hipSuccess;
   const int64_t blocks = (P + kTP - 1) / kTP;
@@ -240,7 +293,7 @@ hipError_t launch_ao_adjoint(hipStream_t s, const DeviceBasis &b, int64_t P, con
   if (groups < 1) groups = 1;
   if (blocks > 0x7fffffff || groups > 65535) return hipErrorInvalidValue;
   hipLaunchKernelGGL(ao_adjoint_kernel, dim3((unsigned)blocks, (unsigned)groups), dim3(kThreads), 0, s, P,
-                     (int)b.shell_num, (int)b.ao_num, (int)b.nucl_num, coord, nucl, b.shells, b.ao_abc,
+                     (int)b.shell_num, (int)b.ao_num, (int)b.nucl_num, coord, perm, nucl, b.shells, b.ao_abc,
                      ao_factor, b.expo, b.coef_norm, b.nucl_range, G, partR, partA, b.prim_num, (int)groups);
   return hipGetLastError();
 }

@@ -175,14 +175,16 @@ frag_b_f64_kernel(int64_t K, int64_t Ncol, const double *__restrict__ B, int64_t
   }
 }
 
-// out[r][j] = in[r][map[j]]
+// out[r][j] = in[rmap[r]][cmap[j]] (a null map: the identity)
 __global__ void __launch_bounds__(kThreads)
-gather_cols_kernel(int64_t rows, int64_t cols, const double *__restrict__ in, int64_t ld_in,
-                   const int32_t *__restrict__ map, double *__restrict__ out, int64_t ld_out) {
+gather_kernel(int64_t rows, int64_t cols, const double *__restrict__ in, int64_t ld_in,
+              const int32_t *__restrict__ rmap, const int32_t *__restrict__ cmap, double *__restrict__ out,
+              int64_t ld_out) {
   const int64_t n = rows * cols;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t r = i / cols, j = i - r * cols;
-    out[r * ld_out + j] = in[r * ld_in + map[j]];
+    const int64_t rs = rmap != nullptr ? (int64_t)rmap[r] : r;
+    out[r * ld_out + j] = in[rs * ld_in + (cmap != nullptr ? (int64_t)cmap[j] : j)];
   }
 }
 
@@ -206,6 +208,12 @@ coef_blocks_kernel(int64_t M, int64_t K, const double *__restrict__ coef, int64_
 // TRANSA: the A operand is stored K-major (A[k*lda + row]).  gridDim.z
 // splits K into chunks of kchunk (a multiple of 16); split z writes its
 // partial product to C + z*R*ldc.  flags are for the unsplit provider path.
+// tflags (tnkb bytes per tile of 320 rows of the 64-point tiles' (point,
+// component) rows): with TRANSA the K rows' tiles, flags over the output
+// rows in 32-row blocks; without, the output rows' tiles, flags over the
+// output columns in 32-column blocks: a 16-column block whose flag is zero
+// is neither computed nor written (its entries keep what they held; the
+// caller reads none of them).
 template <int CBW, bool TRANSA>
 __global__ void __launch_bounds__(kThreads, 2)
 gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restrict__ A, int64_t lda,
@@ -229,8 +237,17 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
   const int nst = kend > kbeg ? (int)((kend - kbeg + kBK - 1) / kBK) : 0;
   C += (int64_t)blockIdx.z * R * ldc;
 
+  // output column blocks gb0 + b that the tile's flags keep
+  uint32_t omask = 0xffffffffu;
+  if (!TRANSA && tflags != nullptr) {
+    const uint8_t *tf = tflags + (row0 / 320) * tnkb;
+    omask = 0;
+#pragma unroll
+    for (int b = 0; b < 4 * CBW; ++b)
+      if ((gb0 + b) * 16 < Ncol && tf[(gb0 + b) >> 1] != 0) omask |= 1u << b;
+  }
   // stage list: live stages and, per stage, bit b for live column block gb0 + b
-  const bool listed = flags != nullptr || plan.blk != nullptr || (TRANSA && tflags != nullptr);
+  const bool listed = flags != nullptr || plan.blk != nullptr || tflags != nullptr;
   int count = nst;
   if (listed) {
     if (wave == 0) {
@@ -247,7 +264,8 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
           const int64_t b0 = 2 * (int64_t)blockIdx.x;
           live = tf[b0] != 0 || (b0 + 1 < tnkb && tf[b0 + 1] != 0);
         }
-        uint32_t lv = 0xffffffffu;
+        uint32_t lv = omask;
+        if (!TRANSA && tflags != nullptr) live = live && omask != 0;
         if (live && plan.blk != nullptr) {
           // all loads first (independent, no branches), then the bits
           const int64_t g = g0 + st;
@@ -257,10 +275,11 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
             const int64_t blk = gb0 + b < plan.nblk ? gb0 + b : plan.nblk - 1;
             w[b] = plan.blk[blk * plan.nw + (g >> 6)];
           }
-          lv = 0;
+          uint32_t bits = 0;
 #pragma unroll
           for (int b = 0; b < 4 * CBW; ++b)
-            lv |= (gb0 + b < plan.nblk ? (uint32_t)((w[b] >> (g & 63)) & 1ull) : 0u) << b;
+            bits |= (gb0 + b < plan.nblk ? (uint32_t)((w[b] >> (g & 63)) & 1ull) : 0u) << b;
+          lv &= bits;
           live = lv != 0;
         }
         const uint64_t m = __ballot(live);
@@ -392,7 +411,7 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
 #pragma unroll
     for (int cb = 0; cb < CBW; ++cb) {
       const int64_t col = (cb0 + 4 * cb) * 16 + li;
-      if (col >= Ncol) continue;
+      if (col >= Ncol || !((omask >> (wv + 4 * cb)) & 1u)) continue;
 #pragma unroll
       for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
@@ -451,8 +470,8 @@ hipError_t launch_frag(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const 
   // the forward product addresses one workgroup's column blocks with 32-bit offsets
   if (!TRANSA && (int64_t)4 * CBW * nk4 * 512 >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
   if (TRANSA && (plan.col_mo != nullptr || plan.blk != nullptr)) return hipErrorInvalidValue;
-  if (!TRANSA && (kmap != nullptr || tflags != nullptr)) return hipErrorInvalidValue;
-  if (tflags != nullptr && ((int64_t)tnkb * 32 < R || (kchunk + kBK - 1) / kBK > kMaxStages16))
+  if (!TRANSA && (kmap != nullptr || (tflags != nullptr && plan.col_mo != nullptr))) return hipErrorInvalidValue;
+  if (tflags != nullptr && ((int64_t)tnkb * 32 < (TRANSA ? R : Ncol) || (kchunk + kBK - 1) / kBK > kMaxStages16))
     return hipErrorInvalidValue;
   hipLaunchKernelGGL((gemm_f64_frag_kernel<CBW, TRANSA>), dim3((unsigned)gx, (unsigned)gy, (unsigned)splits),
                      dim3(kThreads), 0, s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, kchunk, plan, kmap,
@@ -525,14 +544,14 @@ hipError_t launch_frag_b_f64(hipStream_t s, int64_t K, int64_t Ncol, const doubl
   return hipGetLastError();
 }
 
-hipError_t launch_gather_cols(hipStream_t s, int64_t rows, int64_t cols, const double *in, int64_t ld_in,
-                              const int32_t *map, double *out, int64_t ld_out) {
+hipError_t launch_gather(hipStream_t s, int64_t rows, int64_t cols, const double *in, int64_t ld_in,
+                         const int32_t *rmap, const int32_t *cmap, double *out, int64_t ld_out) {
   const int64_t n = rows * cols;
   if (n <= 0) return hipSuccess;
   int64_t g = (n + kThreads - 1) / kThreads;
   if (g > 65536) g = 65536;
-  hipLaunchKernelGGL(gather_cols_kernel, dim3((unsigned)g), dim3(kThreads), 0, s, rows, cols, in, ld_in, map, out,
-                     ld_out);
+  hipLaunchKernelGGL(gather_kernel, dim3((unsigned)g), dim3(kThreads), 0, s, rows, cols, in, ld_in, rmap, cmap,
+                     out, ld_out);
   return hipGetLastError();
 }
 
@@ -548,10 +567,10 @@ hipError_t launch_coef_blocks(hipStream_t s, int64_t M, int64_t K, const double 
 
 hipError_t launch_mo_gemm_f64_frag(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const double *A,
                                    int64_t lda, const double *Bf, int64_t nk4, double *C, int64_t ldc,
-                                   const uint8_t *flags, int nkb, ColPlan plan) {
+                                   const uint8_t *flags, int nkb, ColPlan plan, const uint8_t *ocols, int onkb) {
   if (R <= 0 || Ncol <= 0) return hipSuccess;
   if (K <= 0 || nk4 * 4 < K) return hipErrorInvalidValue;
-  return dispatch_frag<false>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, 1, plan);
+  return dispatch_frag<false>(s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, 1, plan, nullptr, ocols, onkb);
 }
 
 int gemm_tn_frag_splits(int64_t R, int64_t K) {

@@ -19,9 +19,10 @@ from helpers import say
 pytestmark = pytest.mark.gpu
 
 
-def gpu_adjoint(s, w):
+def gpu_adjoint(s, w, point_order=-1):
     with Context(0) as c:
         c.set_system(s)
+        c.call("qmckl_set_mo_basis_tile_config_device", point_order, 0)
         wd = DeviceArray.from_numpy(c, w)
         dR = DeviceArray(c, (s.nucl_num, 3))
         dA = DeviceArray(c, (s.prim_num,))
@@ -40,8 +41,11 @@ def rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
-@pytest.mark.parametrize("name", ["n2_like", "chbrclf", "water64_2w"])
-def test_adjoint_vs_oracle(oracle_lib, name):
+@pytest.mark.parametrize("name,point_order", [("n2_like", -1), ("chbrclf", -1), ("water64_2w", -1),
+                                              ("n2_like", 1), ("chbrclf", 1), ("water64_2w", 1)])
+def test_adjoint_vs_oracle(oracle_lib, name, point_order):
+    """point_order 1 visits the points in spatial tiles on these small
+    systems too (136 to 1,280 points, the last tile ragged)."""
     if name == "n2_like":
         s = S.n2_like(walk_num=3)
     elif name == "chbrclf":
@@ -49,10 +53,10 @@ def test_adjoint_vs_oracle(oracle_lib, name):
     else:
         s = S.water64(walk_num=2)
     w = np.random.default_rng(5).normal(size=(s.point_num, 5, s.mo_num))
-    gR, gA, gC = gpu_adjoint(s, w)
+    gR, gA, gC = gpu_adjoint(s, w, point_order)
     oR, oA, oC = oracle_lib.mo_vgl_adjoint(s, w)
     e = (rel(gR, oR), rel(gA, oA), rel(gC, oC))
-    say(f"{name}: rel err dR {e[0]:.2e} dalpha {e[1]:.2e} dC {e[2]:.2e}")
+    say(f"{name} point_order {point_order}: rel err dR {e[0]:.2e} dalpha {e[1]:.2e} dC {e[2]:.2e}")
     assert max(e) <= 1e-10, e
 
 

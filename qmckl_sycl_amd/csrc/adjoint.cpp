@@ -71,9 +71,9 @@ qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_adjoint_device(qmckl_context_de
     perm = mo.adj_perm;
     kmap = mo.adj_kmap;
   }
-  // the AO screen per (64-point tile, 32-AO block): where it is zero, every
+  // the AO screen per (64-point tile, 16-AO block): where it is zero, every
   // AO of the block vanishes at every point of the tile
-  const int tnkb = (int)((A + 31) / 32);
+  const int tnkb = (int)((A + 15) / 16);
   const uint8_t *tflags = nullptr;
   if (perm != nullptr) {
     if (!grow_bytes(c, reinterpret_cast<void **>(&mo.adj_tflags), &mo.adj_tflags_cap, (P + 63) / 64 * tnkb))
@@ -126,6 +126,7 @@ qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_adjoint_device(qmckl_context_de
     // AO adjoint reads G only where a shell passes the same screen for its
     // point, so those blocks are neither computed nor stored.
     const bool gather = mo.adj_planned || kmap != nullptr;
+    const uint8_t *gcols = (M + 15) / 16 <= 2 * kMaxFlagBlocks ? tflags : nullptr;  // within the stage list
     if (gather && !grow_d(c, &mo.adj_wperm, &mo.adj_wperm_cap, 5 * P * M))
       return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
     {
@@ -140,10 +141,10 @@ qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_adjoint_device(qmckl_context_de
         gp.nw = mo.adj_blk_nw;
         gp.nblk = (A + 15) / 16;
         e = launch_mo_gemm_f64_frag(c->stream, 5 * P, M, A, wa, M, mo.adj_cfrag, mo.adj_cfrag_nk4, mo.adj_G, A,
-                                    nullptr, 0, gp, tflags, tnkb);
+                                    nullptr, 0, gp, gcols, tnkb);
       } else if (e == hipSuccess) {
         e = launch_mo_gemm_f64_frag(c->stream, 5 * P, M, A, wa, M, mo.adj_cfrag, mo.adj_cfrag_nk4, mo.adj_G, A,
-                                    nullptr, 0, ColPlan(), tflags, tnkb);
+                                    nullptr, 0, ColPlan(), gcols, tnkb);
       }
       t.stop();
     }
@@ -178,7 +179,7 @@ qmckl_exit_code_device qmckl_get_mo_basis_mo_vgl_adjoint_device(qmckl_context_de
         !grow_d(c, &mo.adj_wfrag, &mo.adj_wfrag_cap, wlen))
       return fail(context, QMCKL_ALLOCATION_FAILED_DEVICE, fn, nullptr);
     const int64_t kchunk = ((5 * P + splits - 1) / splits + 15) / 16 * 16;
-    const bool tiles = tflags != nullptr && tnkb <= kMaxFlagBlocks && kchunk / 16 <= 2 * kMaxFlagBlocks;
+    const bool tiles = tflags != nullptr && kchunk / 16 <= 2 * kMaxFlagBlocks;
     int64_t nk4w = 0;
     if (e == hipSuccess)
       e = launch_frag_b_f64(c->stream, 5 * P, M, weight, M, mo.adj_wfrag, &nk4w, nullptr, tiles ? kmap : nullptr);

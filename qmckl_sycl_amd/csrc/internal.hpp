@@ -268,7 +268,7 @@ struct MoBasis {
   int64_t adj_perm_cap = 0;
   void *adj_order_ws = nullptr;
   int64_t adj_order_ws_cap = 0;
-  uint8_t *adj_tflags = nullptr;  // screen flags per (64-point tile, 32-AO block)
+  uint8_t *adj_tflags = nullptr;  // screen flags per (64-point tile, 16-AO block)
   int64_t adj_tflags_cap = 0;
   // with a column plan, G = w C runs with K over the MOs in the plan's
   // order: the weights' columns gathered into that order (per call), the
@@ -483,7 +483,7 @@ hipError_t launch_gather(hipStream_t s, int64_t rows, int64_t cols, const double
 // with plan.col_mo), column j of the product is stored at C column
 // plan.col_mo[j], and the MFMAs and coefficient loads of (column block,
 // 16-AO block) pairs whose coefficients are all zero are skipped.  ocols
-// (onkb bytes per tile of 320 output rows, one per 32 output columns; not
+// (onkb bytes per tile of 320 output rows, one per 16 output columns; not
 // with plan.col_mo): a 16-column block of a tile whose flag is zero is
 // neither computed nor stored.
 hipError_t launch_mo_gemm_f64_frag(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const double *A,
@@ -512,7 +512,7 @@ int64_t plan_mo_columns(const uint64_t *words, int64_t M, int64_t nw, int64_t nk
 // over K into `splits` partials [splits][R][Ncol] reduced in a fixed order.
 // With kmap (K-row map, [K]), row k of the product is row kmap[k] of A (B
 // already in the mapped order); with tflags ([K/320 rounded up][tnkb]), the
-// K rows of 320-row tile t contribute nothing to the 32-row output blocks b
+// K rows of 320-row tile t contribute nothing to the 16-row output blocks b
 // whose flag (t, b) is zero, and are skipped.
 int gemm_tn_frag_splits(int64_t R, int64_t K);
 hipError_t launch_gemm_tn_f64_frag(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const double *A,
@@ -522,9 +522,10 @@ hipError_t launch_gemm_tn_f64_frag(hipStream_t s, int64_t R, int64_t K, int64_t 
 // kmap[5 i + c] = 5 perm[i] + c for the (point, component) rows of P points.
 hipError_t launch_point_rows_map(hipStream_t s, int64_t P, const int32_t *perm, int32_t *kmap);
 // Screen flags of P points in 64-point tiles (tile t = points perm[64 t ..],
-// perm null: in order): tflags[t][b] = 1 when some shell overlapping 32-AO
+// perm null: in order): tflags[t][b] = 1 when some shell overlapping 16-AO
 // block b passes the AO screen for some point of the tile, else 0 (every
-// AO of the block is then zero at every point of the tile).
+// AO of the block is then zero at every point of the tile); nkb = A / 16
+// rounded up.
 hipError_t launch_tile_screen(hipStream_t s, const DeviceBasis &b, int64_t P, const double *coord,
                               const int32_t *perm, const double *nucl, uint8_t *tflags, int nkb);
 // Per-block partials of dL/dR and dL/dalpha from G: block j holds the

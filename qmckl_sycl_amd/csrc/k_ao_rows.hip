@@ -604,9 +604,10 @@ ao_vgl_f64_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord,
   }
 }
 
-// Screen flags per (64-point tile, 32-AO block): wavefront = (tile, group
-// of 64-AO stages), lane = point; per stage the lane screen of each half,
-// ORed over the tile's points by a ballot.
+// Screen flags per (64-point tile, 16-AO block): wavefront = (tile, group
+// of 64-AO stages), lane = point; per stage the lane's screen of each
+// quarter (a shell counts for every quarter its AOs overlap, under the
+// test of lane_screen), ORed over the tile's points by a ballot.
 __global__ void __launch_bounds__(kThreads)
 tile_screen_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord, const int32_t *__restrict__ perm,
                    const double *__restrict__ nucl, const ShellDesc *__restrict__ shells,
@@ -628,12 +629,26 @@ tile_screen_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord
   for (int64_t st = g * nst / groups; st < (g + 1) * nst / groups; ++st) {
     const int k_lo = (int)(64 * st);
     const int k_end = (int)(64 * (st + 1) < A ? 64 * (st + 1) : A);
-    const uint32_t hs = lane_screen(stage_shell[st], S, k_lo, k_end, N, inside, px, py, pz, nucl, shells,
-                                    nucl_range);
-    const bool h0 = __ballot(hs & 1u) != 0, h1 = __ballot(hs & 2u) != 0;
-    if (lane == 0) {
-      tflags[pt * nkb + 2 * st] = h0 ? 1 : 0;
-      if (2 * st + 1 < nkb) tflags[pt * nkb + 2 * st + 1] = h1 ? 1 : 0;
+    uint32_t qs = 0;
+    for (int64_t si = stage_shell[st]; si < S; ++si) {
+      const ShellDesc d = shells[si];
+      if (d.ao_start >= k_end) break;
+      const double x = px - nucl[d.nucl];
+      const double y = py - nucl[N + d.nucl];
+      const double z = pz - nucl[2 * N + d.nucl];
+      const double r2 = fma(x, x, fma(y, y, z * z));
+      const bool live = inside && !(r2 > kCutoff * nucl_range[d.nucl]) && !(d.a_min * r2 > kCutoff);
+      if (live) {
+        const int a_end = d.ao_start + (d.l + 1) * (d.l + 2) / 2;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (d.ao_start < k_lo + 16 * (q + 1) && a_end > k_lo + 16 * q) qs |= 1u << q;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const bool any = __ballot((qs >> q) & 1u) != 0;
+      if (lane == 0 && 4 * st + q < nkb) tflags[pt * nkb + 4 * st + q] = any ? 1 : 0;
     }
   }
 }
@@ -731,7 +746,7 @@ hipError_t launch_tile_screen(hipStream_t s, const DeviceBasis &b, int64_t P, co
                               const int32_t *perm, const double *nucl, uint8_t *tflags, int nkb) {
   if (P <= 0) return hipSuccess;
   const int64_t A = b.ao_num;
-  if (A <= 0 || b.stage_shell == nullptr || (int64_t)nkb != (A + 31) / 32 || P > 0x7fffffff)
+  if (A <= 0 || b.stage_shell == nullptr || (int64_t)nkb != (A + 15) / 16 || P > 0x7fffffff)
     return hipErrorInvalidValue;
   const int64_t npt = (P + 63) / 64, nst = (A + 63) / 64;
   int64_t groups = (64 * 256 + npt - 1) / npt;  // about 16,384 wavefronts

@@ -210,10 +210,12 @@ coef_blocks_kernel(int64_t M, int64_t K, const double *__restrict__ coef, int64_
 // partial product to C + z*R*ldc.  flags are for the unsplit provider path.
 // tflags (tnkb bytes per tile of 320 rows of the 64-point tiles' (point,
 // component) rows): with TRANSA the K rows' tiles, flags over the output
-// rows in 32-row blocks; without, the output rows' tiles, flags over the
-// output columns in 32-column blocks: a 16-column block whose flag is zero
-// is neither computed nor written (its entries keep what they held; the
-// caller reads none of them).
+// rows in 16-row blocks: a stage's rows are neither loaded nor multiplied
+// into the 16-row blocks whose flag is zero, and a stage with none left is
+// dropped; without, the output rows' tiles, flags over the output columns
+// in 16-column blocks: a block whose flag is zero is neither computed nor
+// written (its entries keep what they held; the caller reads none of
+// them).
 template <int CBW, bool TRANSA>
 __global__ void __launch_bounds__(kThreads, 2)
 gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restrict__ A, int64_t lda,
@@ -244,7 +246,7 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
     omask = 0;
 #pragma unroll
     for (int b = 0; b < 4 * CBW; ++b)
-      if ((gb0 + b) * 16 < Ncol && tf[(gb0 + b) >> 1] != 0) omask |= 1u << b;
+      if ((gb0 + b) * 16 < Ncol && tf[gb0 + b] != 0) omask |= 1u << b;
   }
   // stage list: live stages and, per stage, bit b for live column block gb0 + b
   const bool listed = flags != nullptr || plan.blk != nullptr || tflags != nullptr;
@@ -257,14 +259,19 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
       for (int s0 = 0; s0 < nst; s0 += 64) {
         const int st = s0 + lane;
         bool live = st < nst && (f == nullptr || f[st >> 1] != 0);
+        uint32_t lv = omask;
         if (TRANSA && tflags != nullptr && live) {
           // the K rows of stage st lie in one 320-row tile (20 stages); the
-          // workgroup's 64 output rows are blocks 2 bx and 2 bx + 1
+          // workgroup's 64 output rows are 16-row blocks 4 bx .. 4 bx + 3;
+          // lv: bit rb for live row block rb
           const uint8_t *tf = tflags + ((g0 + st) / 20) * tnkb;
-          const int64_t b0 = 2 * (int64_t)blockIdx.x;
-          live = tf[b0] != 0 || (b0 + 1 < tnkb && tf[b0 + 1] != 0);
+          const int64_t b0 = 4 * (int64_t)blockIdx.x;
+          lv = 0;
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (b0 + q < tnkb && tf[b0 + q] != 0) lv |= 1u << q;
+          live = lv != 0;
         }
-        uint32_t lv = omask;
         if (!TRANSA && tflags != nullptr) live = live && omask != 0;
         if (live && plan.blk != nullptr) {
           // all loads first (independent, no branches), then the bits
@@ -296,8 +303,14 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
     count = s_count;
   }
   // this wavefront's live column blocks at stage s: bit 4j for block cb0 + 4j
+  // (TRANSA: all of them, and the live 16-row blocks in row_live)
   auto wave_live = [&](int s) -> uint32_t {
+    if (TRANSA) return 0xffffffffu;
     return listed ? (uint32_t)__builtin_amdgcn_readfirstlane(s_live[s]) >> wv : 0xffffffffu;
+  };
+  auto row_live = [&](int s) -> uint32_t {
+    if (!TRANSA || tflags == nullptr) return 0xfu;
+    return (uint32_t)__builtin_amdgcn_readfirstlane(s_live[s]);
   };
 
   // A slot: row ar, 4 consecutive K at aq (K-major storage: 4 consecutive
@@ -322,9 +335,9 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
     const int64_t k = stage_k0(s) + aq4;
     return k < kend ? kmap[k] : 0;
   };
-  auto load_a = [&](int64_t k0, int32_t kr_mapped) {
+  auto load_a = [&](int64_t k0, int32_t kr_mapped, uint32_t rows) {
     if constexpr (TRANSA) {
-      const bool kin = k0 + aq4 < kend;
+      const bool kin = k0 + aq4 < kend && ((rows >> (ar4 >> 4)) & 1u);  // rows of a dead 16-row block read zero
       const int64_t kr = kmap != nullptr ? (int64_t)kr_mapped : k0 + aq4;
 #pragma unroll
       for (int j = 0; j < 4; ++j) ra[j] = kin && row0 + ar4 + j < R ? Ap[kr * lda + j] : 0.0;
@@ -371,11 +384,12 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
 #pragma unroll
     for (int cb = 0; cb < CBW; ++cb) acc[rb][cb] = d4{0.0, 0.0, 0.0, 0.0};
 
-  uint32_t live0 = 0;
+  uint32_t live0 = 0, rows0 = 0;
   if (count > 0) {
     const int64_t k0 = stage_k0(0);
     live0 = wave_live(0);
-    load_a(k0, map_row(0));
+    rows0 = row_live(0);
+    load_a(k0, map_row(0), rows0);
     kr_next = map_row(1);
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) load_b(k0, ks, live0);
@@ -387,8 +401,9 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
     const bool more = s + 1 < count;
     const int64_t k1 = more ? stage_k0(s + 1) : 0;
     const uint32_t live1 = more ? wave_live(s + 1) : 0u;
+    const uint32_t rows1 = more ? row_live(s + 1) : 0u;
     if (more) {
-      load_a(k1, kr_next);
+      load_a(k1, kr_next, rows1);
       kr_next = map_row(s + 2);
     }
 #pragma unroll
@@ -401,11 +416,13 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
         if ((live0 >> (4 * cb)) & 1u) {
 #pragma unroll
           for (int rb = 0; rb < 4; ++rb)
-            acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], b[ks][cb], acc[rb][cb], 0, 0, 0);
+            if (!TRANSA || ((rows0 >> rb) & 1u))
+              acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], b[ks][cb], acc[rb][cb], 0, 0, 0);
         }
       if (more) load_b(k1, ks, live1);  // this K step's fragments are free again
     }
     live0 = live1;
+    rows0 = rows1;
   }
   if (TRANSA || plan.col_mo == nullptr) {
 #pragma unroll
@@ -471,7 +488,7 @@ hipError_t launch_frag(hipStream_t s, int64_t R, int64_t K, int64_t Ncol, const 
   if (!TRANSA && (int64_t)4 * CBW * nk4 * 512 >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
   if (TRANSA && (plan.col_mo != nullptr || plan.blk != nullptr)) return hipErrorInvalidValue;
   if (!TRANSA && (kmap != nullptr || (tflags != nullptr && plan.col_mo != nullptr))) return hipErrorInvalidValue;
-  if (tflags != nullptr && ((int64_t)tnkb * 32 < (TRANSA ? R : Ncol) || (kchunk + kBK - 1) / kBK > kMaxStages16))
+  if (tflags != nullptr && ((int64_t)tnkb * 16 < (TRANSA ? R : Ncol) || (kchunk + kBK - 1) / kBK > kMaxStages16))
     return hipErrorInvalidValue;
   hipLaunchKernelGGL((gemm_f64_frag_kernel<CBW, TRANSA>), dim3((unsigned)gx, (unsigned)gy, (unsigned)splits),
                      dim3(kThreads), 0, s, R, K, Ncol, A, lda, Bf, nk4, C, ldc, flags, nkb, kchunk, plan, kmap,

@@ -231,13 +231,28 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wv = __builtin_amdgcn_readfirstlane(wave);
   const int li = lane & 15, lk = lane >> 4;
-  const int64_t row0 = (int64_t)blockIdx.x * kBM;
-  const int64_t gb0 = (int64_t)blockIdx.y * 4 * CBW;  // the workgroup's first column block
-  const int64_t cb0 = gb0 + wv;                       // this wavefront's first column block
-  const int64_t kbeg = (int64_t)blockIdx.z * kchunk;
+  // The workgroup's tile (bx, by, bz).  With a K split the tiles are
+  // numbered so that each XCD takes consecutive ones: the workgroups are
+  // handed to the 8 XCDs in turn by their linear index, and with the tiles
+  // in grid order the row tiles of one K chunk, which all read that
+  // chunk's B fragments, would be spread over every XCD's L2 at once.
+  int64_t bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+  if (TRANSA && gridDim.z > 1) {
+    const int64_t n = (int64_t)gridDim.x * gridDim.y * gridDim.z;
+    const int64_t lin = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
+    const int64_t q = n / 8, r = n % 8, xcd = lin % 8, slot = lin / 8;
+    const int64_t t = xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+    bx = t % gridDim.x;
+    by = (t / gridDim.x) % gridDim.y;
+    bz = t / ((int64_t)gridDim.x * gridDim.y);
+  }
+  const int64_t row0 = bx * kBM;
+  const int64_t gb0 = by * 4 * CBW;  // the workgroup's first column block
+  const int64_t cb0 = gb0 + wv;      // this wavefront's first column block
+  const int64_t kbeg = bz * kchunk;
   const int64_t kend = kbeg + kchunk < K ? kbeg + kchunk : K;
   const int nst = kend > kbeg ? (int)((kend - kbeg + kBK - 1) / kBK) : 0;
-  C += (int64_t)blockIdx.z * R * ldc;
+  C += bz * R * ldc;
 
   // output column blocks gb0 + b that the tile's flags keep
   uint32_t omask = 0xffffffffu;
@@ -253,7 +268,7 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
   int count = nst;
   if (listed) {
     if (wave == 0) {
-      const uint8_t *f = flags != nullptr ? flags + (int64_t)blockIdx.x * nkb : nullptr;
+      const uint8_t *f = flags != nullptr ? flags + bx * nkb : nullptr;
       const int64_t g0 = kbeg / kBK;  // 16-AO block of the chunk's first stage
       int base = 0;
       for (int s0 = 0; s0 < nst; s0 += 64) {
@@ -265,7 +280,7 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
           // workgroup's 64 output rows are 16-row blocks 4 bx .. 4 bx + 3;
           // lv: bit rb for live row block rb
           const uint8_t *tf = tflags + ((g0 + st) / 20) * tnkb;
-          const int64_t b0 = 4 * (int64_t)blockIdx.x;
+          const int64_t b0 = 4 * bx;
           lv = 0;
 #pragma unroll
           for (int q = 0; q < 4; ++q)

@@ -171,6 +171,24 @@ __device__ __forceinline__ void build_stage_list(int lane, int nst, int64_t rt, 
   if (lane == 0) *s_count = base;
 }
 
+// The row-tile part bx (H parts of 64 / H rows per 64-row tile) and the
+// column group by of the t-th workgroup (or work item) of a split
+// contraction: the parts and the five components of one 64-point tile take
+// consecutive numbers, then the point tiles, then the column groups.  The
+// ten workgroups of a point tile have the same live stages and read the
+// same coefficient fragments, so they are dispatched together (in row
+// order, component-major, FP16X2 on 64 waters at 128 walkers ran 2.40 ms,
+// in this order 2.35 ms).
+__device__ __forceinline__ void split_item(int64_t t, int64_t npt, int64_t H, int64_t &bx, int64_t &by) {
+  const int64_t part = t % H;
+  int64_t u = t / H;
+  const int64_t comp = u % 5;
+  u /= 5;
+  const int64_t pt = u % npt;
+  by = u / npt;
+  bx = (comp * npt + pt) * H + part;
+}
+
 // One 32-row x 64*CBW-column tile (bx, by) of a split contraction; the
 // workgroup's LDS comes in as pointers so that one kernel can hold tiles of
 // several widths (the persistent batch kernel below).
@@ -405,8 +423,10 @@ gemm_split_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *
   __shared__ uint64_t s_live[kMaxStages];
   __shared__ int s_count;
   __shared__ double s_out[kOutRows * 64 * CBW];  // output rows in MO order (planned columns)
-  split_tile<SP, CBW, WR>(blockIdx.x, blockIdx.y, P, npt, K, Ncol, A, lda, rowmax, Bs, ldb, colinv, flags, nkb, perm,
-                          C, ldc, plan, sA, s_inv, s_list, s_live, &s_count, s_out);
+  int64_t bx, by;
+  split_item(blockIdx.x + (int64_t)gridDim.x * blockIdx.y, npt, 64 / WR, bx, by);
+  split_tile<SP, CBW, WR>(bx, by, P, npt, K, Ncol, A, lda, rowmax, Bs, ldb, colinv, flags, nkb, perm, C, ldc, plan,
+                          sA, s_inv, s_list, s_live, &s_count, s_out);
 }
 
 // Live-block form of the split tile (the default for the split forms).
@@ -681,15 +701,16 @@ gemm_split_live_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const fl
   __shared__ int32_t s_perm[16 * RB];
   __shared__ int32_t s_mloc[64 * CBW];
   __shared__ float s_ci[64 * CBW];
-  split_tile_live<SP, CBW, RB, NT>(blockIdx.x, blockIdx.y, P, npt, K, Ncol, A, lda, rowmax, Bs, ldb, colinv, flags,
-                                   nkb, perm, C, ldc, plan, sA, s_inv, s_list, s_live, &s_count, s_perm, s_mloc,
-                                   s_ci);
+  int64_t bx, by;
+  split_item(blockIdx.x + (int64_t)gridDim.x * blockIdx.y, npt, 64 / (16 * RB), bx, by);
+  split_tile_live<SP, CBW, RB, NT>(bx, by, P, npt, K, Ncol, A, lda, rowmax, Bs, ldb, colinv, flags, nkb, perm, C,
+                                   ldc, plan, sA, s_inv, s_list, s_live, &s_count, s_perm, s_mloc, s_ci);
 }
 
 // Persistent form over a batch: 2 workgroups per CU take (molecule, row
 // tile, column group) items from an atomic counter until the batch is done;
 // jobs[j] describes molecule j's contraction (all of tile width CBW) and
-// owns items [item0, item0 + items), x-major over its gx row-tile halves.
+// owns items [item0, item0 + items), in the order of split_item.
 // Every workgroup leaves the loop once the counter passes `total`.  Each
 // item runs split_tile_live (LIVE) or split_tile.
 template <int SP, int CBW, bool LIVE>
@@ -733,7 +754,8 @@ gemm_split_persistent_kernel(const MixedGemmJob *__restrict__ jobs, int njobs, i
     if (item >= total) return;
     const MixedGemmJob &g = jobs[j];
     const int64_t local = item - g.item0;
-    const int64_t bx = local % g.gx, by = local / g.gx;
+    int64_t bx, by;
+    split_item(local, g.npt, 2, bx, by);
     if constexpr (LIVE)
       split_tile_live<SP, CBW, 2, true>(bx, by, g.P, g.npt, g.K, g.Ncol, g.A, g.lda, g.rowmax, g.Bs, g.ldb,
                                         g.colinv, g.flags, g.nkb, g.perm, g.C, g.ldc, g.plan, sA, s_inv, s_list,

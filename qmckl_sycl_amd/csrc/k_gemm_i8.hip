@@ -133,10 +133,16 @@ gemm_i8_kernel(int64_t P, int64_t npt, int64_t K, int64_t Ncol, const float *__r
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lg = lane >> 4;
   const int cq = __builtin_amdgcn_readfirstlane(wave);
-  const int64_t rt = blockIdx.x / 2;
-  const int r0 = (int)(blockIdx.x % 2) * kRows;
-  const int64_t comp = rt / npt, p0 = (rt - comp * npt) * 64 + r0;
-  const int64_t gb0 = (int64_t)blockIdx.y * 4 * CBW;
+  // workgroup t: its 32-row half, component, 64-point tile and column
+  // group, in that order from the fastest (the ten workgroups of a point
+  // tile have the same live stages and are dispatched together)
+  const int64_t t = blockIdx.x + (int64_t)gridDim.x * blockIdx.y;
+  const int half = (int)(t % 2);
+  const int64_t comp = (t / 2) % 5, pt = (t / 10) % npt, by = t / (10 * npt);
+  const int64_t rt = comp * npt + pt;
+  const int r0 = half * kRows;
+  const int64_t p0 = pt * 64 + r0;
+  const int64_t gb0 = by * 4 * CBW;
   const int nst = (int)((K + kStageK - 1) / kStageK);
 
   // epilogue tables

@@ -131,7 +131,8 @@ ao_adjoint_kernel(int64_t P, int S, int A, int N, const double *__restrict__ coo
     // A lane is live when the nucleus screen passes and some primitive is
     // not screened: a_min s > cutoff means a s > cutoff for every exponent
     // of the shell (a >= a_min, and rounding is monotonic), so every term of
-    // a lane that is not live is zero.
+    // a lane that is not live is zero.  The test (and s) are those of the
+    // tile screen (k_ao_rows.hip), whose zero blocks of G are not written.
     const bool live = inb && !(s > kCutoff * nucl_range[d.nucl]) && !(d.a_min * s > kCutoff);
     if (!__any(live)) {
       // what the sums below give for a shell that no lane reaches: -(+0)

@@ -607,7 +607,11 @@ ao_vgl_f64_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord,
 // Screen flags per (64-point tile, 16-AO block): wavefront = (tile, group
 // of 64-AO stages), lane = point; per stage the lane's screen of each
 // quarter (a shell counts for every quarter its AOs overlap, under the
-// test of lane_screen), ORed over the tile's points by a ballot.
+// test of lane_screen), ORed over the tile's points by a ballot.  The
+// adjoint leaves the blocks of G whose flag is zero unwritten and its AO
+// kernel (k_adjoint.hip) reads G where its own test passes: the two tests
+// are the same expressions on the same inputs (r^2 by the same fmas), so
+// they agree bit for bit.
 __global__ void __launch_bounds__(kThreads)
 tile_screen_kernel(int64_t P, int N, int64_t S, const double *__restrict__ coord, const int32_t *__restrict__ perm,
                    const double *__restrict__ nucl, const ShellDesc *__restrict__ shells,

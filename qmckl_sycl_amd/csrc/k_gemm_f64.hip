@@ -317,11 +317,19 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
     __syncthreads();
     count = s_count;
   }
+  // this wavefront's column blocks that hold columns: bit 4j for block
+  // cb0 + 4j (the zero padding of the fragments past Ncol is neither
+  // loaded nor multiplied)
+  const int64_t nvb = (Ncol + 15) / 16 - gb0;
+  uint32_t vmask = 0;
+#pragma unroll
+  for (int cb = 0; cb < CBW; ++cb)
+    if (wv + 4 * cb < nvb) vmask |= 1u << (4 * cb);
   // this wavefront's live column blocks at stage s: bit 4j for block cb0 + 4j
-  // (TRANSA: all of them, and the live 16-row blocks in row_live)
+  // (TRANSA: all that hold columns, and the live 16-row blocks in row_live)
   auto wave_live = [&](int s) -> uint32_t {
-    if (TRANSA) return 0xffffffffu;
-    return listed ? (uint32_t)__builtin_amdgcn_readfirstlane(s_live[s]) >> wv : 0xffffffffu;
+    if (TRANSA || !listed) return vmask;
+    return ((uint32_t)__builtin_amdgcn_readfirstlane(s_live[s]) >> wv) & vmask;
   };
   auto row_live = [&](int s) -> uint32_t {
     if (!TRANSA || tflags == nullptr) return 0xfu;
@@ -384,7 +392,7 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
 #pragma unroll
     for (int cb = 0; cb < CBW; ++cb) {
       if constexpr (TRANSA) {
-        b[ks][cb] = Bp[cb * cb_stride + (k0 / 4 + ks) * 64];
+        if ((live >> (4 * cb)) & 1u) b[ks][cb] = Bp[cb * cb_stride + (k0 / 4 + ks) * 64];
       } else {
         const int vo = (live >> (4 * cb)) & 1u ? 8 * lane : (int)0x80000000;
         const int so = (wv + 4 * cb) * blk_bytes + (int)(k0 / 4 + ks) * 512;

@@ -213,6 +213,12 @@ __device__ __forceinline__ void split_tile(int64_t bx, int64_t by, int64_t P, in
   const int64_t comp = rt / npt, p0 = (rt - comp * npt) * 64 + r0;
   const int64_t gb0 = by * 4 * CBW;  // the workgroup's first column block
   const int nst = (int)((K + kStage - 1) / kStage);
+  // this wavefront's column blocks cq + 4 cb that hold MOs (the prepared
+  // coefficients are zero-padded to the workgroup width: on the 48-molecule
+  // batch 22% of the 64- and 128-wide tiles' columns are padding); the
+  // others are neither loaded nor multiplied
+  const int64_t nvb = (Ncol + 15) / 16 - gb0;
+  const int ncb = nvb <= cq ? 0 : (int)((nvb - cq + 3) / 4);
 
   // stage list: live stages and, per stage, bit b (first half) and 32 + b
   // (second half) for live column block gb0 + b
@@ -272,6 +278,7 @@ __device__ __forceinline__ void split_tile(int64_t bx, int64_t by, int64_t P, in
   auto load_b = [&](int64_t k0, int ks, uint32_t live) {
 #pragma unroll
     for (int cb = 0; cb < CBW; ++cb) {
+      if (cb >= ncb) continue;  // padding columns (wavefront-uniform)
       const int vo = (live >> (4 * cb)) & 1u ? lane_off : (int)0x80000000;
       const int so = (cq + 4 * cb) * tile_bytes + (int)(k0 / 32 + ks) * T * 1024;
 #pragma unroll
@@ -325,19 +332,23 @@ __device__ __forceinline__ void split_tile(int64_t bx, int64_t by, int64_t P, in
 #pragma unroll
       for (int pr = 0; pr < NP; ++pr)
 #pragma unroll
-        for (int cb = 0; cb < CBW; ++cb)
+        for (int cb = 0; cb < CBW; ++cb) {
+          if (cb >= ncb) continue;
 #pragma unroll
           for (int r = 0; r < RB; ++r)
             d[r][cb] = mfma<SP>(a[pa[pr]][r], bq[ks][cb][pb[pr]],
                                 pr == 0 && ks == 0 ? f4{0.f, 0.f, 0.f, 0.f} : d[r][cb]);
+        }
       if (more) load_b(k1, ks, live1[ks]);  // this half-stage's C registers are free again
     }
 #pragma unroll
     for (int r = 0; r < RB; ++r)
 #pragma unroll
-      for (int cb = 0; cb < CBW; ++cb)
+      for (int cb = 0; cb < CBW; ++cb) {
+        if (cb >= ncb) continue;
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[r][cb][j] += (double)d[r][cb][j];
+      }
   };
 
   if (count > 0) {
@@ -507,13 +518,21 @@ __device__ __forceinline__ void split_tile_live(int64_t bx, int64_t by, int64_t 
     __syncthreads();
     count = *s_count;
   }
+  // this wavefront's column blocks that hold MOs: bit 4j for block
+  // gb0 + cq + 4j (the padding past the last MO is neither loaded nor
+  // multiplied)
+  const int64_t nvb = (Ncol + 15) / 16 - gb0;
+  uint32_t vmask = 0;
+#pragma unroll
+  for (int cb = 0; cb < CBW; ++cb)
+    if (cq + 4 * cb < nvb) vmask |= 1u << (4 * cb);
   // this wavefront's live blocks in half ks of stage s: bit 4j for block
   // gb0 + cq + 4j
   auto wave_live = [&](int s, int ks) -> uint32_t {
-    if (!listed) return 0xffffffffu;
+    if (!listed) return vmask;
     const uint64_t lv = s_live[s];
     const uint32_t w = ks == 0 ? (uint32_t)lv : (uint32_t)(lv >> 32);
-    return (uint32_t)__builtin_amdgcn_readfirstlane(w) >> cq;
+    return ((uint32_t)__builtin_amdgcn_readfirstlane(w) >> cq) & vmask;
   };
 
   const int ar = tid % WR, aq = (tid / WR) * KQ;

@@ -392,7 +392,10 @@ gemm_f64_frag_kernel(int64_t R, int64_t K, int64_t Ncol, const double *__restric
 #pragma unroll
     for (int cb = 0; cb < CBW; ++cb) {
       if constexpr (TRANSA) {
-        if ((live >> (4 * cb)) & 1u) b[ks][cb] = Bp[cb * cb_stride + (k0 / 4 + ks) * 64];
+        // loaded whatever the live bits (a conditional load measured 0.2 ms
+        // slower on the adjoint's dC product); the fragments past Ncol are
+        // zero padding, in bounds, and not multiplied
+        b[ks][cb] = Bp[cb * cb_stride + (k0 / 4 + ks) * 64];
       } else {
         const int vo = (live >> (4 * cb)) & 1u ? 8 * lane : (int)0x80000000;
         const int so = (wv + 4 * cb) * blk_bytes + (int)(k0 / 4 + ks) * 512;

@@ -60,6 +60,32 @@ def test_adjoint_vs_oracle(oracle_lib, name, point_order):
     assert max(e) <= 1e-10, e
 
 
+def test_adjoint_after_points_move(oracle_lib):
+    """Two adjoint calls in one context with the points in spatial tiles,
+    the electrons moved between them: the blocks of G the screen zeroes are
+    not written, so the second call's G keeps the first call's values there,
+    and the second gradient must still match the oracle on the new points."""
+    s = S.water64(walk_num=2)
+    rng = np.random.default_rng(11)
+    w = rng.normal(size=(s.point_num, 5, s.mo_num))
+    moved = s.with_walkers(s.elec_coord + rng.normal(scale=0.7, size=s.elec_coord.shape))
+    with Context(0) as c:
+        c.set_system(s)
+        c.call("qmckl_set_mo_basis_tile_config_device", 1, 0)
+        wd = DeviceArray.from_numpy(c, w)
+        dR = DeviceArray(c, (s.nucl_num, 3))
+        dA = DeviceArray(c, (s.prim_num,))
+        dC = DeviceArray(c, (s.mo_num, s.ao_num))
+        c.call("qmckl_get_mo_basis_mo_vgl_adjoint_device", wd.ptr, wd.size, dR.ptr, dA.ptr, dC.ptr)
+        c.set_electron_coord(moved.elec_coord)
+        c.call("qmckl_get_mo_basis_mo_vgl_adjoint_device", wd.ptr, wd.size, dR.ptr, dA.ptr, dC.ptr)
+        gR, gA, gC = dR.numpy(), dA.numpy(), dC.numpy()
+    oR, oA, oC = oracle_lib.mo_vgl_adjoint(moved, w)
+    e = (rel(gR, oR), rel(gA, oA), rel(gC, oC))
+    say(f"moved points: rel err dR {e[0]:.2e} dalpha {e[1]:.2e} dC {e[2]:.2e}")
+    assert max(e) <= 1e-10, e
+
+
 def test_adjoint_water64_32_walkers(oracle_lib):
     """The differentiation workload: 64 waters, 32 walkers (20,480 points),
     w ~ N(0, 1)."""
